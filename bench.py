@@ -1,0 +1,231 @@
+"""Benchmark: pull + push + backward at 256^3 FP32, degrees 1-3, on one MI355X.
+
+One step = for each degree d in (1, 2, 3), on a 256^3 single-channel volume with a
+smooth deformation (identity + displacement of max 6 voxels, reflect boundary, ext=1):
+    f1   = pull(f, phi)                 (12 + 8C bytes/voxel algorithmic)
+    df   = push(delta, phi)             (12 + 8C)
+    dphi = sum_c delta_c grad pull(f)_c (24 + 8C)   -- the phi pullback
+i.e. 48 + 24C = 72 bytes/voxel of compulsory HBM traffic per degree. Inputs are resident
+in HBM before timing. value = 3 * 256^3 voxels / step time, in Gvoxel/s.
+
+The CPU baseline is the reference's own C (artifacts/C/pushpull.c, compiled from source
+into oracle/_ref by oracle/Makefile, single-threaded as shipped) timed on this host on a
+z-slab of the same workload; when that build is absent the project's FP32 restatement
+of it (oracle/_build) is timed instead.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as ct
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "pushpull.jl_amd"))
+
+N = 256
+DEGS = (1, 2, 3)
+HBM_PEAK_GBS = 8000.0
+BYTES = {"pull": 20, "push": 20, "backward": 32}  # per voxel at C = 1
+
+
+def _smooth_noise(torch, shape, sigma, gen, dev):
+    """Gaussian-smoothed standard-normal noise (separable, +-4 sigma, reflect)."""
+    x = torch.randn(shape, generator=gen, device=dev, dtype=torch.float32)
+    r = int(4 * sigma + 0.5)
+    k = torch.exp(-0.5 * (torch.arange(-r, r + 1, device=dev, dtype=torch.float32) / sigma) ** 2)
+    k = (k / k.sum()).view(1, 1, -1)
+    v = x.reshape((-1,) + tuple(shape[-3:]))
+    for ax in (1, 2, 3):
+        v = v.movedim(ax, -1)
+        s = v.shape
+        w = torch.nn.functional.pad(v.reshape(-1, 1, s[-1]), (r, r), mode="reflect")
+        v = torch.nn.functional.conv1d(w, k).reshape(s).movedim(-1, ax)
+    return v.reshape(shape).contiguous()
+
+
+def make_inputs(torch, dev, seed=1):
+    import pushpull as pp
+
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    d = (N, N, N)
+    f = _smooth_noise(torch, d, 2.0, gen, dev)
+    f /= f.std()
+    u = _smooth_noise(torch, (3,) + d, 4.0, gen, dev)
+    u *= 6.0 / u.abs().max()
+    phi = pp.id(dim=d, like=None)
+    phi = torch.from_numpy(phi).to(dev) + u
+    delta = _smooth_noise(torch, d, 2.0, gen, dev)
+    delta /= delta.std()
+    return f.contiguous(), phi.contiguous(), delta.contiguous()
+
+
+def cpu_baseline(f, phi, delta, budget_s=20.0):
+    """Reference CPU path on a z-slab of the workload, single thread."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    kind = "reference" if oracle.ref_available() else "port"
+    prec = "ref" if kind == "reference" else "f32"
+    if kind == "port" and not os.path.exists(oracle.ORACLE_SO):
+        return None
+
+    class _S:
+        def __init__(self, deg):
+            self.deg, self.bnd, self.ext = (deg,) * 3, [(1, 1, 1)], 1
+
+    fz = f.cpu().numpy()
+    dz_all = delta.cpu().numpy()
+    pz_all = phi.cpu().numpy()
+    nz = 8
+    total_vox = 0
+    t_total = 0.0
+    while True:
+        for deg in DEGS:
+            s = _S(deg)
+            z0 = 120
+            ph = np.ascontiguousarray(pz_all[:, z0:z0 + nz])
+            dl = np.ascontiguousarray(dz_all[z0:z0 + nz])
+            t0 = time.perf_counter()
+            oracle.pull(fz, ph, s, prec=prec)
+            oracle.push(dl, ph, fz.shape, s, prec=prec)
+            g = oracle.pull_grad(fz, ph, s, prec=prec)
+            _ = (g[0, 0] * dl[None]).astype(np.float32)  # channel contraction of the Julia rrule
+            t_total += time.perf_counter() - t0
+            total_vox += ph[0].size
+        if t_total > budget_s / 4 or nz >= N:
+            break
+        nz = min(N, nz * 2)
+    return {"value": total_vox / t_total / 1e9, "unit": "Gvoxel/s", "cores": 1, "kind": kind,
+            "sample": f"pull+push+pull_grad(+contraction) on 256x256 x{nz}-slice slabs of the same inputs, "
+                      f"degrees 1-3, {t_total:.1f} s of single-thread CPU work"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+
+    import torch
+
+    import pushpull as pp
+
+    dev = torch.device("cuda:0")
+    f, phi, delta = make_inputs(torch, dev)
+    n1 = N ** 3
+    outs = {
+        "f1": torch.empty_like(f),
+        "df": torch.empty_like(f),
+        "dphi": torch.empty_like(phi),
+    }
+    setts = {deg: pp.Settings(deg, 1, 1) for deg in DEGS}
+    from pushpull import ops
+
+    def step(events=None):
+        for deg in DEGS:
+            s = setts[deg]
+            marks = []
+            if events is not None:
+                marks.append(events.record())
+            outs["f1"] = ops.pull_raw(f, phi, s)
+            if events is not None:
+                marks.append(events.record())
+            ops._push_family("push", 1, delta, phi, (N, N, N), s, accumulate_into=None)
+            if events is not None:
+                marks.append(events.record())
+            ops.pull_backward(f, phi, delta, s)
+            if events is not None:
+                marks.append(events.record())
+                events.add(deg, marks)
+
+    class Ev:
+        def __init__(self):
+            self.rows = []
+
+        def record(self):
+            e = torch.cuda.Event(enable_timing=True)
+            e.record()
+            return e
+
+        def add(self, deg, m):
+            self.rows.append((deg, m))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = Ev()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record()
+    for _ in range(args.steps):
+        step(ev)
+    end.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = start.elapsed_time(end) / args.steps
+
+    per = {deg: {"pull": [], "push": [], "backward": []} for deg in DEGS}
+    for deg, m in ev.rows:
+        per[deg]["pull"].append(m[0].elapsed_time(m[1]))
+        per[deg]["push"].append(m[1].elapsed_time(m[2]))
+        per[deg]["backward"].append(m[2].elapsed_time(m[3]))
+    detail = {}
+    dominant = None
+    for deg in DEGS:
+        row = {}
+        tri = 0.0
+        for op in ("pull", "push", "backward"):
+            ms = float(np.mean(per[deg][op]))
+            tri += ms
+            gbs = BYTES[op] * n1 / (ms * 1e-3) / 1e9
+            row[op] = {"ms": round(ms, 4), "GB_s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+            if dominant is None or ms > dominant[2]:
+                dominant = (deg, op, ms, gbs)
+        row["triple_ms"] = round(tri, 4)
+        row["Gvoxel_s"] = round(n1 / (tri * 1e-3) / 1e9, 2)
+        row["triple_frac"] = round(72 * n1 / (tri * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        detail[f"deg{deg}"] = row
+
+    value = len(DEGS) * n1 / (ms_step * 1e-3) / 1e9
+    deg, op, ms, gbs = dominant
+    res = {
+        "metric": "Gvoxel/s pull+push+backward at 256^3 FP32 deg1-3",
+        "value": round(value, 3),
+        "unit": "Gvoxel/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (smoothed Gaussian noise image and upstream gradient; identity + smooth displacement, max 6 voxels)",
+        "config": {"workload": "pull+push+backward, 256^3 x 1 channel, degrees 1,2,3 (one triple each per step), "
+                               "reflect boundary, ext=1", "volume": [N, N, N], "channels": 1, "degrees": list(DEGS)},
+        "roofline": {"bound": "hbm", "kernel": f"{op} deg{deg}", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": BYTES[op] * n1, "avg_ms": round(ms, 4)},
+        "per_degree": detail,
+        "wall_s": round(wall, 3),
+    }
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(f, phi, delta)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,140 @@
+/*
+ * pushpull_hip.h -- C ABI of libpushpull_hip.so, the gfx950 (MI355X) implementation of
+ * the B-spline pull/push sampling path of PushPull.jl.
+ *
+ * Plain C types only: host or device pointers, sizes, int status codes; a HIP stream
+ * is passed as `void *` (NULL = the default stream).
+ *
+ * Two tiers.
+ *
+ * Tier A: symbol-compatible with the reference's native library (artifacts/C/pushpull.h:24-31
+ *   plus pullh, artifacts/C/pushpull.c:111). Host pointers, one (channel, batch) slice per
+ *   call, synchronous: upload, launch on the GPU, download. The reference's Julia wrappers
+ *   (src/pushpull_cpu.jl:45-49, :100-104, :140-144, :195-199, :251-255) can ccall these
+ *   unchanged once `pplib` (src/pushpull_init.jl:4) points at this library. Like the
+ *   reference they return nothing; a failure is reported through pp_last_error() and
+ *   pp_last_status(), and the output is left filled with NaN.
+ *
+ * Tier B: device-resident, stream-ordered, re-entrant. Arguments are device pointers
+ *   holding whole Julia-layout arrays (all channels and batches), described by a pp_desc
+ *   passed by pointer and copied into kernel arguments at launch (no module globals,
+ *   unlike the reference CUDA module's __constant__ settings, artifacts/C/pushpull.cu:16-24).
+ *   Every function returns 0 on success or a negative PP_E* code, with a message in
+ *   pp_last_error() (thread-local).
+ *
+ * Array layout (column-major, as Julia): image (X0,Y0,Z0,C,B); deformation phi
+ * (X1,Y1,Z1,3,B) holding 1-BASED voxel coordinates into the image grid; pull output
+ * (X1,Y1,Z1,C,B); pull_grad output (X1,Y1,Z1,3,C,B); pull_hess output (X1,Y1,Z1,3,3,C,B).
+ */
+#ifndef PUSHPULL_HIP_H
+#define PUSHPULL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* USIZE_t of the reference (artifacts/C/cuheader.h:14,22) */
+typedef unsigned long long pp_usize;
+
+/* ---------------------------------------------------------------- Tier A ---- */
+
+/* f1[n1] = pull of f0 (grid d0) at phi (3*n1, 1-based). NaN where the sample is invalid.
+ * Replaces artifacts/C/pushpull.c:55 `pull`. dp = degree+1 per axis, in [1,5]. */
+void pull(float *f1, const float *phi, const float *f0, const pp_usize *d0, pp_usize n1,
+          const int *bnd, const pp_usize *dp, int ext);
+
+/* g1[3*n1] = spatial gradient of the pulled image.  Replaces pushpull.c:78 `pullg`. */
+void pullg(float *g1, const float *phi, const float *f0, const pp_usize *d0, pp_usize n1,
+           const int *bnd, const pp_usize *dp, int ext);
+
+/* h1[9*n1] = Hessian, column-major 3x3 with symmetric copies. Replaces pushpull.c:111 `pullh`. */
+void pullh(float *h1, const float *phi, const float *f0, const pp_usize *d0, pp_usize n1,
+           const int *bnd, const pp_usize *dp, int ext);
+
+/* f0 (grid d0) += push of f1[n1] at phi: the adjoint of pull. Accumulates into the caller's
+ * f0 like the reference. Replaces pushpull.c:147 `push`. */
+void push(float *f0, const float *phi, const float *f1, const pp_usize *d0, pp_usize n1,
+          const int *bnd, const pp_usize *dp, int ext);
+
+/* f0 += push of gradients g1[3*n1]: the adjoint of pullg. Replaces pushpull.c:168 `pushg`
+ * with the semantics of artifacts/C/pushpull.cu:145-163 (the CPU version's inverted
+ * finiteness test at pushpull.c:185 is not reproduced). */
+void pushg(float *f0, const float *phi, const float *g1, const pp_usize *d0, pp_usize n1,
+           const int *bnd, const pp_usize *dp, int ext);
+
+/* ---------------------------------------------------------------- Tier B ---- */
+
+enum {
+  PP_OK = 0,
+  PP_EINVAL = -1,   /* bad argument (shape, degree, boundary code, null pointer) */
+  PP_EDEVICE = -2,  /* HIP runtime error */
+  PP_ENOMEM = -3,   /* allocation failed */
+  PP_ERANGE = -4    /* size beyond the supported range */
+};
+
+#define PP_MAX_CHAN_BND 64
+
+/* Settings of one call (src/pushpull_general.jl:15-60) plus the array geometry. */
+typedef struct pp_desc {
+  int64_t d0[3];   /* image grid (X,Y,Z): sampled by pull, scattered into by push */
+  int64_t d1[3];   /* grid of phi (X,Y,Z) */
+  int32_t nchan;   /* C */
+  int32_t nbatch;  /* B */
+  int32_t deg[3];  /* interpolation degree per axis, 0..4 */
+  int32_t ext;     /* 0: samples outside [-0.01, d-0.99] are invalid; 1: extrapolate */
+  int32_t nbnd;    /* 1 (shared) or nchan (per channel) */
+  int32_t bnd[3 * PP_MAX_CHAN_BND]; /* boundary codes per axis (0 circulant, 1 reflect, 2 antisymmetric) */
+} pp_desc;
+
+/* Julia: pull(f0, phi, sett)            src/pushpull_cpu.jl:17      out (d1,C,B) */
+int pp_pull(const pp_desc *d, float *f1, const float *phi, const float *f0, void *stream);
+/* Julia: pull_grad(f0, phi, sett)       src/pushpull_cpu.jl:69      out (d1,3,C,B) */
+int pp_pull_grad(const pp_desc *d, float *g1, const float *phi, const float *f0, void *stream);
+/* Julia: pull_hess(f0, phi, sett)       src/pushpull_cpu.jl:111     out (d1,3,3,C,B) */
+int pp_pull_hess(const pp_desc *d, float *h1, const float *phi, const float *f0, void *stream);
+/* Julia: push(f1, phi, d0, sett)        src/pushpull_cpu.jl:165     out (d0,C,B);
+ * accumulate=0 zeroes f0 first (in-stream), 1 adds to its contents. */
+int pp_push(const pp_desc *d, float *f0, const float *phi, const float *f1, int accumulate, void *stream);
+/* Julia: push_grad(g1, phi, d0, sett)   src/pushpull_cpu.jl:220     in (d1,3,C,B) */
+int pp_push_grad(const pp_desc *d, float *f0, const float *phi, const float *g1, int accumulate, void *stream);
+
+/* Pullback of pull wrt phi (src/pushpull_pb.jl:13-17): dphi(d1,3,B) = sum_c delta_c * pull_grad(f0)_c,
+ * NaN where the sample is invalid. Also the phi-pullback of push (src/pushpull_pb.jl:144-147)
+ * with (f0, delta) := (upstream gradient, pushed image). */
+int pp_pull_backward(const pp_desc *d, float *dphi, const float *phi, const float *f0, const float *delta,
+                     void *stream);
+/* Pullback of pull_grad wrt phi (src/pushpull_pb.jl:128-132):
+ * dphi_b = sum_c sum_a delta(a,c) * pull_hess(f0)(a,b,c); delta is (d1,3,C,B). */
+int pp_pull_grad_backward(const pp_desc *d, float *dphi, const float *phi, const float *f0, const float *delta,
+                          void *stream);
+
+/* Fused objective and both gradients for L = 0.5*||pull(f0,phi) - g||^2 (no counterpart in
+ * the reference; it composes pull, push and pp_pull_backward):
+ *   dphi (d1,3,B) = sum_c r_c grad pull(f0)_c (NaN where invalid), r = pull(f0,phi) - g,
+ *   df0  (d0,C,B) = push(r) (zeroed first unless accumulate), loss[0] (double, device) =
+ *   0.5 * sum of r^2 over finite r (zeroed first unless accumulate). */
+int pp_objective_grad(const pp_desc *d, float *dphi, float *df0, double *loss, const float *phi,
+                      const float *f0, const float *g, int accumulate, void *stream);
+
+/* ---- device memory helpers, so a Julia caller needs no GPU package ---------- */
+int pp_malloc(void **p, size_t bytes);
+int pp_free(void *p);
+int pp_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int pp_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int pp_memset_zero(float *p, size_t count, void *stream);
+int pp_stream_sync(void *stream);
+int pp_device_sync(void);
+
+/* ---- diagnostics ---------------------------------------------------------- */
+const char *pp_last_error(void);  /* message of the last failure on this thread */
+int pp_last_status(void);          /* status of the last call on this thread */
+const char *pp_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PUSHPULL_HIP_H */

@@ -1,0 +1,215 @@
+"""CPU oracle for the pull/push path (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of bench.py
+import this module, and only as the checker. The product path (``pushpull``) never
+imports it.
+
+Two libraries sit behind it:
+
+* ``_build/libpp_oracle.so`` -- this project's C restatement (pp_oracle.c), in FP32
+  (``prec="f32"``, bit-exact against the reference C) and FP64 (``prec="f64"``, masks
+  and knots in FP32, arithmetic in double).
+* ``_ref/libpushpull_ref.so`` -- the reference's own ``artifacts/C/pushpull.c``
+  compiled from source by ``make ref`` (present only where that tree was available).
+
+Array convention (identical memory to the Julia arrays of src/pushpull_cpu.jl):
+images are C-ordered ``(B, C, Z, Y, X)`` (Julia ``(X, Y, Z, C, B)``); deformations are
+``(B, 3, Z, Y, X)`` holding 1-based voxel coordinates, component 0 along X.
+The per-(batch, channel) slice loop mirrors src/pushpull_cpu.jl:42-51 (pull),
+:95-106 (pull_grad), :137-146 (pull_hess), :192-201 (push), :248-257 (push_grad).
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(_HERE, "_build", "libpp_oracle.so")
+REF_SO = os.path.join(_HERE, "_ref", "libpushpull_ref.so")
+
+_P = ct.c_void_p
+_SIG = [_P, _P, _P, _P, ct.c_ulonglong, _P, _P, ct.c_int]
+
+_lib = None
+_ref = None
+
+
+def lib():
+    """The restatement library (built by ``make -C oracle``)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"oracle library missing: {ORACLE_SO} (run make -C oracle)")
+        L = ct.CDLL(ORACLE_SO)
+        for p in ("f32", "f64"):
+            for op in ("pull", "pullg", "pullh", "push", "pushg"):
+                fn = getattr(L, f"ppo_{op}_{p}")
+                fn.argtypes = _SIG
+                fn.restype = None
+        L.ppo_knot_export.argtypes = [ct.c_float, ct.c_int]
+        L.ppo_knot_export.restype = ct.c_longlong
+        L.ppo_fold_export.argtypes = [ct.c_int, ct.c_longlong, ct.c_ulonglong, _P, _P]
+        L.ppo_fold_export.restype = None
+        L.ppo_weights_f64.argtypes = [ct.c_int, ct.c_double, _P, _P, _P]
+        L.ppo_weights_f32.argtypes = [ct.c_int, ct.c_float, _P, _P, _P]
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    """The reference C compiled from source (``make -C oracle ref``)."""
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise RuntimeError(f"reference build missing: {REF_SO}")
+        L = ct.CDLL(REF_SO)
+        for op in ("pull", "pullg", "pullh", "push", "pushg"):
+            fn = getattr(L, op)
+            fn.argtypes = _SIG
+            fn.restype = None
+        _ref = L
+    return _ref
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ct.c_void_p)
+
+
+def _fn(op: str, prec: str):
+    if prec == "ref":
+        return getattr(ref(), op)
+    return getattr(lib(), f"ppo_{op}_{prec}")
+
+
+def _dtype(prec):
+    return np.float64 if prec == "f64" else np.float32
+
+
+def _as5(a, lead):
+    """Promote a Julia-style 3..5-D array (reversed axes) to (B, C, Z, Y, X)."""
+    a = np.asarray(a)
+    while a.ndim < 5:
+        a = a[None]
+    return a
+
+
+def _bnd_for(sett, c):
+    b = sett.bnd[0] if len(sett.bnd) == 1 else sett.bnd[c]
+    return np.ascontiguousarray(b, dtype=np.int32)
+
+
+def _dp(sett):
+    return np.ascontiguousarray([d + 1 for d in sett.deg], dtype=np.uint64)
+
+
+def pull(f0, phi, sett, prec="f32"):
+    """pull (src/pushpull_cpu.jl:17-53): f0 (B,C,Z0,Y0,X0), phi (B,3,Z1,Y1,X1)."""
+    return _pull_family("pull", 1, f0, phi, sett, prec)
+
+
+def pull_grad(f0, phi, sett, prec="f32"):
+    """pull_grad (src/pushpull_cpu.jl:69-108): out (B,C,3,Z1,Y1,X1)."""
+    return _pull_family("pullg", 3, f0, phi, sett, prec)
+
+
+def pull_hess(f0, phi, sett, prec="f32"):
+    """pull_hess (src/pushpull_cpu.jl:111-148): out (B,C,3,3,Z1,Y1,X1), column-major 3x3."""
+    return _pull_family("pullh", 9, f0, phi, sett, prec)
+
+
+def _pull_family(op, ncomp, f0, phi, sett, prec):
+    dt = _dtype(prec)
+    f0 = np.ascontiguousarray(_as5(f0, 5), dtype=dt)
+    phi = np.ascontiguousarray(_as5(phi, 5), dtype=np.float32)
+    B, C = f0.shape[:2]
+    assert phi.shape[1] == 3 and phi.shape[0] == B
+    assert len(sett.bnd) in (1, C)
+    d0 = np.ascontiguousarray(f0.shape[:1:-1], dtype=np.uint64)  # (X0, Y0, Z0)
+    d1 = phi.shape[2:]
+    n1 = int(np.prod(d1))
+    out = np.zeros((B, C, ncomp) + d1, dtype=dt)
+    dp = _dp(sett)
+    fn = _fn(op, prec)
+    for b in range(B):
+        for c in range(C):
+            bnd = _bnd_for(sett, c)
+            o = out[b, c]
+            fn(_ptr(o), _ptr(phi[b]), _ptr(f0[b, c]), _ptr(d0), n1, _ptr(bnd), _ptr(dp), int(sett.ext))
+    if ncomp == 1:
+        return out[:, :, 0]
+    if ncomp == 9:
+        return out.reshape((B, C, 3, 3) + d1)
+    return out
+
+
+def push(f1, phi, dim, sett, prec="f32"):
+    """push (src/pushpull_cpu.jl:165-203): f1 (B,C,Z1,Y1,X1) -> (B,C)+dim, dim=(Z0,Y0,X0)."""
+    return _push_family("push", 1, f1, phi, dim, sett, prec)
+
+
+def push_grad(g1, phi, dim, sett, prec="f32"):
+    """push_grad (src/pushpull_cpu.jl:220-259) with the adjoint semantics of
+    ext/pushpull_gpu.jl / pushpull.cu:145-163: g1 (B,C,3,Z1,Y1,X1)."""
+    return _push_family("pushg", 3, g1, phi, dim, sett, prec)
+
+
+def _push_family(op, ncomp, f1, phi, dim, sett, prec):
+    dt = _dtype(prec)
+    f1 = np.asarray(f1)
+    if ncomp == 3:
+        while f1.ndim < 6:
+            f1 = f1[None]
+    else:
+        f1 = _as5(f1, 5)
+    f1 = np.ascontiguousarray(f1, dtype=dt)
+    phi = np.ascontiguousarray(_as5(phi, 5), dtype=np.float32)
+    B, C = f1.shape[:2]
+    assert phi.shape[0] == B and phi.shape[1] == 3
+    assert tuple(f1.shape[-3:]) == tuple(phi.shape[2:])
+    dim = tuple(int(v) for v in dim)
+    d0 = np.ascontiguousarray(dim[::-1], dtype=np.uint64)
+    n1 = int(np.prod(phi.shape[2:]))
+    out = np.zeros((B, C) + dim, dtype=dt)
+    dp = _dp(sett)
+    fn = _fn(op, prec)
+    for b in range(B):
+        for c in range(C):
+            bnd = _bnd_for(sett, c)
+            fn(_ptr(out[b, c]), _ptr(phi[b]), _ptr(f1[b, c]), _ptr(d0), n1, _ptr(bnd), _ptr(dp), int(sett.ext))
+    return out
+
+
+def knot(x: float, dp: int) -> int:
+    return int(lib().ppo_knot_export(ct.c_float(x), dp))
+
+
+def fold(code: int, i: int, d: int):
+    b = ct.c_ulonglong()
+    s = ct.c_int()
+    lib().ppo_fold_export(code, i, d, ct.byref(b), ct.byref(s))
+    return int(b.value), int(s.value)
+
+
+def weights(dp: int, t: float, prec="f64"):
+    dt = _dtype(prec)
+    w = np.zeros(5, dt)
+    dw = np.zeros(5, dt)
+    hw = np.zeros(5, dt)
+    fn = lib().ppo_weights_f64 if prec == "f64" else lib().ppo_weights_f32
+    fn(dp, t, _ptr(w), _ptr(dw), _ptr(hw))
+    return w[:dp], dw[:dp], hw[:dp]
+
+
+def backward_phi(f0, phi, delta, sett, prec="f64"):
+    """dL/dphi for L with upstream gradient delta on pull(f0, phi): the channel
+    contraction sum_c delta_c * pull_grad(f0, phi)_c of src/pushpull_pb.jl:13-17."""
+    g = pull_grad(f0, phi, sett, prec)  # (B,C,3,...)
+    d = np.asarray(delta, dtype=g.dtype)
+    d = _as5(d, 5)
+    return (d[:, :, None] * g).sum(axis=1)  # (B,3,...)

@@ -1,0 +1,403 @@
+// pp_abi.cpp -- extern "C" entry points of libpushpull_hip.so (declared in
+// include/pushpull_hip.h): argument checking, stride set-up, Tier A host staging.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "pp_kernels.hpp"
+#include "pushpull_hip.h"
+
+namespace {
+
+thread_local std::string t_err;
+thread_local int t_status = 0;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  t_err = buf;
+  t_status = code;
+  return code;
+}
+
+int ok() {
+  t_status = PP_OK;
+  return PP_OK;
+}
+
+int hip_fail(hipError_t e, const char* where) {
+  return fail(PP_EDEVICE, "%s: %s", where, hipGetErrorString(e));
+}
+
+constexpr long long kMaxVox = (1LL << 31) - 1;  // per-volume voxel count (32-bit in-volume offsets)
+constexpr long long kMaxDim = 1LL << 29;        // per-axis size (2*d must fit an int)
+constexpr int kBlock = 256;
+
+struct Geo {
+  long long n0, n1;
+};
+
+// Validate a descriptor (src/pushpull_general.jl:26-57 normalises the same fields;
+// here out-of-range values are rejected, not clamped).
+int check_desc(const pp_desc* d, Geo& g) {
+  if (!d) return fail(PP_EINVAL, "null descriptor");
+  g.n0 = 1;
+  g.n1 = 1;
+  for (int a = 0; a < 3; ++a) {
+    if (d->d0[a] < 1 || d->d1[a] < 1) return fail(PP_EINVAL, "grid sizes must be >= 1 (axis %d)", a);
+    if (d->d0[a] > kMaxDim || d->d1[a] > kMaxDim) return fail(PP_ERANGE, "axis %d larger than 2^29", a);
+    g.n0 *= d->d0[a];
+    g.n1 *= d->d1[a];
+    if (d->deg[a] < 0 || d->deg[a] > 4) return fail(PP_EINVAL, "degree %d on axis %d outside 0..4", d->deg[a], a);
+  }
+  if (g.n0 > kMaxVox || g.n1 > kMaxVox) return fail(PP_ERANGE, "a volume holds more than 2^31-1 voxels");
+  if (d->nchan < 1) return fail(PP_EINVAL, "nchan must be >= 1");
+  if (d->nbatch < 1 || d->nbatch > 65535) return fail(PP_EINVAL, "nbatch must be in 1..65535");
+  if (!(d->nbnd == 1 || d->nbnd == d->nchan)) return fail(PP_EINVAL, "nbnd must be 1 or nchan");
+  if (d->nbnd > PP_MAX_CHAN_BND) return fail(PP_EINVAL, "more than %d per-channel boundary triples", PP_MAX_CHAN_BND);
+  for (int i = 0; i < 3 * d->nbnd; ++i)
+    if (d->bnd[i] < 0 || d->bnd[i] > 2) return fail(PP_EINVAL, "boundary code %d not in {0,1,2}", d->bnd[i]);
+  return PP_OK;
+}
+
+void base_args(const pp_desc* d, pp::KArgs& a, const Geo& g) {
+  std::memset(&a, 0, sizeof a);
+  a.n1 = (int)g.n1;
+  a.d0x = (int)d->d0[0];
+  a.d0y = (int)d->d0[1];
+  a.d0z = (int)d->d0[2];
+  a.dpx = d->deg[0] + 1;
+  a.dpy = d->deg[1] + 1;
+  a.dpz = d->deg[2] + 1;
+  a.ext = d->ext != 0;
+  a.lx = (float)d->d0[0] - 0.99f;
+  a.ly = (float)d->d0[1] - 0.99f;
+  a.lz = (float)d->d0[2] - 0.99f;
+  a.phi_bs = 3 * g.n1;
+}
+
+bool uniform_bnd(const pp_desc* d) {
+  for (int c = 1; c < d->nbnd; ++c)
+    for (int q = 0; q < 3; ++q)
+      if (d->bnd[3 * c + q] != d->bnd[q]) return false;
+  return true;
+}
+
+// Launch `op` over all channels: one launch when the boundary codes are shared, else
+// one per channel (pointers offset by the channel strides; channel-reducing outputs
+// accumulate after the first launch).
+int run(pp::Op op, const pp_desc* d, const Geo& g, pp::KArgs a, hipStream_t s) {
+  const bool reduce = (op == pp::OP_PULL_BWD || op == pp::OP_PULLG_BWD || op == pp::OP_OBJGRAD);
+  if (uniform_bnd(d)) {
+    a.bx = d->bnd[0];
+    a.by = d->bnd[1];
+    a.bz = d->bnd[2];
+    a.nchan = d->nchan;
+    hipError_t e = pp::launch_op(op, a, d->nbatch, s, kBlock);
+    return e == hipSuccess ? ok() : hip_fail(e, "kernel launch");
+  }
+  for (int c = 0; c < d->nchan; ++c) {
+    pp::KArgs b = a;
+    b.bx = d->bnd[3 * c];
+    b.by = d->bnd[3 * c + 1];
+    b.bz = d->bnd[3 * c + 2];
+    b.nchan = 1;
+    b.f = a.f + c * a.f_cs;
+    if (a.aux) b.aux = a.aux + c * a.aux_cs;
+    if (reduce) {
+      b.accum = c > 0 ? 1 : a.accum;
+      if (a.out2) b.out2 = a.out2 + c * a.out2_cs;
+    } else {
+      b.out = a.out + c * a.out_cs;
+    }
+    hipError_t e = pp::launch_op(op, b, d->nbatch, s, kBlock);
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+  }
+  (void)g;
+  return ok();
+}
+
+hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+int pull_family(pp::Op op, int ncomp, const pp_desc* d, float* out, const float* phi, const float* f0, void* s) {
+  Geo g;
+  if (int r = check_desc(d, g)) return r;
+  if (!out || !phi || !f0) return fail(PP_EINVAL, "null array");
+  pp::KArgs a;
+  base_args(d, a, g);
+  a.f = f0;
+  a.f_cs = g.n0;
+  a.f_bs = g.n0 * d->nchan;
+  a.phi = phi;
+  a.out = out;
+  a.out_cs = ncomp * g.n1;
+  a.out_bs = ncomp * g.n1 * d->nchan;
+  return run(op, d, g, a, S(s));
+}
+
+int push_family(pp::Op op, int ncomp, const pp_desc* d, float* f0, const float* phi, const float* f1, int accumulate,
+                void* s) {
+  Geo g;
+  if (int r = check_desc(d, g)) return r;
+  if (!f0 || !phi || !f1) return fail(PP_EINVAL, "null array");
+  if (!accumulate) {
+    hipError_t e = pp::launch_fill(f0, g.n0 * d->nchan * d->nbatch, 0.0f, S(s));
+    if (e != hipSuccess) return hip_fail(e, "zero fill");
+  }
+  pp::KArgs a;
+  base_args(d, a, g);
+  a.f = f1;
+  a.f_cs = ncomp * g.n1;
+  a.f_bs = ncomp * g.n1 * d->nchan;
+  a.phi = phi;
+  a.out = f0;
+  a.out_cs = g.n0;
+  a.out_bs = g.n0 * d->nchan;
+  return run(op, d, g, a, S(s));
+}
+
+int backward_family(pp::Op op, int ncomp, const pp_desc* d, float* dphi, const float* phi, const float* f0,
+                    const float* delta, void* s) {
+  Geo g;
+  if (int r = check_desc(d, g)) return r;
+  if (!dphi || !phi || !f0 || !delta) return fail(PP_EINVAL, "null array");
+  pp::KArgs a;
+  base_args(d, a, g);
+  a.f = f0;
+  a.f_cs = g.n0;
+  a.f_bs = g.n0 * d->nchan;
+  a.phi = phi;
+  a.aux = delta;
+  a.aux_cs = ncomp * g.n1;
+  a.aux_bs = ncomp * g.n1 * d->nchan;
+  a.out = dphi;
+  a.out_bs = 3 * g.n1;
+  return run(op, d, g, a, S(s));
+}
+
+// ---- Tier A: host pointers, one slice, synchronous ---------------------------
+struct Staging {
+  std::mutex mu;
+  void* buf = nullptr;
+  size_t cap = 0;
+  int dev = -1;
+};
+Staging g_stage[64];
+
+// Checks a Tier A call and fills a one-slice descriptor view.
+int tierA_args(const pp_usize* d0, pp_usize n1, const int* bnd, const pp_usize* dp, int ext, pp::KArgs& a,
+               long long& n0) {
+  if (!d0 || !bnd || !dp) return fail(PP_EINVAL, "null d0/bnd/dp");
+  n0 = 1;
+  for (int q = 0; q < 3; ++q) {
+    if (d0[q] < 1 || d0[q] > (pp_usize)kMaxDim) return fail(PP_EINVAL, "d0[%d] out of range", q);
+    if (dp[q] < 1 || dp[q] > 5) return fail(PP_EINVAL, "dp[%d]=%llu outside 1..5", q, (unsigned long long)dp[q]);
+    n0 *= (long long)d0[q];
+  }
+  if (n0 > kMaxVox || n1 > (pp_usize)kMaxVox) return fail(PP_ERANGE, "slice larger than 2^31-1 voxels");
+  std::memset(&a, 0, sizeof a);
+  a.n1 = (int)n1;
+  a.d0x = (int)d0[0];
+  a.d0y = (int)d0[1];
+  a.d0z = (int)d0[2];
+  a.dpx = (int)dp[0];
+  a.dpy = (int)dp[1];
+  a.dpz = (int)dp[2];
+  a.bx = bnd[0];
+  a.by = bnd[1];
+  a.bz = bnd[2];
+  a.ext = ext != 0;
+  a.lx = (float)d0[0] - 0.99f;
+  a.ly = (float)d0[1] - 0.99f;
+  a.lz = (float)d0[2] - 0.99f;
+  a.nchan = 1;
+  a.phi_bs = 3 * (long long)n1;
+  return PP_OK;
+}
+
+// in_n: floats of the gathered/pushed input; out_n: floats of the output; the output is
+// uploaded first when `upload_out` (push accumulates into the caller's array).
+int tierA(pp::Op op, float* out, long long out_n, bool upload_out, const float* phi, const float* in, long long in_n,
+          const pp_usize* d0, pp_usize n1, const int* bnd, const pp_usize* dp, int ext) {
+  pp::KArgs a;
+  long long n0;
+  if (int r = tierA_args(d0, n1, bnd, dp, ext, a, n0)) return r;
+  if (n1 == 0) return ok();
+  if (!out || !phi || !in) return fail(PP_EINVAL, "null array");
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  Staging& st = g_stage[dev & 63];
+  std::lock_guard<std::mutex> lock(st.mu);
+  const size_t phi_b = 3 * (size_t)n1 * 4, in_b = (size_t)in_n * 4, out_b = (size_t)out_n * 4;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t need = al(phi_b) + al(in_b) + al(out_b);
+  if (st.cap < need || st.dev != dev) {
+    if (st.buf) (void)hipFree(st.buf);
+    st.buf = nullptr;
+    st.cap = 0;
+    e = hipMalloc(&st.buf, need);
+    if (e != hipSuccess) return fail(PP_ENOMEM, "hipMalloc(%zu): %s", need, hipGetErrorString(e));
+    st.cap = need;
+    st.dev = dev;
+  }
+  char* base = static_cast<char*>(st.buf);
+  float* dphi = reinterpret_cast<float*>(base);
+  float* din = reinterpret_cast<float*>(base + al(phi_b));
+  float* dout = reinterpret_cast<float*>(base + al(phi_b) + al(in_b));
+  hipStream_t s = nullptr;
+  if ((e = hipMemcpyAsync(dphi, phi, phi_b, hipMemcpyHostToDevice, s)) != hipSuccess) return hip_fail(e, "upload phi");
+  if ((e = hipMemcpyAsync(din, in, in_b, hipMemcpyHostToDevice, s)) != hipSuccess) return hip_fail(e, "upload input");
+  if (upload_out) {
+    if ((e = hipMemcpyAsync(dout, out, out_b, hipMemcpyHostToDevice, s)) != hipSuccess) return hip_fail(e, "upload output");
+  }
+  a.phi = dphi;
+  a.f = din;
+  a.out = dout;
+  // one channel, one batch: channel/batch strides are never applied
+  a.f_cs = in_n;
+  a.out_cs = out_n;
+  if ((e = pp::launch_op(op, a, 1, s, kBlock)) != hipSuccess) return hip_fail(e, "kernel launch");
+  if ((e = hipMemcpyAsync(out, dout, out_b, hipMemcpyDeviceToHost, s)) != hipSuccess) return hip_fail(e, "download");
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "synchronize");
+  return ok();
+}
+
+void nan_fill(float* p, long long n) {
+  if (!p) return;
+  for (long long i = 0; i < n; ++i) p[i] = NAN;
+}
+
+long long prod3(const pp_usize* d) { return d ? (long long)(d[0] * d[1] * d[2]) : 0; }
+
+}  // namespace
+
+extern "C" {
+
+// ---- Tier A --------------------------------------------------------------------
+void pull(float* f1, const float* phi, const float* f0, const pp_usize* d0, pp_usize n1, const int* bnd,
+          const pp_usize* dp, int ext) {
+  if (tierA(pp::OP_PULL, f1, (long long)n1, false, phi, f0, prod3(d0), d0, n1, bnd, dp, ext))
+    nan_fill(f1, (long long)n1);
+}
+
+void pullg(float* g1, const float* phi, const float* f0, const pp_usize* d0, pp_usize n1, const int* bnd,
+           const pp_usize* dp, int ext) {
+  if (tierA(pp::OP_PULLG, g1, 3 * (long long)n1, false, phi, f0, prod3(d0), d0, n1, bnd, dp, ext))
+    nan_fill(g1, 3 * (long long)n1);
+}
+
+void pullh(float* h1, const float* phi, const float* f0, const pp_usize* d0, pp_usize n1, const int* bnd,
+           const pp_usize* dp, int ext) {
+  if (tierA(pp::OP_PULLH, h1, 9 * (long long)n1, false, phi, f0, prod3(d0), d0, n1, bnd, dp, ext))
+    nan_fill(h1, 9 * (long long)n1);
+}
+
+void push(float* f0, const float* phi, const float* f1, const pp_usize* d0, pp_usize n1, const int* bnd,
+          const pp_usize* dp, int ext) {
+  (void)tierA(pp::OP_PUSH, f0, prod3(d0), true, phi, f1, (long long)n1, d0, n1, bnd, dp, ext);
+}
+
+void pushg(float* f0, const float* phi, const float* g1, const pp_usize* d0, pp_usize n1, const int* bnd,
+           const pp_usize* dp, int ext) {
+  (void)tierA(pp::OP_PUSHG, f0, prod3(d0), true, phi, g1, 3 * (long long)n1, d0, n1, bnd, dp, ext);
+}
+
+// ---- Tier B --------------------------------------------------------------------
+int pp_pull(const pp_desc* d, float* f1, const float* phi, const float* f0, void* s) {
+  return pull_family(pp::OP_PULL, 1, d, f1, phi, f0, s);
+}
+int pp_pull_grad(const pp_desc* d, float* g1, const float* phi, const float* f0, void* s) {
+  return pull_family(pp::OP_PULLG, 3, d, g1, phi, f0, s);
+}
+int pp_pull_hess(const pp_desc* d, float* h1, const float* phi, const float* f0, void* s) {
+  return pull_family(pp::OP_PULLH, 9, d, h1, phi, f0, s);
+}
+int pp_push(const pp_desc* d, float* f0, const float* phi, const float* f1, int accumulate, void* s) {
+  return push_family(pp::OP_PUSH, 1, d, f0, phi, f1, accumulate, s);
+}
+int pp_push_grad(const pp_desc* d, float* f0, const float* phi, const float* g1, int accumulate, void* s) {
+  return push_family(pp::OP_PUSHG, 3, d, f0, phi, g1, accumulate, s);
+}
+int pp_pull_backward(const pp_desc* d, float* dphi, const float* phi, const float* f0, const float* delta, void* s) {
+  return backward_family(pp::OP_PULL_BWD, 1, d, dphi, phi, f0, delta, s);
+}
+int pp_pull_grad_backward(const pp_desc* d, float* dphi, const float* phi, const float* f0, const float* delta,
+                          void* s) {
+  return backward_family(pp::OP_PULLG_BWD, 3, d, dphi, phi, f0, delta, s);
+}
+
+int pp_objective_grad(const pp_desc* d, float* dphi, float* df0, double* loss, const float* phi, const float* f0,
+                      const float* gt, int accumulate, void* s) {
+  Geo g;
+  if (int r = check_desc(d, g)) return r;
+  if (!dphi || !df0 || !loss || !phi || !f0 || !gt) return fail(PP_EINVAL, "null array");
+  if (!accumulate) {
+    hipError_t e = pp::launch_fill(df0, g.n0 * d->nchan * d->nbatch, 0.0f, S(s));
+    if (e == hipSuccess) e = pp::launch_fill_d(loss, 1, 0.0, S(s));
+    if (e != hipSuccess) return hip_fail(e, "zero fill");
+  }
+  pp::KArgs a;
+  base_args(d, a, g);
+  a.f = f0;
+  a.f_cs = g.n0;
+  a.f_bs = g.n0 * d->nchan;
+  a.phi = phi;
+  a.aux = gt;
+  a.aux_cs = g.n1;
+  a.aux_bs = g.n1 * d->nchan;
+  a.out = dphi;
+  a.out_bs = 3 * g.n1;
+  a.out2 = df0;
+  a.out2_cs = g.n0;
+  a.out2_bs = g.n0 * d->nchan;
+  a.red = loss;
+  return run(pp::OP_OBJGRAD, d, g, a, S(s));
+}
+
+int pp_malloc(void** p, size_t bytes) {
+  if (!p) return fail(PP_EINVAL, "null pointer");
+  hipError_t e = hipMalloc(p, bytes);
+  return e == hipSuccess ? ok() : fail(PP_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+}
+int pp_free(void* p) {
+  hipError_t e = hipFree(p);
+  return e == hipSuccess ? ok() : hip_fail(e, "hipFree");
+}
+int pp_memcpy_h2d(void* dst, const void* src, size_t bytes, void* s) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S(s));
+  if (e == hipSuccess) e = hipStreamSynchronize(S(s));
+  return e == hipSuccess ? ok() : hip_fail(e, "memcpy h2d");
+}
+int pp_memcpy_d2h(void* dst, const void* src, size_t bytes, void* s) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, S(s));
+  if (e == hipSuccess) e = hipStreamSynchronize(S(s));
+  return e == hipSuccess ? ok() : hip_fail(e, "memcpy d2h");
+}
+int pp_memset_zero(float* p, size_t count, void* s) {
+  hipError_t e = pp::launch_fill(p, (long long)count, 0.0f, S(s));
+  return e == hipSuccess ? ok() : hip_fail(e, "zero fill");
+}
+int pp_stream_sync(void* s) {
+  hipError_t e = hipStreamSynchronize(S(s));
+  return e == hipSuccess ? ok() : hip_fail(e, "hipStreamSynchronize");
+}
+int pp_device_sync(void) {
+  hipError_t e = hipDeviceSynchronize();
+  return e == hipSuccess ? ok() : hip_fail(e, "hipDeviceSynchronize");
+}
+
+const char* pp_last_error(void) { return t_err.c_str(); }
+int pp_last_status(void) { return t_status; }
+const char* pp_version(void) { return "pushpull_hip 0.1.0 (gfx950)"; }
+
+}  // extern "C"

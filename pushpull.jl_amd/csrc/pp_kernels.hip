@@ -1,0 +1,434 @@
+// pp_kernels.hip -- gfx950 kernels of the pull/push path.
+//
+// One work-item per voxel of the phi grid. Per sample the three axes' weights and
+// folded offsets live in registers (pp_bspline.hpp) and are reused across all channels
+// of the launch. Gathers contract separably: each row of dp taps along x is reduced
+// with the x weights first, then weighted by w_y*w_z. Scatters use the hardware float
+// atomic (global_atomic_add_f32). Block ids are remapped so that each XCD's L2 sees a
+// contiguous range of voxels (neighbouring rows share most of their source footprint).
+#include "pp_bspline.hpp"
+#include "pp_kernels.hpp"
+
+namespace pp {
+
+constexpr int kXcds = 8;
+
+// Round-robin dispatch puts block b on XCD b % 8; give XCD k a contiguous chunk.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  const int per = nb / kXcds, rem = nb % kXcds;
+  const int x = b % kXcds, i = b / kXcds;
+  return (x < rem) ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
+}
+
+template <int DP, int NEED>
+__device__ __forceinline__ bool setup(const KArgs& a, const float* phi, int v, Axis<DP>& ax, Axis<DP>& ay,
+                                      Axis<DP>& az) {
+  const float x = phi[v] - 1.0f;
+  const float y = phi[v + a.n1] - 1.0f;
+  const float z = phi[v + 2 * (long long)a.n1] - 1.0f;
+  if (!voxok(x, y, z, a.ext, a.lx, a.ly, a.lz)) return false;
+  make_axis<DP, NEED>(x, a.dpx, a.d0x, 1, a.bx, ax);
+  make_axis<DP, NEED>(y, a.dpy, a.d0y, a.d0x, a.by, ay);
+  make_axis<DP, NEED>(z, a.dpz, a.d0z, a.d0x * a.d0y, a.bz, az);
+  return true;
+}
+
+#define PP_TAP(DP, ax, i) if (DP > 0 || (i) < (ax).n)
+
+// Value: sum_ijk wx_i wy_j wz_k f[...]
+template <int DP>
+__device__ __forceinline__ float gather_val(const float* __restrict__ f, const Axis<DP>& ax, const Axis<DP>& ay,
+                                            const Axis<DP>& az) {
+  constexpr int NP = Axis<DP>::NP;
+  float acc = 0.f;
+#pragma unroll
+  for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
+#pragma unroll
+      for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
+          const float* row = f + az.off[k] + ay.off[j];
+          float r = 0.f;
+#pragma unroll
+          for (int i = 0; i < NP; ++i) PP_TAP(DP, ax, i) r += ax.w[i] * row[ax.off[i]];
+          acc += (az.w[k] * ay.w[j]) * r;
+        }
+    }
+  return acc;
+}
+
+// Gradient: (d/dx, d/dy, d/dz)
+template <int DP>
+__device__ __forceinline__ void gather_grad(const float* __restrict__ f, const Axis<DP>& ax, const Axis<DP>& ay,
+                                            const Axis<DP>& az, float& gx, float& gy, float& gz) {
+  constexpr int NP = Axis<DP>::NP;
+  gx = gy = gz = 0.f;
+#pragma unroll
+  for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
+#pragma unroll
+      for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
+          const float* row = f + az.off[k] + ay.off[j];
+          float r = 0.f, rd = 0.f;
+#pragma unroll
+          for (int i = 0; i < NP; ++i) PP_TAP(DP, ax, i) {
+              const float v = row[ax.off[i]];
+              r += ax.w[i] * v;
+              rd += ax.dw[i] * v;
+            }
+          gx += (az.w[k] * ay.w[j]) * rd;
+          gy += (az.w[k] * ay.dw[j]) * r;
+          gz += (az.dw[k] * ay.w[j]) * r;
+        }
+    }
+}
+
+// Value and gradient from one footprint
+template <int DP>
+__device__ __forceinline__ void gather_val_grad(const float* __restrict__ f, const Axis<DP>& ax, const Axis<DP>& ay,
+                                                const Axis<DP>& az, float& val, float& gx, float& gy, float& gz) {
+  constexpr int NP = Axis<DP>::NP;
+  val = gx = gy = gz = 0.f;
+#pragma unroll
+  for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
+#pragma unroll
+      for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
+          const float* row = f + az.off[k] + ay.off[j];
+          float r = 0.f, rd = 0.f;
+#pragma unroll
+          for (int i = 0; i < NP; ++i) PP_TAP(DP, ax, i) {
+              const float v = row[ax.off[i]];
+              r += ax.w[i] * v;
+              rd += ax.dw[i] * v;
+            }
+          const float wyz = az.w[k] * ay.w[j];
+          val += wyz * r;
+          gx += wyz * rd;
+          gy += (az.w[k] * ay.dw[j]) * r;
+          gz += (az.dw[k] * ay.w[j]) * r;
+        }
+    }
+}
+
+// Hessian: xx, yy, zz, xy, xz, yz
+template <int DP>
+__device__ __forceinline__ void gather_hess(const float* __restrict__ f, const Axis<DP>& ax, const Axis<DP>& ay,
+                                            const Axis<DP>& az, float* h) {
+  constexpr int NP = Axis<DP>::NP;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) h[q] = 0.f;
+#pragma unroll
+  for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
+#pragma unroll
+      for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
+          const float* row = f + az.off[k] + ay.off[j];
+          float r = 0.f, rd = 0.f, rh = 0.f;
+#pragma unroll
+          for (int i = 0; i < NP; ++i) PP_TAP(DP, ax, i) {
+              const float v = row[ax.off[i]];
+              r += ax.w[i] * v;
+              rd += ax.dw[i] * v;
+              rh += ax.hw[i] * v;
+            }
+          h[0] += (az.w[k] * ay.w[j]) * rh;
+          h[1] += (az.w[k] * ay.hw[j]) * r;
+          h[2] += (az.hw[k] * ay.w[j]) * r;
+          h[3] += (az.w[k] * ay.dw[j]) * rd;
+          h[4] += (az.dw[k] * ay.w[j]) * rd;
+          h[5] += (az.dw[k] * ay.dw[j]) * r;
+        }
+    }
+}
+
+// Scatter: dst[...] += (wx_i * (cy_j*cz_k)) * val  -- wy/wz are passed per op
+template <int DP>
+__device__ __forceinline__ void scatter_val(float* __restrict__ dst, const Axis<DP>& ax, const Axis<DP>& ay,
+                                            const Axis<DP>& az, float val) {
+  constexpr int NP = Axis<DP>::NP;
+#pragma unroll
+  for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
+#pragma unroll
+      for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
+          float* row = dst + az.off[k] + ay.off[j];
+          const float wv = (az.w[k] * ay.w[j]) * val;
+#pragma unroll
+          for (int i = 0; i < NP; ++i) PP_TAP(DP, ax, i) unsafeAtomicAdd(row + ax.off[i], ax.w[i] * wv);
+        }
+    }
+}
+
+// Scatter of a gradient triple: weight = g0 dwx wy wz + g1 wx dwy wz + g2 wx wy dwz
+template <int DP>
+__device__ __forceinline__ void scatter_grad(float* __restrict__ dst, const Axis<DP>& ax, const Axis<DP>& ay,
+                                             const Axis<DP>& az, float g0, float g1, float g2) {
+  constexpr int NP = Axis<DP>::NP;
+#pragma unroll
+  for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
+#pragma unroll
+      for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
+          float* row = dst + az.off[k] + ay.off[j];
+          const float cx = (az.w[k] * ay.w[j]) * g0;
+          const float cyz = (az.w[k] * ay.dw[j]) * g1 + (az.dw[k] * ay.w[j]) * g2;
+#pragma unroll
+          for (int i = 0; i < NP; ++i) PP_TAP(DP, ax, i)
+            unsafeAtomicAdd(row + ax.off[i], ax.dw[i] * cx + ax.w[i] * cyz);
+        }
+    }
+}
+
+// Write (or add, for the later channel groups of a channel-reducing op) a 3-vector.
+__device__ __forceinline__ void put3(float* o, long long n1, float a0, float a1, float a2, int accum) {
+  if (accum) { a0 += o[0]; a1 += o[n1]; a2 += o[2 * n1]; }
+  o[0] = a0; o[n1] = a1; o[2 * n1] = a2;
+}
+
+#define PP_KERNEL_PROLOGUE                                       \
+  const int bb = blockIdx.y;                                       \
+  const int v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; \
+  if (v >= a.n1) return;                                           \
+  const float* phi = a.phi + bb * a.phi_bs;
+
+template <int DP>
+__global__ __launch_bounds__(256) void k_pull(KArgs a) {
+  PP_KERNEL_PROLOGUE
+  Axis<DP> ax, ay, az;
+  float* out = a.out + bb * a.out_bs + v;
+  if (!setup<DP, kW>(a, phi, v, ax, ay, az)) {
+    for (int c = 0; c < a.nchan; ++c) out[c * a.out_cs] = __builtin_nanf("");
+    return;
+  }
+  const float* f = a.f + bb * a.f_bs;
+  for (int c = 0; c < a.nchan; ++c) out[c * a.out_cs] = gather_val<DP>(f + c * a.f_cs, ax, ay, az);
+}
+
+template <int DP>
+__global__ __launch_bounds__(256) void k_pullg(KArgs a) {
+  PP_KERNEL_PROLOGUE
+  Axis<DP> ax, ay, az;
+  float* out = a.out + bb * a.out_bs + v;
+  const long long n1 = a.n1;
+  if (!setup<DP, kW | kDW>(a, phi, v, ax, ay, az)) {
+    const float nan = __builtin_nanf("");
+    for (int c = 0; c < a.nchan; ++c) {
+      float* o = out + c * a.out_cs;
+      o[0] = nan; o[n1] = nan; o[2 * n1] = nan;
+    }
+    return;
+  }
+  const float* f = a.f + bb * a.f_bs;
+  for (int c = 0; c < a.nchan; ++c) {
+    float gx, gy, gz;
+    gather_grad<DP>(f + c * a.f_cs, ax, ay, az, gx, gy, gz);
+    float* o = out + c * a.out_cs;
+    o[0] = gx; o[n1] = gy; o[2 * n1] = gz;
+  }
+}
+
+template <int DP>
+__global__ __launch_bounds__(256) void k_pullh(KArgs a) {
+  PP_KERNEL_PROLOGUE
+  Axis<DP> ax, ay, az;
+  float* out = a.out + bb * a.out_bs + v;
+  const long long n1 = a.n1;
+  if (!setup<DP, kW | kDW | kHW>(a, phi, v, ax, ay, az)) {
+    const float nan = __builtin_nanf("");
+    for (int c = 0; c < a.nchan; ++c)
+      for (int q = 0; q < 9; ++q) out[c * a.out_cs + q * n1] = nan;
+    return;
+  }
+  const float* f = a.f + bb * a.f_bs;
+  for (int c = 0; c < a.nchan; ++c) {
+    float h[6];
+    gather_hess<DP>(f + c * a.f_cs, ax, ay, az, h);
+    float* o = out + c * a.out_cs;
+    // column-major 3x3 (pushpull.c:129-134)
+    o[0] = h[0]; o[4 * n1] = h[1]; o[8 * n1] = h[2];
+    o[1 * n1] = h[3]; o[3 * n1] = h[3];
+    o[2 * n1] = h[4]; o[6 * n1] = h[4];
+    o[5 * n1] = h[5]; o[7 * n1] = h[5];
+  }
+}
+
+template <int DP>
+__global__ __launch_bounds__(256) void k_push(KArgs a) {
+  PP_KERNEL_PROLOGUE
+  Axis<DP> ax, ay, az;
+  const float* f1 = a.f + bb * a.f_bs + v;
+  if (!setup<DP, kW>(a, phi, v, ax, ay, az)) return;
+  float* dst = a.out + bb * a.out_bs;
+  for (int c = 0; c < a.nchan; ++c) {
+    const float val = f1[c * a.f_cs];
+    if (isfinite(val)) scatter_val<DP>(dst + c * a.out_cs, ax, ay, az, val);
+  }
+}
+
+template <int DP>
+__global__ __launch_bounds__(256) void k_pushg(KArgs a) {
+  PP_KERNEL_PROLOGUE
+  Axis<DP> ax, ay, az;
+  if (!setup<DP, kW | kDW>(a, phi, v, ax, ay, az)) return;
+  const float* g1 = a.f + bb * a.f_bs + v;
+  const long long n1 = a.n1;
+  float* dst = a.out + bb * a.out_bs;
+  for (int c = 0; c < a.nchan; ++c) {
+    const float* g = g1 + c * a.f_cs;
+    scatter_grad<DP>(dst + c * a.out_cs, ax, ay, az, g[0], g[n1], g[2 * n1]);
+  }
+}
+
+// dphi_a = sum_c delta_c * d f_c / d x_a   (NaN where the sample is invalid)
+template <int DP>
+__global__ __launch_bounds__(256) void k_pull_bwd(KArgs a) {
+  PP_KERNEL_PROLOGUE
+  Axis<DP> ax, ay, az;
+  float* out = a.out + bb * a.out_bs + v;
+  const long long n1 = a.n1;
+  if (!setup<DP, kW | kDW>(a, phi, v, ax, ay, az)) {
+    const float nan = __builtin_nanf("");
+    out[0] = nan; out[n1] = nan; out[2 * n1] = nan;
+    return;
+  }
+  const float* f = a.f + bb * a.f_bs;
+  const float* d = a.aux + bb * a.aux_bs + v;
+  float sx = 0.f, sy = 0.f, sz = 0.f;
+  for (int c = 0; c < a.nchan; ++c) {
+    float gx, gy, gz;
+    gather_grad<DP>(f + c * a.f_cs, ax, ay, az, gx, gy, gz);
+    const float dc = d[c * a.aux_cs];
+    sx += dc * gx; sy += dc * gy; sz += dc * gz;
+  }
+  put3(out, n1, sx, sy, sz, a.accum);
+}
+
+// dphi_b = sum_c sum_a delta_{a,c} H_{a,b,c}
+template <int DP>
+__global__ __launch_bounds__(256) void k_pullg_bwd(KArgs a) {
+  PP_KERNEL_PROLOGUE
+  Axis<DP> ax, ay, az;
+  float* out = a.out + bb * a.out_bs + v;
+  const long long n1 = a.n1;
+  if (!setup<DP, kW | kDW | kHW>(a, phi, v, ax, ay, az)) {
+    const float nan = __builtin_nanf("");
+    out[0] = nan; out[n1] = nan; out[2 * n1] = nan;
+    return;
+  }
+  const float* f = a.f + bb * a.f_bs;
+  const float* d = a.aux + bb * a.aux_bs + v;
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+  for (int c = 0; c < a.nchan; ++c) {
+    float h[6];
+    gather_hess<DP>(f + c * a.f_cs, ax, ay, az, h);
+    const float* dc = d + c * a.aux_cs;
+    const float d0 = dc[0], d1 = dc[n1], d2 = dc[2 * n1];
+    // H = [xx xy xz; xy yy yz; xz yz zz]
+    s0 += d0 * h[0] + d1 * h[3] + d2 * h[4];
+    s1 += d0 * h[3] + d1 * h[1] + d2 * h[5];
+    s2 += d0 * h[4] + d1 * h[5] + d2 * h[2];
+  }
+  put3(out, n1, s0, s1, s2, a.accum);
+}
+
+// Fused matching objective: r_c = pull(f)_c - g_c; loss += r_c^2/2 over finite r;
+// dphi = sum_c r_c grad f_c (NaN where invalid); df0 += push(r) (skips non-finite r).
+template <int DP>
+__global__ __launch_bounds__(256) void k_objgrad(KArgs a) {
+  const int bb = blockIdx.y;
+  const int v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  float lsum = 0.f;
+  if (v < a.n1) {
+    const float* phi = a.phi + bb * a.phi_bs;
+    Axis<DP> ax, ay, az;
+    float* out = a.out + bb * a.out_bs + v;
+    const long long n1 = a.n1;
+    if (!setup<DP, kW | kDW>(a, phi, v, ax, ay, az)) {
+      const float nan = __builtin_nanf("");
+      out[0] = nan; out[n1] = nan; out[2 * n1] = nan;
+    } else {
+      const float* f = a.f + bb * a.f_bs;
+      const float* g = a.aux + bb * a.aux_bs + v;
+      float* df = a.out2 + bb * a.out2_bs;
+      float sx = 0.f, sy = 0.f, sz = 0.f;
+      for (int c = 0; c < a.nchan; ++c) {
+        float val, gx, gy, gz;
+        gather_val_grad<DP>(f + c * a.f_cs, ax, ay, az, val, gx, gy, gz);
+        const float r = val - g[c * a.aux_cs];
+        sx += r * gx; sy += r * gy; sz += r * gz;
+        if (isfinite(r)) {
+          lsum += 0.5f * r * r;
+          scatter_val<DP>(df + c * a.out2_cs, ax, ay, az, r);
+        }
+      }
+      put3(out, n1, sx, sy, sz, a.accum);
+    }
+  }
+  // block reduction of the loss, one double atomic per block
+  for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o);
+  __shared__ float wsum[4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) wsum[wid] = lsum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += (double)wsum[w];
+    unsafeAtomicAdd(a.red, s);
+  }
+}
+
+__global__ void k_fill(float* p, long long n, float v) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+__global__ void k_fill_d(double* p, long long n, double v) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------------
+template <int DP>
+static hipError_t launch_dp(Op op, const KArgs& a, int nbatch, hipStream_t s, int block) {
+  const dim3 grid((unsigned)((a.n1 + block - 1) / block), (unsigned)nbatch, 1);
+  switch (op) {
+    case OP_PULL: hipLaunchKernelGGL(k_pull<DP>, grid, dim3(block), 0, s, a); break;
+    case OP_PULLG: hipLaunchKernelGGL(k_pullg<DP>, grid, dim3(block), 0, s, a); break;
+    case OP_PULLH: hipLaunchKernelGGL(k_pullh<DP>, grid, dim3(block), 0, s, a); break;
+    case OP_PUSH: hipLaunchKernelGGL(k_push<DP>, grid, dim3(block), 0, s, a); break;
+    case OP_PUSHG: hipLaunchKernelGGL(k_pushg<DP>, grid, dim3(block), 0, s, a); break;
+    case OP_PULL_BWD: hipLaunchKernelGGL(k_pull_bwd<DP>, grid, dim3(block), 0, s, a); break;
+    case OP_PULLG_BWD: hipLaunchKernelGGL(k_pullg_bwd<DP>, grid, dim3(block), 0, s, a); break;
+    case OP_OBJGRAD: hipLaunchKernelGGL(k_objgrad<DP>, grid, dim3(block), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_op(Op op, const KArgs& a, int nbatch, hipStream_t s, int block) {
+  if (a.n1 <= 0 || nbatch <= 0) return hipSuccess;
+  const bool iso = a.dpx == a.dpy && a.dpx == a.dpz;
+  if (!iso) return launch_dp<0>(op, a, nbatch, s, block);
+  switch (a.dpx) {
+    case 1: return launch_dp<1>(op, a, nbatch, s, block);
+    case 2: return launch_dp<2>(op, a, nbatch, s, block);
+    case 3: return launch_dp<3>(op, a, nbatch, s, block);
+    case 4: return launch_dp<4>(op, a, nbatch, s, block);
+    case 5: return launch_dp<5>(op, a, nbatch, s, block);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_fill(float* p, long long n, float v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  long long nb = (n + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)nb), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_d(double* p, long long n, double v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  long long nb = (n + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  hipLaunchKernelGGL(k_fill_d, dim3((unsigned)nb), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+}  // namespace pp

@@ -1,0 +1,53 @@
+// pp_kernels.hpp -- host-side view of the gfx950 kernels (internal to libpushpull_hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pp {
+
+// Operations on the path. Names follow artifacts/C/pushpull.c / src/pushpull_pb.jl.
+enum Op : int {
+  OP_PULL = 0,      // f1 = pull(f0, phi)                          pushpull.c:55
+  OP_PULLG = 1,     // g1 = pull_grad(f0, phi)        (3 comps)    pushpull.c:78
+  OP_PULLH = 2,     // h1 = pull_hess(f0, phi)        (9 comps)    pushpull.c:111
+  OP_PUSH = 3,      // f0 += push(f1, phi)                         pushpull.c:147
+  OP_PUSHG = 4,     // f0 += push_grad(g1, phi)                    pushpull.cu:145
+  OP_PULL_BWD = 5,  // dphi = sum_c delta_c grad f_c               pushpull_pb.jl:13-17
+  OP_PULLG_BWD = 6, // dphi = sum_{a,c} delta_{a,c} H_{a,.,c}      pushpull_pb.jl:128-132
+  OP_OBJGRAD = 7,   // r = pull(f0)-g; dphi = sum_c r_c grad f_c; df0 += push(r); loss += r^2/2
+  OP_COUNT = 8
+};
+
+// Kernel arguments, passed by value (kernarg segment; no module globals, so launches
+// on different streams never share state).
+struct KArgs {
+  const float* f;    // gathered image: f0 (pull family) / pushed values (push family)
+  const float* phi;  // deformation, 1-based coordinates, 3 comps of n1 voxels
+  const float* aux;  // upstream gradient (backward ops) / target image (objgrad)
+  float* out;        // output image / dphi
+  float* out2;       // objgrad: dL/df0 (scatter target)
+  double* red;       // objgrad: loss accumulator (one double)
+  long long f_cs, f_bs;      // channel / batch strides of f (elements)
+  long long phi_bs;          // batch stride of phi
+  long long aux_cs, aux_bs;  // strides of aux
+  long long out_cs, out_bs;  // strides of out
+  long long out2_cs, out2_bs;
+  int n1;                    // voxels in the phi grid
+  int d0x, d0y, d0z;         // grid sampled by pull / scattered into by push
+  int dpx, dpy, dpz;         // dp = degree + 1 per axis (used by the runtime-degree kernels)
+  int bx, by, bz;            // boundary code per axis
+  int ext;                   // extrapolate flag
+  int nchan;                 // channels processed per thread
+  int accum;                 // channel-reducing ops: add to `out` instead of overwriting
+  float lx, ly, lz;          // d0 - 0.99f per axis (VOXOK upper bounds)
+};
+
+// Launch op on `stream`; dp is (DP,DP,DP) when isotropic, else the runtime path is used.
+// grid.y = nbatch. Returns hipSuccess or the launch error.
+hipError_t launch_op(Op op, const KArgs& a, int nbatch, hipStream_t stream, int block);
+
+// Fill n floats with v (graph-capturable; no hipMemsetAsync nodes).
+hipError_t launch_fill(float* p, long long n, float v, hipStream_t stream);
+hipError_t launch_fill_d(double* p, long long n, double v, hipStream_t stream);
+
+}  // namespace pp

@@ -1,0 +1,20 @@
+"""PushPull (MI355X): the B-spline pull/push sampling path of PushPull.jl on gfx950.
+
+Same names and argument order as the Julia module (src/PushPull.jl:19):
+``pull, push, pull_grad, Settings, id`` plus the unexported ``push_grad`` and
+``pull_hess``. Compute runs only in libpushpull_hip.so (hand-written HIP kernels);
+there is no CPU fallback.
+"""
+from .settings import Settings, id  # noqa: A001
+from .autograd import pull, pull_grad, push
+from .ops import push_grad, pull_hess, pull_backward, pull_grad_backward, objective_grad
+from . import _lib
+
+__all__ = [
+    "Settings", "id", "pull", "push", "pull_grad", "push_grad", "pull_hess",
+    "pull_backward", "pull_grad_backward", "objective_grad",
+]
+
+
+def library_path() -> str:
+    return _lib.LIB_PATH

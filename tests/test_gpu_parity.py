@@ -1,0 +1,229 @@
+"""HIP kernels against the oracle and the reference's golden outputs (needs a GPU).
+
+Bar: NaN masks identical; finite entries within 1e-5 norm-wise relative error of the
+FP32 reference (float atomics in push make the summation order nondeterministic, which
+that tolerance covers), and within 1e-5 of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import S, case_settings, identity, rel_err, same_nan_mask, smooth_field
+
+torch = pytest.importorskip("torch")
+pp = pytest.importorskip("pushpull")
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+TOL = 1e-5
+
+
+def T(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+def H(t):
+    return t.detach().cpu().numpy()
+
+
+def _sett(c):
+    return pp.Settings(tuple(int(v) for v in c["deg"]), tuple(int(v) for v in c["bnd"]), int(c["ext"]))
+
+
+def _check(got, ref, what, tol=TOL):
+    assert got.shape == ref.shape, (what, got.shape, ref.shape)
+    assert same_nan_mask(got, ref), f"{what}: NaN mask differs"
+    e = rel_err(got, ref)
+    assert e < tol, f"{what}: rel err {e:.3g}"
+
+
+@pytest.mark.parametrize("op", ["pull", "pullg", "pullh", "push", "pushg"])
+def test_golden_tier_b(golden, op):
+    for i, c in enumerate(golden):
+        s = _sett(c)
+        d0 = c["f0"].shape
+        f0, phi = T(c["f0"]), T(c["phi"])
+        if op == "pull":
+            got, ref = H(pp.pull(f0, phi, s)), c["pull"]
+        elif op == "pullg":
+            got, ref = H(pp.pull_grad(f0, phi, s)), c["pullg"]
+        elif op == "pullh":
+            got, ref = H(pp.pull_hess(f0, phi, s)).reshape(c["pullh"].shape), c["pullh"]
+        elif op == "push":
+            got, ref = H(pp.push(T(c["f1"]), phi, d0, s)), c["push"]
+        else:  # no usable reference output (pushpull.c:185): the oracle's adjoint
+            got = H(pp.push_grad(T(c["g1"]), phi, d0, s))
+            ref = oracle.push_grad(c["g1"], c["phi"], d0, case_settings(c), prec="f64")[0, 0]
+        _check(got, ref, f"case {i} {op}")
+
+
+def test_golden_tier_a(golden):
+    """Host arrays go through the symbol-compatible entry points (pull, pullg, ...)."""
+    for i, c in enumerate(golden[::3]):
+        s = _sett(c)
+        d0 = c["f0"].shape
+        _check(pp.pull(c["f0"], c["phi"], s), c["pull"], f"A{i} pull")
+        _check(pp.pull_grad(c["f0"], c["phi"], s), c["pullg"], f"A{i} pullg")
+        _check(pp.pull_hess(c["f0"], c["phi"], s).reshape(c["pullh"].shape), c["pullh"], f"A{i} pullh")
+        _check(pp.push(c["f1"], c["phi"], d0, s), c["push"], f"A{i} push")
+        ref = oracle.push_grad(c["g1"], c["phi"], d0, case_settings(c), prec="f64")[0, 0]
+        _check(pp.push_grad(c["g1"], c["phi"], d0, s), ref, f"A{i} pushg")
+
+
+def test_tier_a_push_accumulates_into_caller_array():
+    import ctypes as ct
+
+    rng = np.random.default_rng(3)
+    d0 = (6, 5, 4)
+    phi = identity((5, 5, 5)) + rng.standard_normal((3, 5, 5, 5)).astype(np.float32)
+    f1 = rng.standard_normal((5, 5, 5)).astype(np.float32)
+    base = rng.standard_normal(d0).astype(np.float32)
+    out = base.copy()
+    L = pp._lib.lib()
+    dd0 = np.array(d0[::-1], np.uint64)
+    bnd = np.array([1, 1, 1], np.int32)
+    dp = np.array([3, 3, 3], np.uint64)
+    P = lambda a: ct.c_void_p(a.ctypes.data)  # noqa: E731
+    L.push(P(out), P(phi), P(f1), P(dd0), f1.size, P(bnd), P(dp), 1)
+    ref = base.astype(np.float64) + oracle.push(f1, phi, d0, S(2, 1, 1), prec="f64")[0, 0]
+    assert rel_err(out, ref) < TOL
+
+
+def _volume_case(rng, d, sigma_f, maxdisp, out_frac=0.0):
+    f = smooth_field(rng, d, sigma_f, 1.0)[0]
+    f = (f / f.std()).astype(np.float32)
+    phi = identity(d) + smooth_field(rng, d, 4.0, maxdisp)
+    if out_frac:
+        flat = phi.reshape(3, -1)
+        n = flat.shape[1]
+        idx = rng.choice(n, size=int(out_frac * n), replace=False)
+        ax = rng.integers(0, 3, size=idx.size)
+        dims = np.array(d[::-1], np.float32)[ax]
+        lo = rng.uniform(-0.5, 0.0, size=idx.size) * dims
+        hi = dims + 1 + rng.uniform(0.0, 0.5, size=idx.size) * dims
+        flat[ax, idx] = np.where(rng.random(idx.size) < 0.5, lo, hi).astype(np.float32)
+    return f, phi
+
+
+@pytest.mark.parametrize("deg", [1, 2, 3])
+@pytest.mark.parametrize("bnd", [0, 1])
+def test_config2_pull_push_adjoint(deg, bnd):
+    """192^3, smoothed noise, displacement up to 8 voxels, 5% sent outside the FOV."""
+    rng = np.random.default_rng(100 + 10 * deg + bnd)
+    d = (192, 192, 192)
+    f, phi = _volume_case(rng, d, 2.0, 8.0, out_frac=0.05)
+    g = rng.standard_normal(d).astype(np.float32)
+    s = pp.Settings(deg, bnd, 1)
+    so = S(deg, bnd, 1)
+    tf, tphi, tg = T(f), T(phi), T(g)
+    f1 = H(pp.pull(tf, tphi, s))
+    r32 = oracle.pull(f, phi, so)[0, 0]
+    r64 = oracle.pull(f, phi, so, prec="f64")[0, 0]
+    _check(f1, r32, "pull vs f32 ref")
+    _check(f1, r64, "pull vs f64")
+    f0 = H(pp.push(tg, tphi, d, s))
+    p64 = oracle.push(g, phi, d, so, prec="f64")[0, 0]
+    _check(f0, p64, "push vs f64")
+    lhs = float(torch.sum(T(f1).double() * tg.double()))
+    rhs = float(torch.sum(tf.double() * T(f0).double()))
+    assert abs(lhs - rhs) / (abs(lhs) + abs(rhs)) < TOL
+    # FP64 oracle adjointness (pins push as the exact adjoint of pull)
+    lhs64 = float(np.sum(r64 * g.astype(np.float64)))
+    rhs64 = float(np.sum(f.astype(np.float64) * p64))
+    assert abs(lhs64 - rhs64) / (abs(lhs64) + abs(rhs64)) < 1e-12
+
+
+def test_config1_nan_mask_ext0():
+    """64^3, U[0,1), trilinear, reflect, ext=0: the NaN mask must match exactly."""
+    rng = np.random.default_rng(1)
+    d = (64, 64, 64)
+    f = rng.random(d, dtype=np.float32)
+    phi = identity(d) + smooth_field(rng, d, 4.0, 3.0)
+    s = pp.Settings(1, 1, 0)
+    got = H(pp.pull(T(f), T(phi), s))
+    ref = oracle.pull(f, phi, S(1, 1, 0))[0, 0]
+    assert np.isnan(ref).sum() > 0
+    _check(got, ref, "config 1 pull")
+
+
+@pytest.mark.parametrize("deg", [1, 2, 3])
+def test_pull_grad_backward_and_hess(deg):
+    rng = np.random.default_rng(40 + deg)
+    d = (40, 36, 44)
+    f, phi = _volume_case(rng, d, 2.0, 5.0, out_frac=0.02)
+    fc = np.stack([f, np.roll(f, 3, axis=0)])[None]  # (1, 2, ...)
+    delta = rng.standard_normal((1, 2) + d).astype(np.float32)
+    delta3 = rng.standard_normal((1, 2, 3) + d).astype(np.float32)
+    s = pp.Settings(deg, [(1, 1, 1), (0, 2, 1)], 0)
+    so = S(deg, [(1, 1, 1), (0, 2, 1)], 0)
+    tf, tphi = T(fc), T(phi[None])
+    _check(H(pp.pull_grad(tf, tphi, s)), oracle.pull_grad(fc, phi[None], so, prec="f64"), "pull_grad")
+    _check(H(pp.pull_hess(tf, tphi, s)), oracle.pull_hess(fc, phi[None], so, prec="f64"), "pull_hess")
+    _check(H(pp.pull_backward(tf, tphi, T(delta), s)), oracle.backward_phi(fc, phi[None], delta, so), "pull_backward")
+    H64 = oracle.pull_hess(fc, phi[None], so, prec="f64")  # (B, C, 3 (b), 3 (a), ...)
+    # sum_{a,c} delta[a,c] * H[a, b]  (src/pushpull_pb.jl:128-132)
+    ref = np.zeros((1, 3) + d)
+    for c in range(2):
+        for a in range(3):
+            for b in range(3):
+                ref[0, b] += delta3[0, c, a] * H64[0, c, b, a]
+    _check(H(pp.pull_grad_backward(tf, tphi, T(delta3), s)), ref, "pull_grad_backward")
+    g3 = delta3
+    _check(H(pp.push_grad(T(g3), tphi, d, s)), oracle.push_grad(g3, phi[None], d, so, prec="f64"), "push_grad")
+
+
+def test_push_grad_is_adjoint_of_pull_grad():
+    rng = np.random.default_rng(5)
+    d = (48, 40, 32)
+    f, phi = _volume_case(rng, d, 2.0, 6.0, out_frac=0.05)
+    g = rng.standard_normal((3,) + d).astype(np.float32)
+    for deg in (1, 2, 3, 4):
+        s = pp.Settings(deg, (0, 1, 2), 1)
+        lhs = float(torch.sum(pp.pull_grad(T(f), T(phi), s).double() * T(g).double()))
+        rhs = float(torch.sum(T(f).double() * pp.push_grad(T(g), T(phi), d, s).double()))
+        assert abs(lhs - rhs) / (abs(lhs) + abs(rhs)) < TOL, deg
+
+
+def test_batches_channels_per_channel_bounds():
+    rng = np.random.default_rng(9)
+    d0, d1 = (20, 18, 22), (16, 19, 21)
+    B, C = 2, 3
+    f0 = rng.standard_normal((B, C) + d0).astype(np.float32)
+    phi = np.stack([identity(d1) + rng.uniform(-3, 3, (3,) + d1).astype(np.float32) * 2 for _ in range(B)])
+    f1 = rng.standard_normal((B, C) + d1).astype(np.float32)
+    bnds = [(0, 1, 2), (1, 1, 1), (2, 0, 1)]
+    for deg in [(2, 2, 2), (1, 3, 0)]:
+        s = pp.Settings(deg, bnds, 1)
+        so = S(deg, bnds, 1)
+        _check(H(pp.pull(T(f0), T(phi), s)), oracle.pull(f0, phi, so, prec="f64"), f"pull {deg}")
+        _check(H(pp.pull_grad(T(f0), T(phi), s)), oracle.pull_grad(f0, phi, so, prec="f64"), f"pullg {deg}")
+        _check(H(pp.push(T(f1), T(phi), d0, s)), oracle.push(f1, phi, d0, so, prec="f64"), f"push {deg}")
+        _check(H(pp.pull_backward(T(f0), T(phi), T(f1), s)), oracle.backward_phi(f0, phi, f1, so), f"bwd {deg}")
+
+
+@pytest.mark.parametrize("deg", [1, 2, 3])
+def test_objective_grad_fused_matches_composition(deg):
+    rng = np.random.default_rng(70 + deg)
+    d = (36, 40, 44)
+    f, phi = _volume_case(rng, d, 2.0, 6.0)
+    g = smooth_field(rng, d, 2.0, 1.0)[:2][None]
+    fc = np.stack([f, -f])[None]
+    s = pp.Settings(deg, 1, 1)
+    so = S(deg, 1, 1)
+    loss, dphi, df0 = pp.objective_grad(T(fc), T(phi[None]), T(g), s)
+    r = oracle.pull(fc, phi[None], so, prec="f64") - g
+    _check(H(dphi), oracle.backward_phi(fc, phi[None], r, so), "objgrad dphi")
+    _check(H(df0), oracle.push(r, phi[None], d, so, prec="f64"), "objgrad df0")
+    assert abs(float(loss) - 0.5 * np.sum(r ** 2)) / (0.5 * np.sum(r ** 2)) < TOL
+
+
+def test_errors_raise():
+    s = pp.Settings(2, 1, 1)
+    f = torch.zeros((4, 4, 4), device=DEV)
+    phi = torch.zeros((3, 5, 5, 5), device=DEV)
+    with pytest.raises(AssertionError):
+        pp.pull(f, torch.zeros((2, 5, 5, 5), device=DEV), s)
+    with pytest.raises(AssertionError):
+        pp.push(torch.zeros((4, 4, 4), device=DEV), phi, (4, 4, 4), s)
+    with pytest.raises(TypeError):
+        pp.pull(f.double(), phi, s)

@@ -1,0 +1,156 @@
+"""Host-side checks that need no GPU: Settings/id normalisation, the C ABI's exports,
+descriptor layout and argument validation (which returns before any HIP call)."""
+import ctypes as ct
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import pushpull as pp
+from pushpull import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+# ---- Settings (src/pushpull_general.jl:15-60) ------------------------------------
+def test_settings_defaults():
+    s = pp.Settings()
+    assert s.deg == (1, 1, 1) and s.bnd == [(1, 1, 1)] and s.ext == 1
+
+
+def test_settings_clamps_degree_and_ext():
+    s = pp.Settings((7, -2, 3), 0, 5)
+    assert s.deg == (4, 0, 3) and s.ext == 1
+    assert pp.Settings(2, 1, -3).ext == 0
+    assert pp.Settings(9).deg == (4, 4, 4)
+
+
+def test_settings_boundary_forms():
+    assert pp.Settings(1, 2).bnd == [(2, 2, 2)]
+    assert pp.Settings(1, (0, 1, 2)).bnd == [(0, 1, 2)]
+    assert pp.Settings(1, [(0, 1, 2), (2, 2, 1)]).bnd == [(0, 1, 2), (2, 2, 1)]
+    m = np.array([[0, 1], [1, 2], [2, 0]])  # 3 x C matrix, one column per channel
+    assert pp.Settings(1, m).bnd == [(0, 1, 2), (1, 2, 0)]
+
+
+@pytest.mark.parametrize("bad", [3, -1, (0, 1, 3), [(0, 1, 2), (0, 5, 1)], np.array([[0], [1], [4]])])
+def test_settings_rejects_bad_boundary(bad):
+    with pytest.raises(AssertionError):
+        pp.Settings(1, bad)
+
+
+def test_identity_grid_is_one_based():
+    phi = pp.id(dim=(4, 3, 2))  # Julia (X, Y, Z)
+    assert phi.shape == (3, 2, 3, 4)
+    assert phi[0, 0, 0, :].tolist() == [1, 2, 3, 4]
+    assert phi[1, 0, :, 0].tolist() == [1, 2, 3]
+    assert phi[2, :, 0, 0].tolist() == [1, 2]
+
+
+def test_identity_grid_affine():
+    A = np.array([[2.0, 0, 0, 1], [0, 1, 0, -1], [0, 0.5, 1, 0]])
+    phi = pp.id(dim=(3, 2, 2), affine=A)
+    # x = 2*i + 1 at (i,j,k) = (3,1,1); z = 0.5*j + k at j=2, k=2
+    assert phi[0, 0, 0, 2] == 7.0
+    assert phi[2, 1, 1, 0] == 3.0
+
+
+# ---- the C ABI --------------------------------------------------------------------
+def test_library_exports_every_declared_function():
+    names = _lib.declared_functions()
+    assert {"pull", "pullg", "pullh", "push", "pushg", "pp_pull", "pp_push", "pp_last_error"} <= set(names)
+    L = ct.CDLL(_lib.LIB_PATH)
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_desc_layout_matches_header(tmp_path):
+    src = tmp_path / "sz.c"
+    src.write_text('#include "pushpull_hip.h"\n#include <stdio.h>\n#include <stddef.h>\n'
+                   'int main(void){printf("%zu %zu %zu %zu\\n", sizeof(pp_desc), offsetof(pp_desc, nchan),'
+                   ' offsetof(pp_desc, nbnd), offsetof(pp_desc, bnd)); return 0;}\n')
+    exe = tmp_path / "sz"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    D = _lib.PPDesc
+    assert [int(v) for v in out] == [ct.sizeof(D), D.nchan.offset, D.nbnd.offset, D.bnd.offset]
+
+
+def _desc(**kw):
+    d = _lib.PPDesc()
+    for a in range(3):
+        d.d0[a] = 8
+        d.d1[a] = 8
+        d.deg[a] = 1
+    d.nchan, d.nbatch, d.ext, d.nbnd = 1, 1, 1, 1
+    for k, v in kw.items():
+        if k in ("d0", "d1", "deg"):
+            for a in range(3):
+                getattr(d, k)[a] = v[a]
+        elif k == "bnd":
+            for i, b in enumerate(v):
+                d.bnd[i] = b
+        else:
+            setattr(d, k, v)
+    return d
+
+
+@pytest.mark.parametrize("kw,code", [
+    (dict(deg=(1, 5, 1)), -1),
+    (dict(deg=(-1, 1, 1)), -1),
+    (dict(d0=(0, 8, 8)), -1),
+    (dict(d1=(8, 8, 0)), -1),
+    (dict(nchan=0), -1),
+    (dict(nbatch=0), -1),
+    (dict(nchan=3, nbnd=2), -1),
+    (dict(bnd=(0, 1, 3)), -1),
+    (dict(d0=(1 << 30, 8, 8)), -4),
+    (dict(d1=(4096, 4096, 256)), -4),
+])
+def test_descriptor_validation(kw, code):
+    L = _lib.lib()
+    d = _desc(**kw)
+    p = ct.c_void_p(16)  # never dereferenced: validation fails first
+    assert L.pp_pull(ct.byref(d), p, p, p, None) == code
+    assert L.pp_last_status() == code
+    assert len(_lib.last_error()) > 0
+
+
+def test_null_arrays_rejected():
+    L = _lib.lib()
+    d = _desc()
+    assert L.pp_pull(ct.byref(d), None, None, None, None) == -1
+    assert "null" in _lib.last_error()
+
+
+def test_tier_a_rejects_bad_dp_without_touching_outputs_twice():
+    L = _lib.lib()
+    out = np.zeros(4, np.float32)
+    phi = np.ones(12, np.float32)
+    f0 = np.ones(8, np.float32)
+    d0 = np.array([2, 2, 2], np.uint64)
+    dp = np.array([6, 2, 2], np.uint64)  # dp outside 1..5
+    bnd = np.array([1, 1, 1], np.int32)
+    P = lambda a: ct.c_void_p(a.ctypes.data)  # noqa: E731
+    L.pull(P(out), P(phi), P(f0), P(d0), 4, P(bnd), P(dp), 1)
+    assert L.pp_last_status() == -1
+    assert np.all(np.isnan(out))
+
+
+def test_shape_assertions_mirror_julia():
+    s = pp.Settings()
+    f = np.zeros((4, 4, 4), np.float32)
+    with pytest.raises(AssertionError):
+        pp.pull(f, np.zeros((2, 4, 4, 4), np.float32), s)  # phi needs 3 channels
+    with pytest.raises(AssertionError):
+        pp.pull(np.zeros((4, 4), np.float32), np.zeros((3, 4, 4, 4), np.float32), s)
+    with pytest.raises(AssertionError):
+        pp.push(np.zeros((5, 4, 4), np.float32), np.zeros((3, 4, 4, 4), np.float32), (4, 4, 4), s)
+    with pytest.raises(AssertionError):
+        pp.pull(np.zeros((2, 1, 4, 4, 4), np.float32), np.zeros((3, 3, 4, 4, 4), np.float32), s)
+    with pytest.raises(AssertionError):
+        pp.pull(np.zeros((3, 4, 4, 4), np.float32), np.zeros((3, 4, 4, 4), np.float32),
+                pp.Settings(1, [(1, 1, 1), (0, 0, 0)]))
+    with pytest.raises(TypeError):
+        pp.pull(f.astype(np.float64), np.zeros((3, 4, 4, 4), np.float32), s)
