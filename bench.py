@@ -112,6 +112,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--tiling", type=int, default=1, help="1: LDS-tiled kernels, 0: one work-item per sample")
+    ap.add_argument("--shape", type=int, default=0, help="tile shape index (0: 16x8x8, 1: 32x8x4, 2: 16x16x4)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 256 MiB device copy (reference kernel for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -121,30 +125,41 @@ def main():
 
     import pushpull as pp
 
+    from pushpull import _lib, ops
+
     dev = torch.device("cuda:0")
     f, phi, delta = make_inputs(torch, dev)
     n1 = N ** 3
-    outs = {
-        "f1": torch.empty_like(f),
-        "df": torch.empty_like(f),
-        "dphi": torch.empty_like(phi),
-    }
-    setts = {deg: pp.Settings(deg, 1, 1) for deg in DEGS}
-    from pushpull import ops
+    f1 = torch.empty_like(f)
+    df = torch.empty_like(f)
+    dphi = torch.empty_like(phi)
+    L = _lib.lib()
+    _lib.set_option(_lib.PP_OPT_TILING, args.tiling)
+    _lib.set_option(_lib.PP_OPT_TILE_SHAPE, args.shape)
+    if args.copy_calibration:
+        src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev).normal_()
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    descs = {deg: ops._desc(pp.Settings(deg, 1, 1), (N, N, N), (N, N, N), 1, 1) for deg in DEGS}
+    P = lambda t: ct.c_void_p(t.data_ptr())  # noqa: E731
+    stream = ct.c_void_p(torch.cuda.current_stream().cuda_stream)
 
     def step(events=None):
         for deg in DEGS:
-            s = setts[deg]
+            d = ct.byref(descs[deg])
             marks = []
             if events is not None:
                 marks.append(events.record())
-            outs["f1"] = ops.pull_raw(f, phi, s)
+            _lib.check(L.pp_pull(d, P(f1), P(phi), P(f), stream), "pull")
             if events is not None:
                 marks.append(events.record())
-            ops._push_family("push", 1, delta, phi, (N, N, N), s, accumulate_into=None)
+            _lib.check(L.pp_push(d, P(df), P(phi), P(delta), 0, stream), "push")
             if events is not None:
                 marks.append(events.record())
-            ops.pull_backward(f, phi, delta, s)
+            _lib.check(L.pp_pull_backward(d, P(dphi), P(phi), P(f), P(delta), stream), "backward")
             if events is not None:
                 marks.append(events.record())
                 events.add(deg, marks)
@@ -215,7 +230,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (smoothed Gaussian noise image and upstream gradient; identity + smooth displacement, max 6 voxels)",
         "config": {"workload": "pull+push+backward, 256^3 x 1 channel, degrees 1,2,3 (one triple each per step), "
-                               "reflect boundary, ext=1", "volume": [N, N, N], "channels": 1, "degrees": list(DEGS)},
+                               "reflect boundary, ext=1", "volume": [N, N, N], "channels": 1, "degrees": list(DEGS),
+                   "kernels": "tiled" if args.tiling else "per-sample", "tile_shape": args.shape},
         "roofline": {"bound": "hbm", "kernel": f"{op} deg{deg}", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_launch": BYTES[op] * n1, "avg_ms": round(ms, 4)},

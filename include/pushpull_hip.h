@@ -128,6 +128,15 @@ int pp_memset_zero(float *p, size_t count, void *stream);
 int pp_stream_sync(void *stream);
 int pp_device_sync(void);
 
+/* ---- kernel selection (no counterpart in the reference) -------------------- */
+enum {
+  PP_OPT_TILING = 1,     /* 1 (default): LDS-tiled kernels for pull, push, push_grad, the pull
+                            pullback and the fused objective; 0: one work-item per sample */
+  PP_OPT_TILE_SHAPE = 2  /* tile of the phi grid: 0 = 16x8x8 (default), 1 = 32x8x4, 2 = 16x16x4 */
+};
+int pp_set_option(int key, int value);
+int pp_get_option(int key);
+
 /* ---- diagnostics ---------------------------------------------------------- */
 const char *pp_last_error(void);  /* message of the last failure on this thread */
 int pp_last_status(void);          /* status of the last call on this thread */

@@ -2,6 +2,7 @@
 // include/pushpull_hip.h): argument checking, stride set-up, Tier A host staging.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -42,6 +43,19 @@ constexpr long long kMaxVox = (1LL << 31) - 1;  // per-volume voxel count (32-bi
 constexpr long long kMaxDim = 1LL << 29;        // per-axis size (2*d must fit an int)
 constexpr int kBlock = 256;
 
+// Kernel selection (pp_set_option): tiled LDS kernels on/off and their tile shape.
+std::atomic<int> g_tiling{1};
+std::atomic<int> g_shape{0};
+
+hipError_t launch(pp::Op op, const pp::KArgs& a, int nbatch, hipStream_t s) {
+  if (g_tiling.load(std::memory_order_relaxed)) {
+    pp::TileCfg cfg{g_shape.load(std::memory_order_relaxed)};
+    hipError_t e = pp::launch_tiled(op, a, nbatch, s, cfg);
+    if (e != hipErrorNotSupported) return e;
+  }
+  return pp::launch_op(op, a, nbatch, s, kBlock);
+}
+
 struct Geo {
   long long n0, n1;
 };
@@ -72,6 +86,9 @@ int check_desc(const pp_desc* d, Geo& g) {
 void base_args(const pp_desc* d, pp::KArgs& a, const Geo& g) {
   std::memset(&a, 0, sizeof a);
   a.n1 = (int)g.n1;
+  a.d1x = (int)d->d1[0];
+  a.d1y = (int)d->d1[1];
+  a.d1z = (int)d->d1[2];
   a.d0x = (int)d->d0[0];
   a.d0y = (int)d->d0[1];
   a.d0z = (int)d->d0[2];
@@ -102,7 +119,7 @@ int run(pp::Op op, const pp_desc* d, const Geo& g, pp::KArgs a, hipStream_t s) {
     a.by = d->bnd[1];
     a.bz = d->bnd[2];
     a.nchan = d->nchan;
-    hipError_t e = pp::launch_op(op, a, d->nbatch, s, kBlock);
+    hipError_t e = launch(op, a, d->nbatch, s);
     return e == hipSuccess ? ok() : hip_fail(e, "kernel launch");
   }
   for (int c = 0; c < d->nchan; ++c) {
@@ -119,7 +136,7 @@ int run(pp::Op op, const pp_desc* d, const Geo& g, pp::KArgs a, hipStream_t s) {
     } else {
       b.out = a.out + c * a.out_cs;
     }
-    hipError_t e = pp::launch_op(op, b, d->nbatch, s, kBlock);
+    hipError_t e = launch(op, b, d->nbatch, s);
     if (e != hipSuccess) return hip_fail(e, "kernel launch");
   }
   (void)g;
@@ -206,6 +223,9 @@ int tierA_args(const pp_usize* d0, pp_usize n1, const int* bnd, const pp_usize* 
   if (n0 > kMaxVox || n1 > (pp_usize)kMaxVox) return fail(PP_ERANGE, "slice larger than 2^31-1 voxels");
   std::memset(&a, 0, sizeof a);
   a.n1 = (int)n1;
+  a.d1x = (int)n1;  // Tier A sees a flat list of samples
+  a.d1y = 1;
+  a.d1z = 1;
   a.d0x = (int)d0[0];
   a.d0y = (int)d0[1];
   a.d0z = (int)d0[2];
@@ -394,6 +414,28 @@ int pp_stream_sync(void* s) {
 int pp_device_sync(void) {
   hipError_t e = hipDeviceSynchronize();
   return e == hipSuccess ? ok() : hip_fail(e, "hipDeviceSynchronize");
+}
+
+int pp_set_option(int key, int value) {
+  switch (key) {
+    case PP_OPT_TILING:
+      g_tiling.store(value != 0);
+      return ok();
+    case PP_OPT_TILE_SHAPE:
+      if (value < 0 || value > 2) return fail(PP_EINVAL, "tile shape %d not in 0..2", value);
+      g_shape.store(value);
+      return ok();
+    default:
+      return fail(PP_EINVAL, "unknown option %d", key);
+  }
+}
+
+int pp_get_option(int key) {
+  switch (key) {
+    case PP_OPT_TILING: return g_tiling.load();
+    case PP_OPT_TILE_SHAPE: return g_shape.load();
+    default: return fail(PP_EINVAL, "unknown option %d", key);
+  }
 }
 
 const char* pp_last_error(void) { return t_err.c_str(); }

@@ -26,6 +26,9 @@ constexpr int kMaxDp = 5;
 constexpr float kFastLimit = 1048576.0f;  // 2^20
 
 // ---- piecewise reference definitions (slow path and runtime-degree path) -----
+// Evaluated without FMA contraction, so a sample on the large-coordinate path rounds
+// every operation as the reference does.
+#pragma clang fp contract(off)
 __device__ __forceinline__ float bs2(float u) {
   u = fabsf(u);
   if (u < 0.5f) return 0.75f - u * u;
@@ -99,6 +102,8 @@ __device__ __forceinline__ void weights_rt(int dp, float t, float* w, float* dw,
   }
 }
 
+#pragma clang fp contract(fast)
+
 // Weights for compile-time DP with t in the canonical interval [DP/2-1, DP/2]: one
 // polynomial piece per tap (the same pieces the piecewise forms select there).
 template <int DP>
@@ -160,19 +165,19 @@ __device__ __forceinline__ int fold32(int code, int i, int d, bool& neg) {
   return 2 * d - 1 - m;
 }
 
-__device__ __noinline__ int fold64(int code, long long i, long long d, bool& neg) {
-  neg = false;
+// 64-bit fold for the large-coordinate path; returns the folded index in the low 32
+// bits and the antisymmetric sign flag in bit 32.
+__device__ __noinline__ long long fold64(int code, long long i, long long d) {
   if (code == 0) {
     long long r = i % d;
     if (r < 0) r += d;
-    return (int)r;
+    return r;
   }
   const long long p = 2 * d;
   long long m = i % p;
   if (m < 0) m += p;
-  if (m < d) return (int)m;
-  neg = (code == 2);
-  return (int)(p - 1 - m);
+  if (m < d) return m;
+  return (p - 1 - m) | ((code == 2) ? (1LL << 32) : 0LL);
 }
 
 // Weight sets an op needs
@@ -186,6 +191,7 @@ struct Axis {
   float w[NP], dw[NP], hw[NP];
   int off[NP];
   int n;
+  bool fast;  // |x| < 2^20: canonical weights, 32-bit fold
 };
 
 template <int DP, int NEED>
@@ -195,7 +201,8 @@ __device__ __forceinline__ void make_axis(float x, int dp_rt, int d, int stride,
   ax.n = dp;
   const float c = ceilf(x - 0.5f * (float)dp);
   float w[NP], dw[NP], hw[NP];
-  bool fast = fabsf(x) < kFastLimit;
+  const bool fast = fabsf(x) < kFastLimit;
+  ax.fast = fast;
   int o32 = 0;
   long long o64 = 0;
   float t;
@@ -216,7 +223,15 @@ __device__ __forceinline__ void make_axis(float x, int dp_rt, int d, int stride,
   for (int i = 0; i < NP; ++i) {
     bool neg = false;
     int b = 0;
-    if (DP > 0 || i < dp) b = fast ? fold32(code, o32 + i, d, neg) : fold64(code, o64 + i, d, neg);
+    if (DP > 0 || i < dp) {
+      if (fast) {
+        b = fold32(code, o32 + i, d, neg);
+      } else {
+        const long long r = fold64(code, o64 + i, d);
+        b = (int)(r & 0xffffffffLL);
+        neg = (r >> 32) != 0;
+      }
+    }
     const float s = neg ? -1.0f : 1.0f;
     ax.off[i] = b * stride;
     if (NEED & kW) ax.w[i] = w[i] * s;

@@ -6,178 +6,9 @@
 // with the x weights first, then weighted by w_y*w_z. Scatters use the hardware float
 // atomic (global_atomic_add_f32). Block ids are remapped so that each XCD's L2 sees a
 // contiguous range of voxels (neighbouring rows share most of their source footprint).
-#include "pp_bspline.hpp"
-#include "pp_kernels.hpp"
+#include "pp_sample.hpp"
 
 namespace pp {
-
-constexpr int kXcds = 8;
-
-// Round-robin dispatch puts block b on XCD b % 8; give XCD k a contiguous chunk.
-__device__ __forceinline__ int xcd_block(int b, int nb) {
-  const int per = nb / kXcds, rem = nb % kXcds;
-  const int x = b % kXcds, i = b / kXcds;
-  return (x < rem) ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
-}
-
-template <int DP, int NEED>
-__device__ __forceinline__ bool setup(const KArgs& a, const float* phi, int v, Axis<DP>& ax, Axis<DP>& ay,
-                                      Axis<DP>& az) {
-  const float x = phi[v] - 1.0f;
-  const float y = phi[v + a.n1] - 1.0f;
-  const float z = phi[v + 2 * (long long)a.n1] - 1.0f;
-  if (!voxok(x, y, z, a.ext, a.lx, a.ly, a.lz)) return false;
-  make_axis<DP, NEED>(x, a.dpx, a.d0x, 1, a.bx, ax);
-  make_axis<DP, NEED>(y, a.dpy, a.d0y, a.d0x, a.by, ay);
-  make_axis<DP, NEED>(z, a.dpz, a.d0z, a.d0x * a.d0y, a.bz, az);
-  return true;
-}
-
-#define PP_TAP(DP, ax, i) if (DP > 0 || (i) < (ax).n)
-
-// Value: sum_ijk wx_i wy_j wz_k f[...]
-template <int DP>
-__device__ __forceinline__ float gather_val(const float* __restrict__ f, const Axis<DP>& ax, const Axis<DP>& ay,
-                                            const Axis<DP>& az) {
-  constexpr int NP = Axis<DP>::NP;
-  float acc = 0.f;
-#pragma unroll
-  for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
-#pragma unroll
-      for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
-          const float* row = f + az.off[k] + ay.off[j];
-          float r = 0.f;
-#pragma unroll
-          for (int i = 0; i < NP; ++i) PP_TAP(DP, ax, i) r += ax.w[i] * row[ax.off[i]];
-          acc += (az.w[k] * ay.w[j]) * r;
-        }
-    }
-  return acc;
-}
-
-// Gradient: (d/dx, d/dy, d/dz)
-template <int DP>
-__device__ __forceinline__ void gather_grad(const float* __restrict__ f, const Axis<DP>& ax, const Axis<DP>& ay,
-                                            const Axis<DP>& az, float& gx, float& gy, float& gz) {
-  constexpr int NP = Axis<DP>::NP;
-  gx = gy = gz = 0.f;
-#pragma unroll
-  for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
-#pragma unroll
-      for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
-          const float* row = f + az.off[k] + ay.off[j];
-          float r = 0.f, rd = 0.f;
-#pragma unroll
-          for (int i = 0; i < NP; ++i) PP_TAP(DP, ax, i) {
-              const float v = row[ax.off[i]];
-              r += ax.w[i] * v;
-              rd += ax.dw[i] * v;
-            }
-          gx += (az.w[k] * ay.w[j]) * rd;
-          gy += (az.w[k] * ay.dw[j]) * r;
-          gz += (az.dw[k] * ay.w[j]) * r;
-        }
-    }
-}
-
-// Value and gradient from one footprint
-template <int DP>
-__device__ __forceinline__ void gather_val_grad(const float* __restrict__ f, const Axis<DP>& ax, const Axis<DP>& ay,
-                                                const Axis<DP>& az, float& val, float& gx, float& gy, float& gz) {
-  constexpr int NP = Axis<DP>::NP;
-  val = gx = gy = gz = 0.f;
-#pragma unroll
-  for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
-#pragma unroll
-      for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
-          const float* row = f + az.off[k] + ay.off[j];
-          float r = 0.f, rd = 0.f;
-#pragma unroll
-          for (int i = 0; i < NP; ++i) PP_TAP(DP, ax, i) {
-              const float v = row[ax.off[i]];
-              r += ax.w[i] * v;
-              rd += ax.dw[i] * v;
-            }
-          const float wyz = az.w[k] * ay.w[j];
-          val += wyz * r;
-          gx += wyz * rd;
-          gy += (az.w[k] * ay.dw[j]) * r;
-          gz += (az.dw[k] * ay.w[j]) * r;
-        }
-    }
-}
-
-// Hessian: xx, yy, zz, xy, xz, yz
-template <int DP>
-__device__ __forceinline__ void gather_hess(const float* __restrict__ f, const Axis<DP>& ax, const Axis<DP>& ay,
-                                            const Axis<DP>& az, float* h) {
-  constexpr int NP = Axis<DP>::NP;
-#pragma unroll
-  for (int q = 0; q < 6; ++q) h[q] = 0.f;
-#pragma unroll
-  for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
-#pragma unroll
-      for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
-          const float* row = f + az.off[k] + ay.off[j];
-          float r = 0.f, rd = 0.f, rh = 0.f;
-#pragma unroll
-          for (int i = 0; i < NP; ++i) PP_TAP(DP, ax, i) {
-              const float v = row[ax.off[i]];
-              r += ax.w[i] * v;
-              rd += ax.dw[i] * v;
-              rh += ax.hw[i] * v;
-            }
-          h[0] += (az.w[k] * ay.w[j]) * rh;
-          h[1] += (az.w[k] * ay.hw[j]) * r;
-          h[2] += (az.hw[k] * ay.w[j]) * r;
-          h[3] += (az.w[k] * ay.dw[j]) * rd;
-          h[4] += (az.dw[k] * ay.w[j]) * rd;
-          h[5] += (az.dw[k] * ay.dw[j]) * r;
-        }
-    }
-}
-
-// Scatter: dst[...] += (wx_i * (cy_j*cz_k)) * val  -- wy/wz are passed per op
-template <int DP>
-__device__ __forceinline__ void scatter_val(float* __restrict__ dst, const Axis<DP>& ax, const Axis<DP>& ay,
-                                            const Axis<DP>& az, float val) {
-  constexpr int NP = Axis<DP>::NP;
-#pragma unroll
-  for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
-#pragma unroll
-      for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
-          float* row = dst + az.off[k] + ay.off[j];
-          const float wv = (az.w[k] * ay.w[j]) * val;
-#pragma unroll
-          for (int i = 0; i < NP; ++i) PP_TAP(DP, ax, i) unsafeAtomicAdd(row + ax.off[i], ax.w[i] * wv);
-        }
-    }
-}
-
-// Scatter of a gradient triple: weight = g0 dwx wy wz + g1 wx dwy wz + g2 wx wy dwz
-template <int DP>
-__device__ __forceinline__ void scatter_grad(float* __restrict__ dst, const Axis<DP>& ax, const Axis<DP>& ay,
-                                             const Axis<DP>& az, float g0, float g1, float g2) {
-  constexpr int NP = Axis<DP>::NP;
-#pragma unroll
-  for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
-#pragma unroll
-      for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
-          float* row = dst + az.off[k] + ay.off[j];
-          const float cx = (az.w[k] * ay.w[j]) * g0;
-          const float cyz = (az.w[k] * ay.dw[j]) * g1 + (az.dw[k] * ay.w[j]) * g2;
-#pragma unroll
-          for (int i = 0; i < NP; ++i) PP_TAP(DP, ax, i)
-            unsafeAtomicAdd(row + ax.off[i], ax.dw[i] * cx + ax.w[i] * cyz);
-        }
-    }
-}
-
-// Write (or add, for the later channel groups of a channel-reducing op) a 3-vector.
-__device__ __forceinline__ void put3(float* o, long long n1, float a0, float a1, float a2, int accum) {
-  if (accum) { a0 += o[0]; a1 += o[n1]; a2 += o[2 * n1]; }
-  o[0] = a0; o[n1] = a1; o[2 * n1] = a2;
-}
 
 #define PP_KERNEL_PROLOGUE                                       \
   const int bb = blockIdx.y;                                       \
@@ -195,7 +26,7 @@ __global__ __launch_bounds__(256) void k_pull(KArgs a) {
     return;
   }
   const float* f = a.f + bb * a.f_bs;
-  for (int c = 0; c < a.nchan; ++c) out[c * a.out_cs] = gather_val<DP>(f + c * a.f_cs, ax, ay, az);
+  for (int c = 0; c < a.nchan; ++c) out[c * a.out_cs] = sample_val<DP>(f + c * a.f_cs, ax, ay, az);
 }
 
 template <int DP>
@@ -215,7 +46,7 @@ __global__ __launch_bounds__(256) void k_pullg(KArgs a) {
   const float* f = a.f + bb * a.f_bs;
   for (int c = 0; c < a.nchan; ++c) {
     float gx, gy, gz;
-    gather_grad<DP>(f + c * a.f_cs, ax, ay, az, gx, gy, gz);
+    sample_grad<DP>(f + c * a.f_cs, ax, ay, az, gx, gy, gz);
     float* o = out + c * a.out_cs;
     o[0] = gx; o[n1] = gy; o[2 * n1] = gz;
   }
@@ -236,7 +67,7 @@ __global__ __launch_bounds__(256) void k_pullh(KArgs a) {
   const float* f = a.f + bb * a.f_bs;
   for (int c = 0; c < a.nchan; ++c) {
     float h[6];
-    gather_hess<DP>(f + c * a.f_cs, ax, ay, az, h);
+    sample_hess<DP>(f + c * a.f_cs, ax, ay, az, h);
     float* o = out + c * a.out_cs;
     // column-major 3x3 (pushpull.c:129-134)
     o[0] = h[0]; o[4 * n1] = h[1]; o[8 * n1] = h[2];
@@ -290,7 +121,7 @@ __global__ __launch_bounds__(256) void k_pull_bwd(KArgs a) {
   float sx = 0.f, sy = 0.f, sz = 0.f;
   for (int c = 0; c < a.nchan; ++c) {
     float gx, gy, gz;
-    gather_grad<DP>(f + c * a.f_cs, ax, ay, az, gx, gy, gz);
+    sample_grad<DP>(f + c * a.f_cs, ax, ay, az, gx, gy, gz);
     const float dc = d[c * a.aux_cs];
     sx += dc * gx; sy += dc * gy; sz += dc * gz;
   }
@@ -314,7 +145,7 @@ __global__ __launch_bounds__(256) void k_pullg_bwd(KArgs a) {
   float s0 = 0.f, s1 = 0.f, s2 = 0.f;
   for (int c = 0; c < a.nchan; ++c) {
     float h[6];
-    gather_hess<DP>(f + c * a.f_cs, ax, ay, az, h);
+    sample_hess<DP>(f + c * a.f_cs, ax, ay, az, h);
     const float* dc = d + c * a.aux_cs;
     const float d0 = dc[0], d1 = dc[n1], d2 = dc[2 * n1];
     // H = [xx xy xz; xy yy yz; xz yz zz]
@@ -347,7 +178,7 @@ __global__ __launch_bounds__(256) void k_objgrad(KArgs a) {
       float sx = 0.f, sy = 0.f, sz = 0.f;
       for (int c = 0; c < a.nchan; ++c) {
         float val, gx, gy, gz;
-        gather_val_grad<DP>(f + c * a.f_cs, ax, ay, az, val, gx, gy, gz);
+        sample_val_grad<DP>(f + c * a.f_cs, ax, ay, az, val, gx, gy, gz);
         const float r = val - g[c * a.aux_cs];
         sx += r * gx; sy += r * gy; sz += r * gz;
         if (isfinite(r)) {

@@ -33,6 +33,7 @@ struct KArgs {
   long long out_cs, out_bs;  // strides of out
   long long out2_cs, out2_bs;
   int n1;                    // voxels in the phi grid
+  int d1x, d1y, d1z;         // phi grid (tiled kernels; n1 == d1x*d1y*d1z)
   int d0x, d0y, d0z;         // grid sampled by pull / scattered into by push
   int dpx, dpy, dpz;         // dp = degree + 1 per axis (used by the runtime-degree kernels)
   int bx, by, bz;            // boundary code per axis
@@ -45,6 +46,14 @@ struct KArgs {
 // Launch op on `stream`; dp is (DP,DP,DP) when isotropic, else the runtime path is used.
 // grid.y = nbatch. Returns hipSuccess or the launch error.
 hipError_t launch_op(Op op, const KArgs& a, int nbatch, hipStream_t stream, int block);
+
+// Tiled variants (pp_tiled.hip): one workgroup per tile of the phi grid, source
+// footprint staged in / scattered into LDS. Returns hipErrorNotSupported for an op or
+// degree they do not cover (the caller then uses launch_op).
+struct TileCfg {
+  int shape;  // index into the compiled tile shapes (0: 16x8x8, 1: 32x8x4, 2: 16x16x4)
+};
+hipError_t launch_tiled(Op op, const KArgs& a, int nbatch, hipStream_t stream, TileCfg cfg);
 
 // Fill n floats with v (graph-capturable; no hipMemsetAsync nodes).
 hipError_t launch_fill(float* p, long long n, float v, hipStream_t stream);

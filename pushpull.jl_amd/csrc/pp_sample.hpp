@@ -1,0 +1,265 @@
+// pp_sample.hpp -- per-sample gather / scatter building blocks shared by the
+// one-work-item-per-voxel kernels (pp_kernels.hip) and the tiled kernels (pp_tiled.hip).
+#pragma once
+#include "pp_bspline.hpp"
+#include "pp_kernels.hpp"
+
+namespace pp {
+
+constexpr int kXcds = 8;
+
+// Round-robin dispatch puts block b on XCD b % 8; give XCD k a contiguous chunk.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  const int per = nb / kXcds, rem = nb % kXcds;
+  const int x = b % kXcds, i = b / kXcds;
+  return (x < rem) ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
+}
+
+template <int DP, int NEED>
+__device__ __forceinline__ bool setup(const KArgs& a, const float* phi, int v, Axis<DP>& ax, Axis<DP>& ay,
+                                      Axis<DP>& az) {
+  const float x = phi[v] - 1.0f;
+  const float y = phi[v + a.n1] - 1.0f;
+  const float z = phi[v + 2 * (long long)a.n1] - 1.0f;
+  if (!voxok(x, y, z, a.ext, a.lx, a.ly, a.lz)) return false;
+  make_axis<DP, NEED>(x, a.dpx, a.d0x, 1, a.bx, ax);
+  make_axis<DP, NEED>(y, a.dpy, a.d0y, a.d0x, a.by, ay);
+  make_axis<DP, NEED>(z, a.dpz, a.d0z, a.d0x * a.d0y, a.bz, az);
+  return true;
+}
+
+#define PP_TAP(DP, ax, i) if (DP > 0 || (i) < (ax).n)
+
+// Value: sum_ijk wx_i wy_j wz_k f[...]
+template <int DP>
+__device__ __forceinline__ float gather_val(const float* __restrict__ f, const Axis<DP>& ax, const Axis<DP>& ay,
+                                            const Axis<DP>& az) {
+  constexpr int NP = Axis<DP>::NP;
+  float acc = 0.f;
+#pragma unroll
+  for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
+#pragma unroll
+      for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
+          const float* row = f + az.off[k] + ay.off[j];
+          float r = 0.f;
+#pragma unroll
+          for (int i = 0; i < NP; ++i) PP_TAP(DP, ax, i) r += ax.w[i] * row[ax.off[i]];
+          acc += (az.w[k] * ay.w[j]) * r;
+        }
+    }
+  return acc;
+}
+
+// Gradient: (d/dx, d/dy, d/dz)
+template <int DP>
+__device__ __forceinline__ void gather_grad(const float* __restrict__ f, const Axis<DP>& ax, const Axis<DP>& ay,
+                                            const Axis<DP>& az, float& gx, float& gy, float& gz) {
+  constexpr int NP = Axis<DP>::NP;
+  gx = gy = gz = 0.f;
+#pragma unroll
+  for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
+#pragma unroll
+      for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
+          const float* row = f + az.off[k] + ay.off[j];
+          float r = 0.f, rd = 0.f;
+#pragma unroll
+          for (int i = 0; i < NP; ++i) PP_TAP(DP, ax, i) {
+              const float v = row[ax.off[i]];
+              r += ax.w[i] * v;
+              rd += ax.dw[i] * v;
+            }
+          gx += (az.w[k] * ay.w[j]) * rd;
+          gy += (az.w[k] * ay.dw[j]) * r;
+          gz += (az.dw[k] * ay.w[j]) * r;
+        }
+    }
+}
+
+// Value and gradient from one footprint
+template <int DP>
+__device__ __forceinline__ void gather_val_grad(const float* __restrict__ f, const Axis<DP>& ax, const Axis<DP>& ay,
+                                                const Axis<DP>& az, float& val, float& gx, float& gy, float& gz) {
+  constexpr int NP = Axis<DP>::NP;
+  val = gx = gy = gz = 0.f;
+#pragma unroll
+  for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
+#pragma unroll
+      for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
+          const float* row = f + az.off[k] + ay.off[j];
+          float r = 0.f, rd = 0.f;
+#pragma unroll
+          for (int i = 0; i < NP; ++i) PP_TAP(DP, ax, i) {
+              const float v = row[ax.off[i]];
+              r += ax.w[i] * v;
+              rd += ax.dw[i] * v;
+            }
+          const float wyz = az.w[k] * ay.w[j];
+          val += wyz * r;
+          gx += wyz * rd;
+          gy += (az.w[k] * ay.dw[j]) * r;
+          gz += (az.dw[k] * ay.w[j]) * r;
+        }
+    }
+}
+
+// Hessian: xx, yy, zz, xy, xz, yz
+template <int DP>
+__device__ __forceinline__ void gather_hess(const float* __restrict__ f, const Axis<DP>& ax, const Axis<DP>& ay,
+                                            const Axis<DP>& az, float* h) {
+  constexpr int NP = Axis<DP>::NP;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) h[q] = 0.f;
+#pragma unroll
+  for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
+#pragma unroll
+      for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
+          const float* row = f + az.off[k] + ay.off[j];
+          float r = 0.f, rd = 0.f, rh = 0.f;
+#pragma unroll
+          for (int i = 0; i < NP; ++i) PP_TAP(DP, ax, i) {
+              const float v = row[ax.off[i]];
+              r += ax.w[i] * v;
+              rd += ax.dw[i] * v;
+              rh += ax.hw[i] * v;
+            }
+          h[0] += (az.w[k] * ay.w[j]) * rh;
+          h[1] += (az.w[k] * ay.hw[j]) * r;
+          h[2] += (az.hw[k] * ay.w[j]) * r;
+          h[3] += (az.w[k] * ay.dw[j]) * rd;
+          h[4] += (az.dw[k] * ay.w[j]) * rd;
+          h[5] += (az.dw[k] * ay.dw[j]) * r;
+        }
+    }
+}
+
+// ---- reference operation order, no FMA contraction --------------------------------
+// Used for samples with a coordinate on the large-coordinate path (|x| >= 2^20), where
+// the weights can be huge and cancel exactly in the reference's order
+// (artifacts/C/pushpull_dev.cu:7-26, 33-58, 91-116); the fast contraction above would
+// leave a rounding residue there.
+#pragma clang fp contract(off)
+template <int DP>
+__device__ __forceinline__ float dot_ref(const float* __restrict__ f, const Axis<DP>& ax, const Axis<DP>& ay,
+                                         const Axis<DP>& az, const float* a, const float* b, const float* c) {
+  constexpr int NP = Axis<DP>::NP;
+  float acc = 0.f;
+  for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
+      for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
+          const float wjj = c[k] * b[j];
+          const float* row = f + az.off[k] + ay.off[j];
+          for (int i = 0; i < NP; ++i) PP_TAP(DP, ax, i) acc = acc + a[i] * wjj * row[ax.off[i]];
+        }
+    }
+  return acc;
+}
+
+template <int DP>
+__device__ __forceinline__ void grad_ref(const float* __restrict__ f, const Axis<DP>& ax, const Axis<DP>& ay,
+                                         const Axis<DP>& az, float& g0, float& g1, float& g2) {
+  constexpr int NP = Axis<DP>::NP;
+  g0 = g1 = g2 = 0.f;
+  for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
+      for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
+          const float wx = az.w[k] * ay.w[j], wy = az.w[k] * ay.dw[j], wz = az.dw[k] * ay.w[j];
+          const float* row = f + az.off[k] + ay.off[j];
+          for (int i = 0; i < NP; ++i) PP_TAP(DP, ax, i) {
+              float t = row[ax.off[i]];
+              g0 = g0 + ax.dw[i] * wx * t;
+              t = t * ax.w[i];
+              g1 = g1 + wy * t;
+              g2 = g2 + wz * t;
+            }
+        }
+    }
+}
+#pragma clang fp contract(fast)
+
+template <int DP>
+__device__ __forceinline__ void hess_ref(const float* __restrict__ f, const Axis<DP>& ax, const Axis<DP>& ay,
+                                         const Axis<DP>& az, float* h) {
+  h[0] = dot_ref<DP>(f, ax, ay, az, ax.hw, ay.w, az.w);
+  h[1] = dot_ref<DP>(f, ax, ay, az, ax.w, ay.hw, az.w);
+  h[2] = dot_ref<DP>(f, ax, ay, az, ax.w, ay.w, az.hw);
+  h[3] = dot_ref<DP>(f, ax, ay, az, ax.dw, ay.dw, az.w);
+  h[4] = dot_ref<DP>(f, ax, ay, az, ax.dw, ay.w, az.dw);
+  h[5] = dot_ref<DP>(f, ax, ay, az, ax.w, ay.dw, az.dw);
+}
+
+template <int DP>
+__device__ __forceinline__ bool is_slow(const Axis<DP>& ax, const Axis<DP>& ay, const Axis<DP>& az) {
+  return !(ax.fast && ay.fast && az.fast);
+}
+
+// Value / gradient / value+gradient / Hessian of one sample, picking the contraction
+template <int DP>
+__device__ __forceinline__ float sample_val(const float* f, const Axis<DP>& ax, const Axis<DP>& ay,
+                                            const Axis<DP>& az) {
+  if (is_slow(ax, ay, az)) return dot_ref<DP>(f, ax, ay, az, ax.w, ay.w, az.w);
+  return gather_val<DP>(f, ax, ay, az);
+}
+template <int DP>
+__device__ __forceinline__ void sample_grad(const float* f, const Axis<DP>& ax, const Axis<DP>& ay,
+                                            const Axis<DP>& az, float& gx, float& gy, float& gz) {
+  if (is_slow(ax, ay, az)) grad_ref<DP>(f, ax, ay, az, gx, gy, gz);
+  else gather_grad<DP>(f, ax, ay, az, gx, gy, gz);
+}
+template <int DP>
+__device__ __forceinline__ void sample_val_grad(const float* f, const Axis<DP>& ax, const Axis<DP>& ay,
+                                                const Axis<DP>& az, float& v, float& gx, float& gy, float& gz) {
+  if (is_slow(ax, ay, az)) {
+    v = dot_ref<DP>(f, ax, ay, az, ax.w, ay.w, az.w);
+    grad_ref<DP>(f, ax, ay, az, gx, gy, gz);
+  } else {
+    gather_val_grad<DP>(f, ax, ay, az, v, gx, gy, gz);
+  }
+}
+template <int DP>
+__device__ __forceinline__ void sample_hess(const float* f, const Axis<DP>& ax, const Axis<DP>& ay,
+                                            const Axis<DP>& az, float* h) {
+  if (is_slow(ax, ay, az)) hess_ref<DP>(f, ax, ay, az, h);
+  else gather_hess<DP>(f, ax, ay, az, h);
+}
+
+// Scatter: dst[...] += (wx_i * (cy_j*cz_k)) * val  -- wy/wz are passed per op
+template <int DP>
+__device__ __forceinline__ void scatter_val(float* __restrict__ dst, const Axis<DP>& ax, const Axis<DP>& ay,
+                                            const Axis<DP>& az, float val) {
+  constexpr int NP = Axis<DP>::NP;
+#pragma unroll
+  for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
+#pragma unroll
+      for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
+          float* row = dst + az.off[k] + ay.off[j];
+          const float wv = (az.w[k] * ay.w[j]) * val;
+#pragma unroll
+          for (int i = 0; i < NP; ++i) PP_TAP(DP, ax, i) unsafeAtomicAdd(row + ax.off[i], ax.w[i] * wv);
+        }
+    }
+}
+
+// Scatter of a gradient triple: weight = g0 dwx wy wz + g1 wx dwy wz + g2 wx wy dwz
+template <int DP>
+__device__ __forceinline__ void scatter_grad(float* __restrict__ dst, const Axis<DP>& ax, const Axis<DP>& ay,
+                                             const Axis<DP>& az, float g0, float g1, float g2) {
+  constexpr int NP = Axis<DP>::NP;
+#pragma unroll
+  for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
+#pragma unroll
+      for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
+          float* row = dst + az.off[k] + ay.off[j];
+          const float cx = (az.w[k] * ay.w[j]) * g0;
+          const float cyz = (az.w[k] * ay.dw[j]) * g1 + (az.dw[k] * ay.w[j]) * g2;
+#pragma unroll
+          for (int i = 0; i < NP; ++i) PP_TAP(DP, ax, i)
+            unsafeAtomicAdd(row + ax.off[i], ax.dw[i] * cx + ax.w[i] * cyz);
+        }
+    }
+}
+
+// Write (or add, for the later channel groups of a channel-reducing op) a 3-vector.
+__device__ __forceinline__ void put3(float* o, long long n1, float a0, float a1, float a2, int accum) {
+  if (accum) { a0 += o[0]; a1 += o[n1]; a2 += o[2 * n1]; }
+  o[0] = a0; o[n1] = a1; o[2 * n1] = a2;
+}
+
+}  // namespace pp

@@ -1,0 +1,538 @@
+// pp_tiled.hip -- tiled gfx950 kernels for the hot ops (pull, push, push_grad, the phi
+// pullback of pull, and the fused matching objective).
+//
+// A 256-thread workgroup owns a TX x TY x TZ tile of the phi grid (TX*TY*TZ/256 samples
+// per thread). Its samples' knots are reduced to a bounding box in the *unfolded* index
+// space of the image grid; the box (<= CAP cells) lives in LDS:
+//   gathers  stage the box once (coalesced row loads, boundary fold and sign applied
+//            while staging), then every sample reads its dp^3 taps from LDS;
+//   scatters accumulate every tap with an LDS float atomic (ds_add_f32), then flush each
+//            non-zero cell with one folded global atomic -- a few global atomics per
+//            sample instead of dp^3, and none of the same-address contention of
+//            neighbouring samples hitting the same HBM lines.
+// Samples whose knot lies outside a window around the tile's centre sample (displaced
+// far away, sent outside the field of view, or on the large-coordinate path) are
+// handled individually through global memory, so a few outliers never inflate the box.
+// Tiles whose box would exceed CAP fall back to the global path entirely.
+#include "pp_sample.hpp"
+
+#include <climits>
+
+namespace pp {
+
+constexpr int kTB = 256;
+
+struct TileState {
+  int lo[3];      // unfolded image index of box cell (0,0,0), per axis
+  int b[3];       // box extent per axis
+  int cells;      // b[0]*b[1]*b[2]
+  int lds;        // 1 when the box is used
+  int ctr[3];     // window centre (knot of the tile's centre sample)
+  int org[3];     // tile origin in the phi grid
+  int red[6][4];  // per-wave partial min/max
+};
+
+template <int TX, int TY, int TZ>
+struct Shape {
+  static constexpr int N = TX * TY * TZ;
+  static constexpr int VPT = N / kTB;
+  static_assert(N % kTB == 0, "a tile is a whole number of workgroup passes");
+  // window half-width: a knot farther than this from the centre sample's is an outlier
+  static constexpr int WX = TX + 8, WY = TY + 8, WZ = TZ + 8;
+  // local index of the centre sample
+  static constexpr int LC = TX / 2 + TX * (TY / 2 + TY * (TZ / 2));
+};
+
+template <int VPT>
+struct Vox {
+  float x[VPT], y[VPT], z[VPT];
+  int v[VPT];
+  unsigned inside, valid, inl;  // bit q: position inside the phi grid / valid sample / in-window
+};
+
+__device__ __forceinline__ int wave_min(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+
+template <int DP>
+__device__ __forceinline__ int knot_fast(float x) {
+  return (int)ceilf(x - 0.5f * (float)DP);
+}
+
+__device__ __forceinline__ bool is_fast3(float x, float y, float z) {
+  return fabsf(x) < kFastLimit && fabsf(y) < kFastLimit && fabsf(z) < kFastLimit;
+}
+
+// Load the tile's samples, classify them and size the LDS box. Ends with a barrier.
+template <int DP, int TX, int TY, int TZ, int CAP>
+__device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restrict__ phi, int tile,
+                                           Vox<Shape<TX, TY, TZ>::VPT>& V, TileState& S) {
+  using SH = Shape<TX, TY, TZ>;
+  constexpr int VPT = SH::VPT;
+  const int ntx = (a.d1x + TX - 1) / TX, nty = (a.d1y + TY - 1) / TY;
+  const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+  const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
+  const int tid = threadIdx.x;
+  if (tid == 0) {  // fallback centre: the tile centre mapped by the grid-size ratio
+    S.org[0] = x0;
+    S.org[1] = y0;
+    S.org[2] = z0;
+    S.ctr[0] = (int)((long long)(x0 + TX / 2) * a.d0x / a.d1x) - (DP - 1) / 2;
+    S.ctr[1] = (int)((long long)(y0 + TY / 2) * a.d0y / a.d1y) - (DP - 1) / 2;
+    S.ctr[2] = (int)((long long)(z0 + TZ / 2) * a.d0z / a.d1z) - (DP - 1) / 2;
+  }
+  __syncthreads();
+  V.inside = 0;
+  V.valid = 0;
+  V.inl = 0;
+#pragma unroll
+  for (int q = 0; q < VPT; ++q) {
+    const int l = tid + kTB * q;
+    const int lx = l % TX, ly = (l / TX) % TY, lz = l / (TX * TY);
+    const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+    const bool in = gx < a.d1x && gy < a.d1y && gz < a.d1z;
+    const int v = in ? gx + a.d1x * (gy + a.d1y * gz) : 0;
+    V.v[q] = v;
+    float x = 0.f, y = 0.f, z = 0.f;
+    bool ok = false;
+    if (in) {
+      V.inside |= 1u << q;
+      x = phi[v] - 1.0f;
+      y = phi[v + a.n1] - 1.0f;
+      z = phi[v + 2 * (long long)a.n1] - 1.0f;
+      ok = voxok(x, y, z, a.ext, a.lx, a.ly, a.lz);
+    }
+    V.x[q] = x;
+    V.y[q] = y;
+    V.z[q] = z;
+    if (ok) V.valid |= 1u << q;
+    if (l == SH::LC && ok && is_fast3(x, y, z)) {
+      S.ctr[0] = knot_fast<DP>(x);
+      S.ctr[1] = knot_fast<DP>(y);
+      S.ctr[2] = knot_fast<DP>(z);
+    }
+  }
+  __syncthreads();
+  const int cx = S.ctr[0], cy = S.ctr[1], cz = S.ctr[2];
+  int mn0 = INT_MAX, mn1 = INT_MAX, mn2 = INT_MAX, mx0 = INT_MIN, mx1 = INT_MIN, mx2 = INT_MIN;
+#pragma unroll
+  for (int q = 0; q < VPT; ++q) {
+    if (!((V.valid >> q) & 1u) || !is_fast3(V.x[q], V.y[q], V.z[q])) continue;
+    const int ox = knot_fast<DP>(V.x[q]), oy = knot_fast<DP>(V.y[q]), oz = knot_fast<DP>(V.z[q]);
+    if (abs(ox - cx) > SH::WX || abs(oy - cy) > SH::WY || abs(oz - cz) > SH::WZ) continue;
+    V.inl |= 1u << q;
+    mn0 = min(mn0, ox); mx0 = max(mx0, ox);
+    mn1 = min(mn1, oy); mx1 = max(mx1, oy);
+    mn2 = min(mn2, oz); mx2 = max(mx2, oz);
+  }
+  mn0 = wave_min(mn0); mn1 = wave_min(mn1); mn2 = wave_min(mn2);
+  mx0 = wave_max(mx0); mx1 = wave_max(mx1); mx2 = wave_max(mx2);
+  const int wid = tid >> 6;
+  if ((tid & 63) == 0) {
+    S.red[0][wid] = mn0; S.red[1][wid] = mn1; S.red[2][wid] = mn2;
+    S.red[3][wid] = mx0; S.red[4][wid] = mx1; S.red[5][wid] = mx2;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    for (int w = 0; w < kTB / 64; ++w)
+      for (int q = 0; q < 3; ++q) {
+        lo[q] = min(lo[q], S.red[q][w]);
+        hi[q] = max(hi[q], S.red[3 + q][w]);
+      }
+    if (lo[0] > hi[0]) {  // no in-window sample
+      S.cells = 0;
+      S.lds = 0;
+    } else {
+      int cells = 1;
+      for (int q = 0; q < 3; ++q) {
+        S.lo[q] = lo[q];
+        S.b[q] = hi[q] - lo[q] + DP;
+        cells *= S.b[q];
+      }
+      S.cells = cells;
+      S.lds = cells <= CAP;
+    }
+  }
+  __syncthreads();
+}
+
+// Stage the box from image channel `src` (fold + boundary sign applied here).
+__device__ __forceinline__ void stage_box(float* box, const TileState& S, const float* __restrict__ src,
+                                          const KArgs& a) {
+  const int BX = S.b[0], BY = S.b[1], rows = S.b[1] * S.b[2];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int r = wave; r < rows; r += kTB / 64) {
+    bool ny, nz;
+    const int fy = fold32(a.by, S.lo[1] + r % BY, a.d0y, ny);
+    const int fz = fold32(a.bz, S.lo[2] + r / BY, a.d0z, nz);
+    const float* srow = src + a.d0x * (fy + a.d0y * fz);
+    float* brow = box + r * BX;
+    for (int c = lane; c < BX; c += 64) {
+      bool nx;
+      const int fx = fold32(a.bx, S.lo[0] + c, a.d0x, nx);
+      const float v = srow[fx];
+      brow[c] = (nx ^ ny ^ nz) ? -v : v;
+    }
+  }
+}
+
+// Flush the accumulated box into image channel `dst` with one global atomic per
+// non-zero cell (a cell nobody touched holds exactly +0 and adds nothing).
+__device__ __forceinline__ void flush_box(const float* box, const TileState& S, float* __restrict__ dst,
+                                          const KArgs& a) {
+  const int BX = S.b[0], BY = S.b[1], rows = S.b[1] * S.b[2];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int r = wave; r < rows; r += kTB / 64) {
+    bool ny, nz;
+    const int fy = fold32(a.by, S.lo[1] + r % BY, a.d0y, ny);
+    const int fz = fold32(a.bz, S.lo[2] + r / BY, a.d0z, nz);
+    float* drow = dst + a.d0x * (fy + a.d0y * fz);
+    const float* brow = box + r * BX;
+    for (int c = lane; c < BX; c += 64) {
+      const float v = brow[c];
+      if (v != 0.0f) {
+        bool nx;
+        const int fx = fold32(a.bx, S.lo[0] + c, a.d0x, nx);
+        unsafeAtomicAdd(drow + fx, (nx ^ ny ^ nz) ? -v : v);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void zero_box(float* box, int cells) {
+  for (int i = threadIdx.x; i < cells; i += kTB) box[i] = 0.0f;
+}
+
+// Unsigned, unfolded weights of one in-window sample and its box offset
+template <int DP>
+struct LSample {
+  float wx[DP], wy[DP], wz[DP], dx[DP], dy[DP], dz[DP];
+  int base;
+};
+
+template <int DP, bool DERIV>
+__device__ __forceinline__ void lsample(float x, float y, float z, const TileState& S, LSample<DP>& L) {
+  const int ox = knot_fast<DP>(x), oy = knot_fast<DP>(y), oz = knot_fast<DP>(z);
+  float h[DP];
+  weights_ct<DP>(x - (float)ox, L.wx, L.dx, h);
+  weights_ct<DP>(y - (float)oy, L.wy, L.dy, h);
+  weights_ct<DP>(z - (float)oz, L.wz, L.dz, h);
+  L.base = (ox - S.lo[0]) + S.b[0] * ((oy - S.lo[1]) + S.b[1] * (oz - S.lo[2]));
+}
+
+// Phi-grid index of sample q of this thread (in: inside the grid)
+template <int TX, int TY, int TZ>
+__device__ __forceinline__ int tile_voxel(const KArgs& a, const TileState& S, int q, bool& in) {
+  const int l = threadIdx.x + kTB * q;
+  const int gx = S.org[0] + l % TX, gy = S.org[1] + (l / TX) % TY, gz = S.org[2] + l / (TX * TY);
+  in = gx < a.d1x && gy < a.d1y && gz < a.d1z;
+  return in ? gx + a.d1x * (gy + a.d1y * gz) : 0;
+}
+
+__device__ __forceinline__ void load_x(const KArgs& a, const float* __restrict__ phi, int v, float& x, float& y,
+                                       float& z) {
+  x = phi[v] - 1.0f;
+  y = phi[v + a.n1] - 1.0f;
+  z = phi[v + 2 * (long long)a.n1] - 1.0f;
+}
+
+// ---- push / push_grad -------------------------------------------------------------
+template <int DP, int TX, int TY, int TZ, int CAP, bool GRAD>
+__global__ __launch_bounds__(kTB) void k_push_tile(KArgs a) {
+  constexpr int VPT = Shape<TX, TY, TZ>::VPT;
+  __shared__ float box[CAP];
+  __shared__ TileState S;
+  const int bb = blockIdx.y;
+  const float* phi = a.phi + bb * a.phi_bs;
+  unsigned valid, inl;
+  {
+    Vox<VPT> V;
+    tile_setup<DP, TX, TY, TZ, CAP>(a, phi, xcd_block(blockIdx.x, gridDim.x), V, S);
+    valid = V.valid;
+    inl = V.inl;
+  }
+  const bool lds = S.lds != 0;
+  const int BX = S.b[0], BXY = S.b[0] * S.b[1];
+  const long long n1 = a.n1;
+  for (int c = 0; c < a.nchan; ++c) {
+    const float* src = a.f + bb * a.f_bs + c * a.f_cs;
+    float* dst = a.out + bb * a.out_bs + c * a.out_cs;
+    if (lds) {
+      zero_box(box, S.cells);
+      __syncthreads();
+    }
+#pragma unroll 1
+    for (int q = 0; q < VPT; ++q) {
+      if (!((valid >> q) & 1u)) continue;
+      bool in;
+      const int v = tile_voxel<TX, TY, TZ>(a, S, q, in);
+      const float g0 = src[v];
+      const float g1 = GRAD ? src[v + n1] : 0.f;
+      const float g2 = GRAD ? src[v + 2 * n1] : 0.f;
+      if (!GRAD && !isfinite(g0)) continue;  // push skips non-finite values (pushpull.c:155)
+      float x, y, z;
+      load_x(a, phi, v, x, y, z);
+      if (lds && ((inl >> q) & 1u)) {
+        LSample<DP> L;
+        lsample<DP, GRAD>(x, y, z, S, L);
+#pragma unroll
+        for (int k = 0; k < DP; ++k)
+#pragma unroll
+          for (int j = 0; j < DP; ++j) {
+            float* row = box + L.base + BX * j + BXY * k;
+            if (GRAD) {
+              const float cx = (L.wz[k] * L.wy[j]) * g0;
+              const float cyz = (L.wz[k] * L.dy[j]) * g1 + (L.dz[k] * L.wy[j]) * g2;
+#pragma unroll
+              for (int i = 0; i < DP; ++i) atomicAdd(row + i, L.dx[i] * cx + L.wx[i] * cyz);
+            } else {
+              const float wv = (L.wz[k] * L.wy[j]) * g0;
+#pragma unroll
+              for (int i = 0; i < DP; ++i) atomicAdd(row + i, L.wx[i] * wv);
+            }
+          }
+      } else {
+        Axis<DP> ax, ay, az;
+        constexpr int NEED = GRAD ? (kW | kDW) : kW;
+        make_axis<DP, NEED>(x, DP, a.d0x, 1, a.bx, ax);
+        make_axis<DP, NEED>(y, DP, a.d0y, a.d0x, a.by, ay);
+        make_axis<DP, NEED>(z, DP, a.d0z, a.d0x * a.d0y, a.bz, az);
+        if (GRAD) scatter_grad<DP>(dst, ax, ay, az, g0, g1, g2);
+        else scatter_val<DP>(dst, ax, ay, az, g0);
+      }
+    }
+    if (lds) {
+      __syncthreads();
+      flush_box(box, S, dst, a);
+      __syncthreads();
+    }
+  }
+}
+
+// ---- gathers: pull (KIND 0), phi pullback of pull (1), fused objective (2) ----------
+template <int DP>
+__device__ __forceinline__ float lds_val(const float* box, const LSample<DP>& L, int BX, int BXY) {
+  float acc = 0.f;
+#pragma unroll
+  for (int k = 0; k < DP; ++k)
+#pragma unroll
+    for (int j = 0; j < DP; ++j) {
+      const float* row = box + L.base + BX * j + BXY * k;
+      float r = 0.f;
+#pragma unroll
+      for (int i = 0; i < DP; ++i) r += L.wx[i] * row[i];
+      acc += (L.wz[k] * L.wy[j]) * r;
+    }
+  return acc;
+}
+
+template <int DP, bool VAL>
+__device__ __forceinline__ void lds_grad(const float* box, const LSample<DP>& L, int BX, int BXY, float& val,
+                                         float& gx, float& gy, float& gz) {
+  val = gx = gy = gz = 0.f;
+#pragma unroll
+  for (int k = 0; k < DP; ++k)
+#pragma unroll
+    for (int j = 0; j < DP; ++j) {
+      const float* row = box + L.base + BX * j + BXY * k;
+      float r = 0.f, rd = 0.f;
+#pragma unroll
+      for (int i = 0; i < DP; ++i) {
+        const float v = row[i];
+        r += L.wx[i] * v;
+        rd += L.dx[i] * v;
+      }
+      const float wyz = L.wz[k] * L.wy[j];
+      if (VAL) val += wyz * r;
+      gx += wyz * rd;
+      gy += (L.wz[k] * L.dy[j]) * r;
+      gz += (L.dz[k] * L.wy[j]) * r;
+    }
+}
+
+template <int DP, int TX, int TY, int TZ, int CAP, int KIND>
+__global__ __launch_bounds__(kTB) void k_gather_tile(KArgs a) {
+  constexpr int N = Shape<TX, TY, TZ>::N;
+  constexpr int VPT = Shape<TX, TY, TZ>::VPT;
+  constexpr int CAP2 = KIND == 2 ? CAP : 1;
+  constexpr int NACC = KIND == 0 ? 1 : 3 * N;
+  __shared__ float box[CAP];
+  __shared__ float box2[CAP2];
+  __shared__ float acc[NACC];  // per-sample phi pullback, summed over channels
+  __shared__ TileState S;
+  __shared__ float wsum[kTB / 64];
+  const int bb = blockIdx.y;
+  const float* phi = a.phi + bb * a.phi_bs;
+  unsigned inside, valid, inl;
+  {
+    Vox<VPT> V;
+    tile_setup<DP, TX, TY, TZ, CAP>(a, phi, xcd_block(blockIdx.x, gridDim.x), V, S);
+    inside = V.inside;
+    valid = V.valid;
+    inl = V.inl;
+  }
+  const bool lds = S.lds != 0;
+  const int BX = S.b[0], BXY = S.b[0] * S.b[1];
+  const long long n1 = a.n1;
+  if (KIND != 0) {
+#pragma unroll
+    for (int q = 0; q < VPT; ++q) {
+      const int l = threadIdx.x + kTB * q;
+      acc[l] = acc[N + l] = acc[2 * N + l] = 0.f;
+    }
+  }
+  float lsum = 0.f;
+  for (int c = 0; c < a.nchan; ++c) {
+    const float* src = a.f + bb * a.f_bs + c * a.f_cs;
+    if (lds) {
+      stage_box(box, S, src, a);
+      if (KIND == 2) zero_box(box2, S.cells);
+      __syncthreads();
+    }
+#pragma unroll 1
+    for (int q = 0; q < VPT; ++q) {
+      if (!((inside >> q) & 1u)) continue;
+      bool in;
+      const int v = tile_voxel<TX, TY, TZ>(a, S, q, in);
+      if (!((valid >> q) & 1u)) {
+        if (KIND == 0) a.out[bb * a.out_bs + c * a.out_cs + v] = __builtin_nanf("");
+        continue;
+      }
+      float x, y, z;
+      load_x(a, phi, v, x, y, z);
+      const bool use_lds = lds && ((inl >> q) & 1u);
+      if (KIND == 0) {
+        float val;
+        if (use_lds) {
+          LSample<DP> L;
+          lsample<DP, false>(x, y, z, S, L);
+          val = lds_val<DP>(box, L, BX, BXY);
+        } else {
+          Axis<DP> ax, ay, az;
+          make_axis<DP, kW>(x, DP, a.d0x, 1, a.bx, ax);
+          make_axis<DP, kW>(y, DP, a.d0y, a.d0x, a.by, ay);
+          make_axis<DP, kW>(z, DP, a.d0z, a.d0x * a.d0y, a.bz, az);
+          val = sample_val<DP>(src, ax, ay, az);
+        }
+        a.out[bb * a.out_bs + c * a.out_cs + v] = val;
+        continue;
+      }
+      const float aux = a.aux[bb * a.aux_bs + c * a.aux_cs + v];
+      float val, gx, gy, gz, w;
+      if (use_lds) {
+        LSample<DP> L;
+        lsample<DP, true>(x, y, z, S, L);
+        lds_grad<DP, KIND == 2>(box, L, BX, BXY, val, gx, gy, gz);
+        w = (KIND == 1) ? aux : val - aux;
+        if (KIND == 2 && isfinite(w)) {
+#pragma unroll
+          for (int k = 0; k < DP; ++k)
+#pragma unroll
+            for (int j = 0; j < DP; ++j) {
+              float* row = box2 + L.base + BX * j + BXY * k;
+              const float wv = (L.wz[k] * L.wy[j]) * w;
+#pragma unroll
+              for (int i = 0; i < DP; ++i) atomicAdd(row + i, L.wx[i] * wv);
+            }
+        }
+      } else {
+        Axis<DP> ax, ay, az;
+        make_axis<DP, kW | kDW>(x, DP, a.d0x, 1, a.bx, ax);
+        make_axis<DP, kW | kDW>(y, DP, a.d0y, a.d0x, a.by, ay);
+        make_axis<DP, kW | kDW>(z, DP, a.d0z, a.d0x * a.d0y, a.bz, az);
+        if (KIND == 2) sample_val_grad<DP>(src, ax, ay, az, val, gx, gy, gz);
+        else sample_grad<DP>(src, ax, ay, az, gx, gy, gz);
+        w = (KIND == 1) ? aux : val - aux;
+        if (KIND == 2 && isfinite(w)) scatter_val<DP>(a.out2 + bb * a.out2_bs + c * a.out2_cs, ax, ay, az, w);
+      }
+      if (KIND == 2 && isfinite(w)) lsum += 0.5f * w * w;
+      const int l = threadIdx.x + kTB * q;
+      acc[l] += w * gx;
+      acc[N + l] += w * gy;
+      acc[2 * N + l] += w * gz;
+    }
+    if (lds) {
+      __syncthreads();
+      if (KIND == 2) {
+        flush_box(box2, S, a.out2 + bb * a.out2_bs + c * a.out2_cs, a);
+        __syncthreads();
+      }
+    }
+  }
+  if (KIND != 0) {
+    const float nan = __builtin_nanf("");
+#pragma unroll 1
+    for (int q = 0; q < VPT; ++q) {
+      if (!((inside >> q) & 1u)) continue;
+      bool in;
+      const int v = tile_voxel<TX, TY, TZ>(a, S, q, in);
+      const int l = threadIdx.x + kTB * q;
+      if ((valid >> q) & 1u) put3(a.out + bb * a.out_bs + v, n1, acc[l], acc[N + l], acc[2 * N + l], a.accum);
+      else put3(a.out + bb * a.out_bs + v, n1, nan, nan, nan, 0);
+    }
+  }
+  if (KIND == 2) {
+    for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = lsum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int w = 0; w < kTB / 64; ++w) t += (double)wsum[w];
+      unsafeAtomicAdd(a.red, t);
+    }
+  }
+}
+
+// ---- launcher ------------------------------------------------------------------
+// Box budget: CAP cells per box (32 KiB); the fused objective holds two boxes.
+constexpr int kCap = 8192;
+
+template <int TX, int TY, int TZ>
+static int tiles_of(const KArgs& a) {
+  return ((a.d1x + TX - 1) / TX) * ((a.d1y + TY - 1) / TY) * ((a.d1z + TZ - 1) / TZ);
+}
+
+template <int DP, int TX, int TY, int TZ>
+static hipError_t launch_shape(Op op, const KArgs& a, int nbatch, hipStream_t s) {
+  const dim3 grid((unsigned)tiles_of<TX, TY, TZ>(a), (unsigned)nbatch, 1), block(kTB);
+  switch (op) {
+    case OP_PULL: hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, kCap, 0>), grid, block, 0, s, a); break;
+    case OP_PULL_BWD: hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, kCap, 1>), grid, block, 0, s, a); break;
+    case OP_OBJGRAD: hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, kCap, 2>), grid, block, 0, s, a); break;
+    case OP_PUSH: hipLaunchKernelGGL((k_push_tile<DP, TX, TY, TZ, kCap, false>), grid, block, 0, s, a); break;
+    case OP_PUSHG: hipLaunchKernelGGL((k_push_tile<DP, TX, TY, TZ, kCap, true>), grid, block, 0, s, a); break;
+    default: return hipErrorNotSupported;
+  }
+  return hipGetLastError();
+}
+
+template <int DP>
+static hipError_t launch_deg(Op op, const KArgs& a, int nbatch, hipStream_t s, TileCfg cfg) {
+  switch (cfg.shape) {
+    case 1: return launch_shape<DP, 32, 8, 4>(op, a, nbatch, s);
+    case 2: return launch_shape<DP, 16, 16, 4>(op, a, nbatch, s);
+    default: return launch_shape<DP, 16, 8, 8>(op, a, nbatch, s);
+  }
+}
+
+hipError_t launch_tiled(Op op, const KArgs& a, int nbatch, hipStream_t s, TileCfg cfg) {
+  if (op != OP_PULL && op != OP_PULL_BWD && op != OP_OBJGRAD && op != OP_PUSH && op != OP_PUSHG)
+    return hipErrorNotSupported;
+  if (a.n1 <= 0 || nbatch <= 0) return hipSuccess;
+  if (a.dpx != a.dpy || a.dpx != a.dpz) return hipErrorNotSupported;
+  if ((long long)a.d1x * a.d1y * a.d1z != a.n1) return hipErrorNotSupported;
+  switch (a.dpx) {
+    case 2: return launch_deg<2>(op, a, nbatch, s, cfg);
+    case 3: return launch_deg<3>(op, a, nbatch, s, cfg);
+    case 4: return launch_deg<4>(op, a, nbatch, s, cfg);
+    default: return hipErrorNotSupported;
+  }
+}
+
+}  // namespace pp

@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(LIB_DIR, "libpushpull_hip.so")
 HEADER = os.path.join(os.path.dirname(os.path.dirname(_PKG)), "include", "pushpull_hip.h")
 
 PP_MAX_CHAN_BND = 64
+PP_OPT_TILING = 1
+PP_OPT_TILE_SHAPE = 2
 
 
 class PPDesc(ct.Structure):
@@ -57,6 +59,8 @@ _SIGS = {
     "pp_memset_zero": ([_P, ct.c_size_t, _P], ct.c_int),
     "pp_stream_sync": ([_P], ct.c_int),
     "pp_device_sync": ([], ct.c_int),
+    "pp_set_option": ([ct.c_int, ct.c_int], ct.c_int),
+    "pp_get_option": ([ct.c_int], ct.c_int),
     "pp_last_error": ([], ct.c_char_p),
     "pp_last_status": ([], ct.c_int),
     "pp_version": ([], ct.c_char_p),
@@ -101,3 +105,11 @@ def declared_functions(header: str = HEADER) -> list[str]:
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     names = re.findall(r"^\s*(?:const\s+)?(?:void|int|char)\s*\*?\s*(\w+)\s*\(", txt, flags=re.M)
     return names
+
+
+def set_option(key: int, value: int) -> None:
+    check(lib().pp_set_option(key, value), "pp_set_option")
+
+
+def get_option(key: int) -> int:
+    return int(lib().pp_get_option(key))

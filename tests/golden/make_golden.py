@@ -86,6 +86,14 @@ def _case(rng, d0, d1, deg, bnd, ext, sigma=1.5, specials=True, smooth=False):
     f1 = rng.standard_normal(d1).astype(np.float32)
     f1.reshape(-1)[rng.choice(f1.size, size=max(1, f1.size // 40), replace=False)] = np.nan
     g1 = rng.standard_normal((3,) + tuple(d1)).astype(np.float32)
+    # Samples with a coordinate beyond 2^20 get huge weights (up to ~1e19 once the knot
+    # saturates at INT64_MIN) whose scattered contributions cancel only in the
+    # reference's sequential order; any unordered (atomic) scatter differs. Those samples
+    # push the value 0: their knots, folds and addresses are still exercised.
+    x = phi.reshape(3, -1) - np.float32(1.0)
+    extreme = np.any(np.isfinite(x) & (np.abs(x) >= 2.0 ** 20), axis=0)
+    f1.reshape(-1)[extreme] = 0.0
+    g1.reshape(3, -1)[:, extreme] = 0.0
     return dict(f0=f0, phi=phi, f1=f1, g1=g1, deg=np.array(deg, np.int32), bnd=np.array(bnd, np.int32),
                 ext=np.array(ext, np.int32))
 

@@ -10,7 +10,12 @@ class S:
     def __init__(self, deg, bnd, ext):
         self.deg = tuple(int(d) for d in np.atleast_1d(deg)) if np.ndim(deg) else (int(deg),) * 3
         b = np.asarray(bnd)
-        self.bnd = [tuple(int(v) for v in b)] if b.ndim == 1 else [tuple(int(v) for v in r) for r in b]
+        if b.ndim == 0:
+            self.bnd = [(int(b),) * 3]
+        elif b.ndim == 1:
+            self.bnd = [tuple(int(v) for v in b)]
+        else:
+            self.bnd = [tuple(int(v) for v in r) for r in b]
         self.ext = int(ext)
 
 

@@ -26,6 +26,25 @@ def H(t):
     return t.detach().cpu().numpy()
 
 
+@pytest.fixture(params=["tiled", "per-sample"])
+def path(request):
+    """Run a test through the LDS-tiled kernels and through the one-work-item-per-sample
+    kernels (PP_OPT_TILING); the tiled kernels cover degrees 1-3 of pull, push,
+    push_grad, the pull pullback and the fused objective."""
+    old = pp._lib.get_option(pp._lib.PP_OPT_TILING)
+    pp._lib.set_option(pp._lib.PP_OPT_TILING, 1 if request.param == "tiled" else 0)
+    yield request.param
+    pp._lib.set_option(pp._lib.PP_OPT_TILING, old)
+
+
+@pytest.fixture(params=[0, 1, 2])
+def tile_shape(request):
+    old = pp._lib.get_option(pp._lib.PP_OPT_TILE_SHAPE)
+    pp._lib.set_option(pp._lib.PP_OPT_TILE_SHAPE, request.param)
+    yield request.param
+    pp._lib.set_option(pp._lib.PP_OPT_TILE_SHAPE, old)
+
+
 def _sett(c):
     return pp.Settings(tuple(int(v) for v in c["deg"]), tuple(int(v) for v in c["bnd"]), int(c["ext"]))
 
@@ -37,8 +56,20 @@ def _check(got, ref, what, tol=TOL):
     assert e < tol, f"{what}: rel err {e:.3g}"
 
 
+def _check_all(results):
+    """results: list of (what, got, ref); report every failing case at once."""
+    bad = []
+    for what, got, ref in results:
+        try:
+            _check(got, ref, what)
+        except AssertionError as ex:
+            bad.append(str(ex))
+    assert not bad, f"{len(bad)} failing: " + "; ".join(bad[:12])
+
+
 @pytest.mark.parametrize("op", ["pull", "pullg", "pullh", "push", "pushg"])
-def test_golden_tier_b(golden, op):
+def test_golden_tier_b(golden, op, path):
+    results = []
     for i, c in enumerate(golden):
         s = _sett(c)
         d0 = c["f0"].shape
@@ -54,7 +85,8 @@ def test_golden_tier_b(golden, op):
         else:  # no usable reference output (pushpull.c:185): the oracle's adjoint
             got = H(pp.push_grad(T(c["g1"]), phi, d0, s))
             ref = oracle.push_grad(c["g1"], c["phi"], d0, case_settings(c), prec="f64")[0, 0]
-        _check(got, ref, f"case {i} {op}")
+        results.append((f"case {i} deg {c['deg'].tolist()} {op}", got, ref))
+    _check_all(results)
 
 
 def test_golden_tier_a(golden):
@@ -107,7 +139,7 @@ def _volume_case(rng, d, sigma_f, maxdisp, out_frac=0.0):
 
 @pytest.mark.parametrize("deg", [1, 2, 3])
 @pytest.mark.parametrize("bnd", [0, 1])
-def test_config2_pull_push_adjoint(deg, bnd):
+def test_config2_pull_push_adjoint(deg, bnd, path):
     """192^3, smoothed noise, displacement up to 8 voxels, 5% sent outside the FOV."""
     rng = np.random.default_rng(100 + 10 * deg + bnd)
     d = (192, 192, 192)
@@ -133,7 +165,7 @@ def test_config2_pull_push_adjoint(deg, bnd):
     assert abs(lhs64 - rhs64) / (abs(lhs64) + abs(rhs64)) < 1e-12
 
 
-def test_config1_nan_mask_ext0():
+def test_config1_nan_mask_ext0(path):
     """64^3, U[0,1), trilinear, reflect, ext=0: the NaN mask must match exactly."""
     rng = np.random.default_rng(1)
     d = (64, 64, 64)
@@ -147,7 +179,7 @@ def test_config1_nan_mask_ext0():
 
 
 @pytest.mark.parametrize("deg", [1, 2, 3])
-def test_pull_grad_backward_and_hess(deg):
+def test_pull_grad_backward_and_hess(deg, path):
     rng = np.random.default_rng(40 + deg)
     d = (40, 36, 44)
     f, phi = _volume_case(rng, d, 2.0, 5.0, out_frac=0.02)
@@ -172,7 +204,7 @@ def test_pull_grad_backward_and_hess(deg):
     _check(H(pp.push_grad(T(g3), tphi, d, s)), oracle.push_grad(g3, phi[None], d, so, prec="f64"), "push_grad")
 
 
-def test_push_grad_is_adjoint_of_pull_grad():
+def test_push_grad_is_adjoint_of_pull_grad(path):
     rng = np.random.default_rng(5)
     d = (48, 40, 32)
     f, phi = _volume_case(rng, d, 2.0, 6.0, out_frac=0.05)
@@ -184,7 +216,7 @@ def test_push_grad_is_adjoint_of_pull_grad():
         assert abs(lhs - rhs) / (abs(lhs) + abs(rhs)) < TOL, deg
 
 
-def test_batches_channels_per_channel_bounds():
+def test_batches_channels_per_channel_bounds(path, tile_shape):
     rng = np.random.default_rng(9)
     d0, d1 = (20, 18, 22), (16, 19, 21)
     B, C = 2, 3
@@ -202,7 +234,7 @@ def test_batches_channels_per_channel_bounds():
 
 
 @pytest.mark.parametrize("deg", [1, 2, 3])
-def test_objective_grad_fused_matches_composition(deg):
+def test_objective_grad_fused_matches_composition(deg, path, tile_shape):
     rng = np.random.default_rng(70 + deg)
     d = (36, 40, 44)
     f, phi = _volume_case(rng, d, 2.0, 6.0)
