@@ -119,6 +119,48 @@ int pp_pull_grad_backward(const pp_desc *d, float *dphi, const float *phi, const
 int pp_objective_grad(const pp_desc *d, float *dphi, float *df0, double *loss, const float *phi,
                       const float *f0, const float *g, int accumulate, void *stream);
 
+/* ---- batches of uneven volumes --------------------------------------------
+ * No counterpart in the reference, whose wrappers loop over same-size (channel, batch)
+ * slices with one native call each (src/pushpull_cpu.jl:42-51, ext/pushpull_gpu.jl:37-48).
+ * A plan describes many volumes of different sizes, channel counts and degrees; running it
+ * is one persistent launch over all their tiles (plus one zeroing launch for the
+ * accumulating ops), on kernels only, so it can be captured into a HIP graph. */
+enum {
+  PP_OP_PULL = 0,          /* out = pull(f, phi)                                   */
+  PP_OP_PUSH = 3,          /* out = push(f, phi) (zeroed first)                    */
+  PP_OP_PUSH_GRAD = 4,     /* out = push_grad(f, phi) (zeroed first)               */
+  PP_OP_PULL_BACKWARD = 5, /* out = dphi = sum_c aux_c grad pull(f)_c              */
+  PP_OP_OBJECTIVE = 7      /* out = dphi, out2 = dL/df (zeroed first), loss[0] = L */
+};
+
+typedef struct pp_volume {
+  const float *f;    /* image (d0,C) for the pull family; pushed values (d1,C) or (d1,3,C) for push */
+  const float *phi;  /* (d1,3), 1-based */
+  const float *aux;  /* PULL_BACKWARD: upstream gradient (d1,C); OBJECTIVE: target g (d1,C) */
+  float *out;        /* PULL: (d1,C); PUSH/PUSH_GRAD: (d0,C); PULL_BACKWARD/OBJECTIVE: dphi (d1,3) */
+  float *out2;       /* OBJECTIVE: dL/df (d0,C) */
+  double *loss;      /* OBJECTIVE: 0.5*sum r^2 of this volume (device, one double) */
+  int64_t d0[3];     /* image grid (X,Y,Z) */
+  int64_t d1[3];     /* phi grid (X,Y,Z) */
+  int32_t nchan;
+  int32_t deg;       /* isotropic degree 0..4 */
+  int32_t ext;
+  int32_t bnd[3];    /* shared by the volume's channels */
+} pp_volume;
+
+typedef struct pp_batch_plan pp_batch_plan;
+/* Validates the volumes and uploads their descriptors (allocates device memory once). */
+int pp_batch_plan_create(int op, const pp_volume *vols, int nvol, pp_batch_plan **plan);
+/* Stream-ordered; allocates nothing; capturable. */
+int pp_batch_run(const pp_batch_plan *plan, void *stream);
+int pp_batch_plan_destroy(pp_batch_plan *plan);
+
+/* ---- HIP graph capture of any sequence of the calls above (stream must not be NULL) ---- */
+int pp_graph_begin(void *stream);
+int pp_graph_end(void *stream, void **graph_exec);
+int pp_graph_launch(void *graph_exec, void *stream);
+int pp_graph_destroy(void *graph_exec);
+
 /* ---- device memory helpers, so a Julia caller needs no GPU package ---------- */
 int pp_malloc(void **p, size_t bytes);
 int pp_free(void *p);
@@ -132,10 +174,24 @@ int pp_device_sync(void);
 enum {
   PP_OPT_TILING = 1,     /* 1 (default): LDS-tiled kernels for pull, push, push_grad, the pull
                             pullback and the fused objective; 0: one work-item per sample */
-  PP_OPT_TILE_SHAPE = 2  /* tile of the phi grid: 0 = 16x8x8 (default), 1 = 32x8x4, 2 = 16x16x4 */
+  PP_OPT_TILE_SHAPE = 2, /* tile of the phi grid: 0 = 16x8x8 (default), 1 = 32x8x4, 2 = 16x16x4 */
+  PP_OPT_USE_TUNED = 3   /* 1 (default): prefer the autotuner's choice when it has one */
 };
 int pp_set_option(int key, int value);
 int pp_get_option(int key);
+
+/* ---- autotuner (no counterpart in the reference) ----------------------------
+ * Times every kernel variant of `op` (PP_OP_PULL, _PUSH, _PUSH_GRAD, _PULL_BACKWARD,
+ * _OBJECTIVE) on the caller's buffers -- one warm-up and `reps` timed calls each, HIP
+ * events on `stream` -- and records the fastest for (op, degree, log2 of the phi-grid
+ * size); later calls with that key use it. The buffers are those of the op's Tier B
+ * function (out2/loss only for the objective, aux for the pullback and the objective).
+ * *variant receives 0 (one work-item per sample) or 1+k (tiled, tile shape k). */
+int pp_tune(int op, const pp_desc *d, float *out, float *out2, double *loss, const float *phi, const float *f,
+            const float *aux, void *stream, int reps, int *variant);
+int pp_tune_save(const char *path);  /* text table: op dp log2(voxels) variant */
+int pp_tune_load(const char *path);
+int pp_tune_clear(void);
 
 /* ---- diagnostics ---------------------------------------------------------- */
 const char *pp_last_error(void);  /* message of the last failure on this thread */

@@ -7,7 +7,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -43,17 +45,42 @@ constexpr long long kMaxVox = (1LL << 31) - 1;  // per-volume voxel count (32-bi
 constexpr long long kMaxDim = 1LL << 29;        // per-axis size (2*d must fit an int)
 constexpr int kBlock = 256;
 
-// Kernel selection (pp_set_option): tiled LDS kernels on/off and their tile shape.
-std::atomic<int> g_tiling{1};
-std::atomic<int> g_shape{0};
+// ---- kernel selection --------------------------------------------------------------
+// Variants: 0 = one work-item per sample; 1 + k = tiled with tile shape k.
+constexpr int kVariants = 4;
+std::atomic<int> g_tiling{1};  // PP_OPT_TILING
+std::atomic<int> g_shape{0};   // PP_OPT_TILE_SHAPE (default when nothing is tuned)
+std::atomic<int> g_tuned{1};   // PP_OPT_USE_TUNED
+thread_local int t_force = -1; // variant forced while pp_tune times candidates
 
-hipError_t launch(pp::Op op, const pp::KArgs& a, int nbatch, hipStream_t s) {
-  if (g_tiling.load(std::memory_order_relaxed)) {
-    pp::TileCfg cfg{g_shape.load(std::memory_order_relaxed)};
-    hipError_t e = pp::launch_tiled(op, a, nbatch, s, cfg);
+// Autotuner table: (op, dp, log2 of the phi-grid size) -> variant.
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int>, int> g_tune;
+
+int size_class(long long n) {
+  int c = 0;
+  while ((1LL << (c + 1)) <= n) ++c;
+  return c;
+}
+
+hipError_t launch_variant(pp::Op op, const pp::KArgs& a, int nbatch, hipStream_t s, int variant) {
+  if (variant > 0) {
+    hipError_t e = pp::launch_tiled(op, a, nbatch, s, pp::TileCfg{variant - 1});
     if (e != hipErrorNotSupported) return e;
   }
   return pp::launch_op(op, a, nbatch, s, kBlock);
+}
+
+hipError_t launch(pp::Op op, const pp::KArgs& a, int nbatch, hipStream_t s) {
+  if (t_force >= 0) return launch_variant(op, a, nbatch, s, t_force);
+  if (!g_tiling.load(std::memory_order_relaxed)) return launch_variant(op, a, nbatch, s, 0);
+  if (g_tuned.load(std::memory_order_relaxed)) {
+    const bool iso = a.dpx == a.dpy && a.dpx == a.dpz;
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto it = g_tune.find({(int)op, iso ? a.dpx : 0, size_class(a.n1)});
+    if (it != g_tune.end()) return launch_variant(op, a, nbatch, s, it->second);
+  }
+  return launch_variant(op, a, nbatch, s, 1 + g_shape.load(std::memory_order_relaxed));
 }
 
 struct Geo {
@@ -384,6 +411,140 @@ int pp_objective_grad(const pp_desc* d, float* dphi, float* df0, double* loss, c
   return run(pp::OP_OBJGRAD, d, g, a, S(s));
 }
 
+// ---- batches of uneven volumes -----------------------------------------------------
+struct pp_batch_plan {
+  pp::Op op;
+  int nvol;
+  int total;  // tiles over all volumes
+  int grid;   // persistent workgroups
+  pp::BatchVol* dvols;
+};
+
+int pp_batch_plan_create(int op, const pp_volume* vols, int nvol, pp_batch_plan** plan) {
+  if (!plan) return fail(PP_EINVAL, "null plan pointer");
+  *plan = nullptr;
+  if (!vols || nvol < 1) return fail(PP_EINVAL, "need at least one volume");
+  if (op != PP_OP_PULL && op != PP_OP_PUSH && op != PP_OP_PUSH_GRAD && op != PP_OP_PULL_BACKWARD &&
+      op != PP_OP_OBJECTIVE)
+    return fail(PP_EINVAL, "unsupported batch op %d", op);
+  std::vector<pp::BatchVol> hv((size_t)nvol);
+  long long total = 0;
+  for (int i = 0; i < nvol; ++i) {
+    const pp_volume& v = vols[i];
+    long long n0 = 1, n1 = 1;
+    for (int q = 0; q < 3; ++q) {
+      if (v.d0[q] < 1 || v.d1[q] < 1 || v.d0[q] > kMaxDim || v.d1[q] > kMaxDim)
+        return fail(PP_EINVAL, "volume %d: grid size out of range on axis %d", i, q);
+      n0 *= v.d0[q];
+      n1 *= v.d1[q];
+      if (v.bnd[q] < 0 || v.bnd[q] > 2) return fail(PP_EINVAL, "volume %d: boundary code %d", i, v.bnd[q]);
+    }
+    if (n0 > kMaxVox || n1 > kMaxVox) return fail(PP_ERANGE, "volume %d holds more than 2^31-1 voxels", i);
+    if (v.nchan < 1) return fail(PP_EINVAL, "volume %d: nchan must be >= 1", i);
+    if (v.deg < 0 || v.deg > 4) return fail(PP_EINVAL, "volume %d: degree %d outside 0..4", i, v.deg);
+    if (!v.f || !v.phi || !v.out) return fail(PP_EINVAL, "volume %d: null array", i);
+    if ((op == PP_OP_PULL_BACKWARD || op == PP_OP_OBJECTIVE) && !v.aux) return fail(PP_EINVAL, "volume %d: null aux", i);
+    if (op == PP_OP_OBJECTIVE && (!v.out2 || !v.loss)) return fail(PP_EINVAL, "volume %d: null out2/loss", i);
+    pp::KArgs& a = hv[i].a;
+    std::memset(&a, 0, sizeof a);
+    a.n1 = (int)n1;
+    a.d1x = (int)v.d1[0];
+    a.d1y = (int)v.d1[1];
+    a.d1z = (int)v.d1[2];
+    a.d0x = (int)v.d0[0];
+    a.d0y = (int)v.d0[1];
+    a.d0z = (int)v.d0[2];
+    a.dpx = a.dpy = a.dpz = v.deg + 1;
+    a.bx = v.bnd[0];
+    a.by = v.bnd[1];
+    a.bz = v.bnd[2];
+    a.ext = v.ext != 0;
+    a.lx = (float)v.d0[0] - 0.99f;
+    a.ly = (float)v.d0[1] - 0.99f;
+    a.lz = (float)v.d0[2] - 0.99f;
+    a.nchan = v.nchan;
+    a.phi = v.phi;
+    a.phi_bs = 3 * n1;
+    a.f = v.f;
+    a.out = v.out;
+    hv[i].zero_n = 0;
+    switch (op) {
+      case PP_OP_PULL: a.f_cs = n0; a.out_cs = n1; break;
+      case PP_OP_PUSH: a.f_cs = n1; a.out_cs = n0; hv[i].zero_n = n0 * v.nchan; break;
+      case PP_OP_PUSH_GRAD: a.f_cs = 3 * n1; a.out_cs = n0; hv[i].zero_n = n0 * v.nchan; break;
+      case PP_OP_PULL_BACKWARD: a.f_cs = n0; a.aux = v.aux; a.aux_cs = n1; break;
+      default:
+        a.f_cs = n0;
+        a.aux = v.aux;
+        a.aux_cs = n1;
+        a.out2 = v.out2;
+        a.out2_cs = n0;
+        a.red = v.loss;
+        hv[i].zero_n = n0 * v.nchan;
+        break;
+    }
+    hv[i].dp = v.deg + 1;
+    hv[i].tile0 = (int)total;
+    total += pp::batch_tiles(a);
+    if (total > kMaxVox) return fail(PP_ERANGE, "too many tiles in one batch");
+  }
+  pp::BatchVol* d = nullptr;
+  hipError_t e = hipMalloc((void**)&d, sizeof(pp::BatchVol) * (size_t)nvol);
+  if (e != hipSuccess) return fail(PP_ENOMEM, "hipMalloc: %s", hipGetErrorString(e));
+  e = hipMemcpy(d, hv.data(), sizeof(pp::BatchVol) * (size_t)nvol, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(d);
+    return hip_fail(e, "upload batch descriptors");
+  }
+  *plan = new pp_batch_plan{static_cast<pp::Op>(op), nvol, (int)total, pp::batch_grid(static_cast<pp::Op>(op)), d};
+  return ok();
+}
+
+int pp_batch_run(const pp_batch_plan* plan, void* s) {
+  if (!plan) return fail(PP_EINVAL, "null plan");
+  hipError_t e = pp::launch_batch(plan->op, plan->dvols, plan->nvol, plan->total, plan->grid, true, S(s));
+  return e == hipSuccess ? ok() : hip_fail(e, "batch launch");
+}
+
+int pp_batch_plan_destroy(pp_batch_plan* plan) {
+  if (!plan) return ok();
+  hipError_t e = hipFree(plan->dvols);
+  delete plan;
+  return e == hipSuccess ? ok() : hip_fail(e, "hipFree");
+}
+
+// ---- HIP graphs --------------------------------------------------------------------
+int pp_graph_begin(void* s) {
+  if (!s) return fail(PP_EINVAL, "graph capture needs a non-default stream");
+  hipError_t e = hipStreamBeginCapture(S(s), hipStreamCaptureModeThreadLocal);
+  return e == hipSuccess ? ok() : hip_fail(e, "hipStreamBeginCapture");
+}
+
+int pp_graph_end(void* s, void** exec) {
+  if (!s || !exec) return fail(PP_EINVAL, "null stream or output");
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(S(s), &g);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+  *exec = x;
+  return ok();
+}
+
+int pp_graph_launch(void* exec, void* s) {
+  if (!exec) return fail(PP_EINVAL, "null graph");
+  hipError_t e = hipGraphLaunch(reinterpret_cast<hipGraphExec_t>(exec), S(s));
+  return e == hipSuccess ? ok() : hip_fail(e, "hipGraphLaunch");
+}
+
+int pp_graph_destroy(void* exec) {
+  if (!exec) return ok();
+  hipError_t e = hipGraphExecDestroy(reinterpret_cast<hipGraphExec_t>(exec));
+  return e == hipSuccess ? ok() : hip_fail(e, "hipGraphExecDestroy");
+}
+
 int pp_malloc(void** p, size_t bytes) {
   if (!p) return fail(PP_EINVAL, "null pointer");
   hipError_t e = hipMalloc(p, bytes);
@@ -425,6 +586,9 @@ int pp_set_option(int key, int value) {
       if (value < 0 || value > 2) return fail(PP_EINVAL, "tile shape %d not in 0..2", value);
       g_shape.store(value);
       return ok();
+    case PP_OPT_USE_TUNED:
+      g_tuned.store(value != 0);
+      return ok();
     default:
       return fail(PP_EINVAL, "unknown option %d", key);
   }
@@ -434,8 +598,100 @@ int pp_get_option(int key) {
   switch (key) {
     case PP_OPT_TILING: return g_tiling.load();
     case PP_OPT_TILE_SHAPE: return g_shape.load();
+    case PP_OPT_USE_TUNED: return g_tuned.load();
     default: return fail(PP_EINVAL, "unknown option %d", key);
   }
+}
+
+// ---- autotuner -----------------------------------------------------------------
+int pp_tune(int op, const pp_desc* d, float* out, float* out2, double* loss, const float* phi, const float* f,
+            const float* aux, void* s, int reps, int* variant) {
+  Geo g;
+  if (int r = check_desc(d, g)) return r;
+  if (reps < 1) reps = 3;
+  auto call = [&]() -> int {
+    switch (op) {
+      case PP_OP_PULL: return pp_pull(d, out, phi, f, s);
+      case PP_OP_PUSH: return pp_push(d, out, phi, f, 0, s);
+      case PP_OP_PUSH_GRAD: return pp_push_grad(d, out, phi, f, 0, s);
+      case PP_OP_PULL_BACKWARD: return pp_pull_backward(d, out, phi, f, aux, s);
+      case PP_OP_OBJECTIVE: return pp_objective_grad(d, out, out2, loss, phi, f, aux, 0, s);
+      default: return fail(PP_EINVAL, "op %d has a single kernel variant", op);
+    }
+  };
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return fail(PP_EDEVICE, "hipEventCreate");
+  float best_ms = 0.f;
+  int best = -1, rc = PP_OK;
+  for (int v = 0; v < kVariants && rc == PP_OK; ++v) {
+    t_force = v;
+    rc = call();  // warm-up (also compiles nothing: all variants are prebuilt)
+    if (rc != PP_OK) break;
+    hipEventRecord(e0, S(s));
+    for (int i = 0; i < reps && rc == PP_OK; ++i) rc = call();
+    hipEventRecord(e1, S(s));
+    if (hipEventSynchronize(e1) != hipSuccess) rc = fail(PP_EDEVICE, "hipEventSynchronize");
+    float ms = 0.f;
+    if (rc == PP_OK && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && (best < 0 || ms < best_ms)) {
+      best_ms = ms;
+      best = v;
+    }
+  }
+  t_force = -1;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != PP_OK) return rc;
+  const bool iso = d->deg[0] == d->deg[1] && d->deg[0] == d->deg[2];
+  const int internal = op;  // PP_OP_* codes equal the internal op numbers
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune[{internal, iso ? d->deg[0] + 1 : 0, size_class(g.n1)}] = best;
+  }
+  if (variant) *variant = best;
+  return ok();
+}
+
+int pp_tune_clear(void) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  g_tune.clear();
+  return ok();
+}
+
+int pp_tune_save(const char* path) {
+  if (!path) return fail(PP_EINVAL, "null path");
+  FILE* fp = std::fopen(path, "w");
+  if (!fp) return fail(PP_EINVAL, "cannot write %s", path);
+  std::fprintf(fp, "# pushpull_hip tuning table: op dp log2(voxels) variant\n");
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    for (const auto& kv : g_tune)
+      std::fprintf(fp, "%d %d %d %d\n", std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), kv.second);
+  }
+  std::fclose(fp);
+  return ok();
+}
+
+int pp_tune_load(const char* path) {
+  if (!path) return fail(PP_EINVAL, "null path");
+  FILE* fp = std::fopen(path, "r");
+  if (!fp) return fail(PP_EINVAL, "cannot read %s", path);
+  char line[256];
+  int n = 0;
+  std::map<std::tuple<int, int, int>, int> t;
+  while (std::fgets(line, sizeof line, fp)) {
+    if (line[0] == '#') continue;
+    int op, dp, sc, v;
+    if (std::sscanf(line, "%d %d %d %d", &op, &dp, &sc, &v) != 4 || v < 0 || v >= kVariants) {
+      std::fclose(fp);
+      return fail(PP_EINVAL, "%s: malformed line %d", path, n + 1);
+    }
+    t[{op, dp, sc}] = v;
+    ++n;
+  }
+  std::fclose(fp);
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  for (const auto& kv : t) g_tune[kv.first] = kv.second;
+  return ok();
 }
 
 const char* pp_last_error(void) { return t_err.c_str(); }

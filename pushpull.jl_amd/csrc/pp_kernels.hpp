@@ -55,6 +55,18 @@ struct TileCfg {
 };
 hipError_t launch_tiled(Op op, const KArgs& a, int nbatch, hipStream_t stream, TileCfg cfg);
 
+// One volume of a batched launch (device-resident array, built by the batch plan).
+struct BatchVol {
+  KArgs a;          // one volume, all its channels; pointers already at the volume
+  int dp;           // isotropic dp = degree + 1
+  int tile0;        // first tile number of this volume
+  long long zero_n; // floats zeroed before a push / objective launch
+};
+int batch_tiles(const KArgs& a);  // tiles of one volume (16x8x8 tiles of the phi grid)
+int batch_grid(Op op);            // workgroups that fit on the device at once (multiple of 8)
+hipError_t launch_batch(Op op, const BatchVol* dvols, int nvol, int total_tiles, int grid, bool zero,
+                        hipStream_t stream);
+
 // Fill n floats with v (graph-capturable; no hipMemsetAsync nodes).
 hipError_t launch_fill(float* p, long long n, float v, hipStream_t stream);
 hipError_t launch_fill_d(double* p, long long n, double v, hipStream_t stream);

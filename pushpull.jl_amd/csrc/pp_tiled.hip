@@ -243,17 +243,16 @@ __device__ __forceinline__ void load_x(const KArgs& a, const float* __restrict__
 }
 
 // ---- push / push_grad -------------------------------------------------------------
+// One tile of push (GRAD=false) or push_grad (GRAD=true) for batch index bb. The
+// caller provides the LDS box (CAP floats) and the tile state.
 template <int DP, int TX, int TY, int TZ, int CAP, bool GRAD>
-__global__ __launch_bounds__(kTB) void k_push_tile(KArgs a) {
+__device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, float* box, TileState& S) {
   constexpr int VPT = Shape<TX, TY, TZ>::VPT;
-  __shared__ float box[CAP];
-  __shared__ TileState S;
-  const int bb = blockIdx.y;
   const float* phi = a.phi + bb * a.phi_bs;
   unsigned valid, inl;
   {
     Vox<VPT> V;
-    tile_setup<DP, TX, TY, TZ, CAP>(a, phi, xcd_block(blockIdx.x, gridDim.x), V, S);
+    tile_setup<DP, TX, TY, TZ, CAP>(a, phi, tile, V, S);
     valid = V.valid;
     inl = V.inl;
   }
@@ -313,6 +312,14 @@ __global__ __launch_bounds__(kTB) void k_push_tile(KArgs a) {
       __syncthreads();
     }
   }
+  __syncthreads();  // the tile state may be reused by the next tile
+}
+
+template <int DP, int TX, int TY, int TZ, int CAP, bool GRAD>
+__global__ __launch_bounds__(kTB) void k_push_tile(KArgs a) {
+  __shared__ float box[CAP];
+  __shared__ TileState S;
+  push_tile<DP, TX, TY, TZ, CAP, GRAD>(a, xcd_block(blockIdx.x, gridDim.x), blockIdx.y, box, S);
 }
 
 // ---- gathers: pull (KIND 0), phi pullback of pull (1), fused objective (2) ----------
@@ -356,23 +363,26 @@ __device__ __forceinline__ void lds_grad(const float* box, const LSample<DP>& L,
     }
 }
 
+// LDS a gather kind needs, in floats: box, second box (objective), per-sample pullback
+template <int TX, int TY, int TZ, int CAP, int KIND>
+struct GatherLds {
+  static constexpr int CAP2 = KIND == 2 ? CAP : 1;
+  static constexpr int NACC = KIND == 0 ? 1 : 3 * Shape<TX, TY, TZ>::N;
+};
+
+// One tile of pull (KIND 0), the pull pullback (1) or the fused objective (2).
+// box: CAP floats; box2: CAP floats for KIND 2; acc: 3*N floats for KIND 1/2;
+// wsum: kTB/64 floats.
 template <int DP, int TX, int TY, int TZ, int CAP, int KIND>
-__global__ __launch_bounds__(kTB) void k_gather_tile(KArgs a) {
+__device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, float* box, float* box2, float* acc,
+                                            float* wsum, TileState& S) {
   constexpr int N = Shape<TX, TY, TZ>::N;
   constexpr int VPT = Shape<TX, TY, TZ>::VPT;
-  constexpr int CAP2 = KIND == 2 ? CAP : 1;
-  constexpr int NACC = KIND == 0 ? 1 : 3 * N;
-  __shared__ float box[CAP];
-  __shared__ float box2[CAP2];
-  __shared__ float acc[NACC];  // per-sample phi pullback, summed over channels
-  __shared__ TileState S;
-  __shared__ float wsum[kTB / 64];
-  const int bb = blockIdx.y;
   const float* phi = a.phi + bb * a.phi_bs;
   unsigned inside, valid, inl;
   {
     Vox<VPT> V;
-    tile_setup<DP, TX, TY, TZ, CAP>(a, phi, xcd_block(blockIdx.x, gridDim.x), V, S);
+    tile_setup<DP, TX, TY, TZ, CAP>(a, phi, tile, V, S);
     inside = V.inside;
     valid = V.valid;
     inl = V.inl;
@@ -424,7 +434,7 @@ __global__ __launch_bounds__(kTB) void k_gather_tile(KArgs a) {
         continue;
       }
       const float aux = a.aux[bb * a.aux_bs + c * a.aux_cs + v];
-      float val, gx, gy, gz, w;
+      float val = 0.f, gx, gy, gz, w;
       if (use_lds) {
         LSample<DP> L;
         lsample<DP, true>(x, y, z, S, L);
@@ -487,6 +497,18 @@ __global__ __launch_bounds__(kTB) void k_gather_tile(KArgs a) {
       unsafeAtomicAdd(a.red, t);
     }
   }
+  __syncthreads();  // the tile state and boxes may be reused by the next tile
+}
+
+template <int DP, int TX, int TY, int TZ, int CAP, int KIND>
+__global__ __launch_bounds__(kTB) void k_gather_tile(KArgs a) {
+  using G = GatherLds<TX, TY, TZ, CAP, KIND>;
+  __shared__ float box[CAP];
+  __shared__ float box2[G::CAP2];
+  __shared__ float acc[G::NACC];  // per-sample phi pullback, summed over channels
+  __shared__ TileState S;
+  __shared__ float wsum[kTB / 64];
+  gather_tile<DP, TX, TY, TZ, CAP, KIND>(a, xcd_block(blockIdx.x, gridDim.x), blockIdx.y, box, box2, acc, wsum, S);
 }
 
 // ---- launcher ------------------------------------------------------------------
@@ -519,6 +541,102 @@ static hipError_t launch_deg(Op op, const KArgs& a, int nbatch, hipStream_t s, T
     case 2: return launch_shape<DP, 16, 16, 4>(op, a, nbatch, s);
     default: return launch_shape<DP, 16, 8, 8>(op, a, nbatch, s);
   }
+}
+
+// ---- batched persistent launch over volumes of uneven size and degree -------------
+// Tiles of all volumes are numbered consecutively (BatchVol::tile0 is the prefix sum).
+// A grid that fits on the device at once walks them; each XCD takes a contiguous
+// range of tile numbers so neighbouring tiles share its L2. Degrees 1-3 use the LDS
+// tile bodies; degrees 0 and 4 run the same bodies with an empty box budget, i.e. per
+// sample through global memory.
+constexpr int kBTX = 16, kBTY = 8, kBTZ = 8;
+
+template <int DP, int KIND>
+__device__ __forceinline__ void batch_tile(const KArgs& a, int tile, float* box, float* box2, float* acc, float* wsum,
+                                           TileState& S) {
+  constexpr int CAPE = (DP >= 2 && DP <= 4) ? kCap : 0;
+  if (KIND <= 2) gather_tile<DP, kBTX, kBTY, kBTZ, CAPE, KIND>(a, tile, 0, box, box2, acc, wsum, S);
+  else push_tile<DP, kBTX, kBTY, kBTZ, CAPE, KIND == 4>(a, tile, 0, box, S);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kTB) void k_batch(const BatchVol* __restrict__ vols, int nvol, int total) {
+  using G = GatherLds<kBTX, kBTY, kBTZ, kCap, KIND <= 2 ? KIND : 0>;
+  __shared__ float box[kCap];
+  __shared__ float box2[G::CAP2];
+  __shared__ float acc[G::NACC];
+  __shared__ TileState S;
+  __shared__ float wsum[kTB / 64];
+  const int nb = gridDim.x / kXcds;  // the host launches a multiple of 8 workgroups
+  const int x = blockIdx.x % kXcds, i = blockIdx.x / kXcds;
+  const int per = (total + kXcds - 1) / kXcds;
+  const int t_end = min(total, (x + 1) * per);
+  for (int t = x * per + i; t < t_end; t += nb) {
+    int lo = 0, hi = nvol - 1;  // last volume whose first tile is <= t
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (vols[mid].tile0 <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const KArgs a = vols[lo].a;
+    const int tile = t - vols[lo].tile0;
+    switch (vols[lo].dp) {
+      case 1: batch_tile<1, KIND>(a, tile, box, box2, acc, wsum, S); break;
+      case 2: batch_tile<2, KIND>(a, tile, box, box2, acc, wsum, S); break;
+      case 3: batch_tile<3, KIND>(a, tile, box, box2, acc, wsum, S); break;
+      case 4: batch_tile<4, KIND>(a, tile, box, box2, acc, wsum, S); break;
+      default: batch_tile<5, KIND>(a, tile, box, box2, acc, wsum, S); break;
+    }
+  }
+}
+
+// Zero what a batched push / objective accumulates into (graph-capturable).
+__global__ void k_batch_zero(const BatchVol* __restrict__ vols, int kind) {
+  const BatchVol& b = vols[blockIdx.y];
+  float* p = (kind == OP_OBJGRAD) ? b.a.out2 : b.a.out;
+  const long long n = b.zero_n;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = 0.0f;
+  if (kind == OP_OBJGRAD && blockIdx.x == 0 && threadIdx.x == 0) b.a.red[0] = 0.0;
+}
+
+int batch_tiles(const KArgs& a) { return tiles_of<kBTX, kBTY, kBTZ>(a); }
+
+hipError_t launch_batch(Op op, const BatchVol* dvols, int nvol, int total, int grid, bool zero, hipStream_t s) {
+  if (nvol <= 0 || total <= 0) return hipSuccess;
+  if (zero && (op == OP_PUSH || op == OP_PUSHG || op == OP_OBJGRAD)) {
+    hipLaunchKernelGGL(k_batch_zero, dim3(512, (unsigned)nvol), dim3(256), 0, s, dvols, (int)op);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  const dim3 g((unsigned)grid), b(kTB);
+  switch (op) {
+    case OP_PULL: hipLaunchKernelGGL(k_batch<0>, g, b, 0, s, dvols, nvol, total); break;
+    case OP_PULL_BWD: hipLaunchKernelGGL(k_batch<1>, g, b, 0, s, dvols, nvol, total); break;
+    case OP_OBJGRAD: hipLaunchKernelGGL(k_batch<2>, g, b, 0, s, dvols, nvol, total); break;
+    case OP_PUSH: hipLaunchKernelGGL(k_batch<3>, g, b, 0, s, dvols, nvol, total); break;
+    case OP_PUSHG: hipLaunchKernelGGL(k_batch<4>, g, b, 0, s, dvols, nvol, total); break;
+    default: return hipErrorNotSupported;
+  }
+  return hipGetLastError();
+}
+
+int batch_grid(Op op) {
+  int per_cu = 0;
+  hipError_t e = hipErrorNotSupported;
+  switch (op) {
+    case OP_PULL: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<0>, kTB, 0); break;
+    case OP_PULL_BWD: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<1>, kTB, 0); break;
+    case OP_OBJGRAD: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<2>, kTB, 0); break;
+    case OP_PUSH: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<3>, kTB, 0); break;
+    case OP_PUSHG: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<4>, kTB, 0); break;
+    default: break;
+  }
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess || per_cu < 1) per_cu = 1;
+  const int g = per_cu * cus;
+  return ((g + kXcds - 1) / kXcds) * kXcds;
 }
 
 hipError_t launch_tiled(Op op, const KArgs& a, int nbatch, hipStream_t s, TileCfg cfg) {

@@ -16,6 +16,7 @@ HEADER = os.path.join(os.path.dirname(os.path.dirname(_PKG)), "include", "pushpu
 PP_MAX_CHAN_BND = 64
 PP_OPT_TILING = 1
 PP_OPT_TILE_SHAPE = 2
+PP_OPT_USE_TUNED = 3
 
 
 class PPDesc(ct.Structure):
@@ -61,6 +62,10 @@ _SIGS = {
     "pp_device_sync": ([], ct.c_int),
     "pp_set_option": ([ct.c_int, ct.c_int], ct.c_int),
     "pp_get_option": ([ct.c_int], ct.c_int),
+    "pp_tune": ([_P, _P, _P, _P, _P, _P, _P, _P, _P, ct.c_int, ct.POINTER(ct.c_int)], ct.c_int),
+    "pp_tune_save": ([ct.c_char_p], ct.c_int),
+    "pp_tune_load": ([ct.c_char_p], ct.c_int),
+    "pp_tune_clear": ([], ct.c_int),
     "pp_last_error": ([], ct.c_char_p),
     "pp_last_status": ([], ct.c_int),
     "pp_version": ([], ct.c_char_p),

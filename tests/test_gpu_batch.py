@@ -1,0 +1,169 @@
+"""Batched uneven volumes (one persistent launch) and HIP graph replay (needs a GPU).
+
+Checks the batch against the per-volume loop of single Tier B calls and against the
+FP64 oracle, and a captured graph against eager execution on every replay.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import S, identity, rel_err, same_nan_mask, smooth_field
+
+torch = pytest.importorskip("torch")
+pp = pytest.importorskip("pushpull")
+from pushpull import batch as ppb  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+TOL = 1e-5
+
+
+def T(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+def H(t):
+    return t.detach().cpu().numpy()
+
+
+def _volumes(seed, n=10, lo=12, hi=40):
+    rng = np.random.default_rng(seed)
+    vols = []
+    for i in range(n):
+        d = tuple(int(v) for v in rng.integers(lo, hi, size=3))
+        C = int(rng.integers(1, 5))
+        deg = int(rng.choice([0, 1, 2, 3, 4])) if i % 3 == 0 else int(rng.choice([1, 3]))
+        f = np.stack([smooth_field(rng, d, 1.5, 1.0)[0] for _ in range(C)]).astype(np.float32)
+        phi = identity(d) + smooth_field(rng, d, 3.0, 4.0)
+        if i % 4 == 1:  # a few samples far outside
+            flat = phi.reshape(3, -1)
+            idx = rng.choice(flat.shape[1], size=flat.shape[1] // 30, replace=False)
+            flat[0, idx] = rng.uniform(-20, d[2] + 20, size=idx.size).astype(np.float32)
+        aux = rng.standard_normal((C,) + d).astype(np.float32)
+        bnd = tuple(int(v) for v in rng.integers(0, 3, size=3))
+        vols.append(dict(f=f, phi=phi, aux=aux, deg=deg, bnd=bnd, ext=int(i % 5 != 2), d=d, C=C))
+    return vols
+
+
+def _dev(v, **extra):
+    return dict(f=T(v["f"]), phi=T(v["phi"]), aux=T(v["aux"]), deg=v["deg"], bnd=v["bnd"], ext=v["ext"], **extra)
+
+
+def _check(got, ref, what):
+    assert same_nan_mask(got, ref), f"{what}: NaN mask differs"
+    e = rel_err(got, ref)
+    assert e < TOL, f"{what}: rel err {e:.3g}"
+
+
+def test_batch_pull_and_pullback_match_oracle():
+    vols = _volumes(1)
+    plan = ppb.BatchPlan("pull", [_dev(v) for v in vols])
+    outs = plan.run()
+    torch.cuda.synchronize()
+    for i, (v, o) in enumerate(zip(vols, outs)):
+        s = S(v["deg"], v["bnd"], v["ext"])
+        _check(H(o["out"]), oracle.pull(v["f"][None], v["phi"][None], s, prec="f64")[0], f"pull vol {i}")
+    plan = ppb.BatchPlan("pull_backward", [_dev(v) for v in vols])
+    outs = plan.run()
+    torch.cuda.synchronize()
+    for i, (v, o) in enumerate(zip(vols, outs)):
+        s = S(v["deg"], v["bnd"], v["ext"])
+        ref = oracle.backward_phi(v["f"][None], v["phi"][None], v["aux"][None], s)[0]
+        _check(H(o["out"]), ref, f"pullback vol {i}")
+
+
+def test_batch_push_and_push_grad_match_oracle():
+    vols = _volumes(2)
+    rng = np.random.default_rng(0)
+    dev = []
+    for v in vols:
+        v["dim"] = tuple(int(x) for x in np.array(v["d"]) + rng.integers(-3, 4, size=3))
+        v["g3"] = rng.standard_normal((v["C"], 3) + v["d"]).astype(np.float32)
+        dev.append(dict(f=T(v["aux"]), phi=T(v["phi"]), deg=v["deg"], bnd=v["bnd"], ext=v["ext"], dim=v["dim"]))
+    outs = ppb.BatchPlan("push", dev).run()
+    torch.cuda.synchronize()
+    for i, (v, o) in enumerate(zip(vols, outs)):
+        s = S(v["deg"], v["bnd"], v["ext"])
+        _check(H(o["out"]), oracle.push(v["aux"][None], v["phi"][None], v["dim"], s, prec="f64")[0], f"push vol {i}")
+    dev = [dict(f=T(v["g3"]), phi=T(v["phi"]), deg=v["deg"], bnd=v["bnd"], ext=v["ext"], dim=v["dim"]) for v in vols]
+    outs = ppb.BatchPlan("push_grad", dev).run()
+    torch.cuda.synchronize()
+    for i, (v, o) in enumerate(zip(vols, outs)):
+        s = S(v["deg"], v["bnd"], v["ext"])
+        ref = oracle.push_grad(v["g3"][None], v["phi"][None], v["dim"], s, prec="f64")[0]
+        _check(H(o["out"]), ref, f"push_grad vol {i}")
+
+
+def test_batch_objective_matches_per_volume_calls():
+    vols = _volumes(3)
+    plan = ppb.BatchPlan("objective", [_dev(v) for v in vols])
+    outs = plan.run()
+    torch.cuda.synchronize()
+    for i, (v, o) in enumerate(zip(vols, outs)):
+        s = pp.Settings(v["deg"], v["bnd"], v["ext"])
+        loss, dphi, df = pp.objective_grad(T(v["f"][None]), T(v["phi"][None]), T(v["aux"][None]), s)
+        _check(H(o["out"]), H(dphi[0]), f"objective dphi vol {i}")
+        _check(H(o["out2"]), H(df[0]), f"objective df vol {i}")
+        assert abs(float(o["loss"]) - float(loss)) <= TOL * abs(float(loss)), i
+
+
+def test_graph_replays_match_eager():
+    vols = _volumes(4, n=6)
+    plan = ppb.BatchPlan("objective", [_dev(v) for v in vols])
+    eager = plan.run()
+    torch.cuda.synchronize()
+    ref = [(H(o["out"]).copy(), H(o["out2"]).copy(), float(o["loss"])) for o in eager]
+    g = ppb.Graph(lambda s: plan.run(s))
+    for rep in range(4):
+        for o in plan.vols:  # scribble over the outputs between replays
+            o["out"].fill_(123.0)
+            o["out2"].fill_(-7.0)
+            o["loss"].fill_(5.0)
+        torch.cuda.synchronize()
+        g.launch()
+        g.stream.synchronize()
+        for i, (o, r) in enumerate(zip(plan.vols, ref)):
+            _check(H(o["out"]), r[0], f"replay {rep} dphi vol {i}")
+            _check(H(o["out2"]), r[1], f"replay {rep} df vol {i}")
+            assert abs(float(o["loss"]) - r[2]) <= TOL * abs(r[2])
+    g.close()
+    plan.close()
+
+
+def test_graph_of_tier_b_calls():
+    """A captured pull + push + pullback sequence replays like the eager one."""
+    import ctypes as ct
+
+    from pushpull import _lib, ops
+
+    rng = np.random.default_rng(8)
+    d = (40, 48, 36)
+    f = T(smooth_field(rng, d, 2.0, 1.0)[0])
+    phi = T(identity(d) + smooth_field(rng, d, 4.0, 5.0))
+    delta = T(rng.standard_normal(d).astype(np.float32))
+    outs = [torch.empty(d, device=DEV), torch.empty(d, device=DEV), torch.empty((3,) + d, device=DEV)]
+    desc = ops._desc(pp.Settings(3, 1, 1), d, d, 1, 1)
+    L = _lib.lib()
+    P = lambda t: ct.c_void_p(t.data_ptr())  # noqa: E731
+
+    def seq(stream):
+        s = ct.c_void_p(stream.cuda_stream)
+        _lib.check(L.pp_pull(ct.byref(desc), P(outs[0]), P(phi), P(f), s), "pull")
+        _lib.check(L.pp_push(ct.byref(desc), P(outs[1]), P(phi), P(delta), 0, s), "push")
+        _lib.check(L.pp_pull_backward(ct.byref(desc), P(outs[2]), P(phi), P(f), P(delta), s), "bwd")
+
+    st = torch.cuda.Stream()
+    with torch.cuda.stream(st):
+        seq(st)
+    st.synchronize()
+    ref = [H(o).copy() for o in outs]
+    g = ppb.Graph(seq, stream=st)
+    for rep in range(3):
+        for o in outs:
+            o.fill_(float("nan"))
+        torch.cuda.synchronize()
+        g.launch()
+        st.synchronize()
+        for o, r in zip(outs, ref):
+            _check(H(o), r, f"graph replay {rep}")
+    g.close()

@@ -32,9 +32,12 @@ def path(request):
     kernels (PP_OPT_TILING); the tiled kernels cover degrees 1-3 of pull, push,
     push_grad, the pull pullback and the fused objective."""
     old = pp._lib.get_option(pp._lib.PP_OPT_TILING)
+    old_t = pp._lib.get_option(pp._lib.PP_OPT_USE_TUNED)
     pp._lib.set_option(pp._lib.PP_OPT_TILING, 1 if request.param == "tiled" else 0)
+    pp._lib.set_option(pp._lib.PP_OPT_USE_TUNED, 0)
     yield request.param
     pp._lib.set_option(pp._lib.PP_OPT_TILING, old)
+    pp._lib.set_option(pp._lib.PP_OPT_USE_TUNED, old_t)
 
 
 @pytest.fixture(params=[0, 1, 2])

@@ -154,3 +154,36 @@ def test_shape_assertions_mirror_julia():
                 pp.Settings(1, [(1, 1, 1), (0, 0, 0)]))
     with pytest.raises(TypeError):
         pp.pull(f.astype(np.float64), np.zeros((3, 4, 4, 4), np.float32), s)
+
+
+def test_tune_table_file_format(tmp_path):
+    L = _lib.lib()
+    p = tmp_path / "t.txt"
+    p.write_text("# op dp log2 variant\n0 4 24 2\n3 2 20 0\n")
+    assert L.pp_tune_clear() == 0
+    assert L.pp_tune_load(str(p).encode()) == 0
+    q = tmp_path / "u.txt"
+    assert L.pp_tune_save(str(q).encode()) == 0
+    rows = sorted(l.split() for l in q.read_text().splitlines() if not l.startswith("#"))
+    assert rows == [["0", "4", "24", "2"], ["3", "2", "20", "0"]]
+    bad = tmp_path / "bad.txt"
+    bad.write_text("0 4 24 9\n")
+    assert L.pp_tune_load(str(bad).encode()) == -1 and "malformed" in _lib.last_error()
+    assert L.pp_tune_clear() == 0
+
+
+def test_batch_plan_validation_without_gpu():
+    import ctypes as ct
+    from pushpull import batch as ppb
+
+    L = ppb._sigs()
+    h = ct.c_void_p()
+    assert L.pp_batch_plan_create(0, None, 0, ct.byref(h)) == -1
+    v = (ppb.PPVolume * 1)()
+    v[0].nchan, v[0].deg = 1, 7
+    for q in range(3):
+        v[0].d0[q] = v[0].d1[q] = 4
+    v[0].f = v[0].phi = v[0].out = 16
+    assert L.pp_batch_plan_create(0, ct.cast(v, ct.c_void_p), 1, ct.byref(h)) == -1
+    assert "degree" in _lib.last_error()
+    assert L.pp_batch_plan_create(9, ct.cast(v, ct.c_void_p), 1, ct.byref(h)) == -1
