@@ -33,18 +33,28 @@ HBM_PEAK_GBS = 8000.0
 BYTES = {"pull": 20, "push": 20, "backward": 32}  # per voxel at C = 1
 
 
+def _progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def _smooth_noise(torch, shape, sigma, gen, dev):
-    """Gaussian-smoothed standard-normal noise (separable, +-4 sigma, reflect)."""
+    """Gaussian-smoothed standard-normal noise (separable, +-4 sigma, reflect), as a sum
+    of shifted slices per axis (elementwise kernels only, no per-shape tuning)."""
     x = torch.randn(shape, generator=gen, device=dev, dtype=torch.float32)
     r = int(4 * sigma + 0.5)
-    k = torch.exp(-0.5 * (torch.arange(-r, r + 1, device=dev, dtype=torch.float32) / sigma) ** 2)
-    k = (k / k.sum()).view(1, 1, -1)
+    k = np.exp(-0.5 * (np.arange(-r, r + 1) / sigma) ** 2)
+    k = (k / k.sum()).astype(np.float32)
     v = x.reshape((-1,) + tuple(shape[-3:]))
     for ax in (1, 2, 3):
-        v = v.movedim(ax, -1)
-        s = v.shape
-        w = torch.nn.functional.pad(v.reshape(-1, 1, s[-1]), (r, r), mode="reflect")
-        v = torch.nn.functional.conv1d(w, k).reshape(s).movedim(-1, ax)
+        n = v.shape[ax]
+        idx = np.arange(-r, n + r)
+        idx = np.where(idx < 0, -idx - 1, idx)             # half-sample reflection
+        idx = np.where(idx >= n, 2 * n - 1 - idx, idx) % n  # (n > r is assumed)
+        p = v.index_select(ax, torch.from_numpy(idx).to(dev))
+        out = torch.zeros_like(v)
+        for t in range(2 * r + 1):
+            out.add_(p.narrow(ax, t, n), alpha=float(k[t]))
+        v = out
     return v.reshape(shape).contiguous()
 
 
@@ -106,6 +116,175 @@ def cpu_baseline(f, phi, delta, budget_s=20.0):
                       f"degrees 1-3, {t_total:.1f} s of single-thread CPU work"}
 
 
+def _time_ms(torch, fn, reps, stream):
+    """Mean ms of fn() over reps calls, HIP events on `stream`."""
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def bench_batched(torch, pp, reps=5, seed=4):
+    """24 volumes of uneven size (X 96-224, Y 96-256, Z 96-192), 1-4 channels, degree 1
+    or 3, each with its own smooth deformation (max 5 voxels): the fused objective (pull,
+    residual, push, pullback) for all volumes as one persistent launch replayed from a HIP
+    graph, against a per-volume loop of the same fused call and against the per-volume
+    loop of separate pull / residual / push / pullback calls."""
+    from pushpull import _lib, ops
+    from pushpull import batch as ppb
+
+    rng = np.random.default_rng(seed)
+    dev = torch.device("cuda:0")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    vols, descs = [], []
+    for _ in range(24):
+        X, Y, Z = int(rng.integers(96, 225)), int(rng.integers(96, 257)), int(rng.integers(96, 193))
+        C, deg = int(rng.integers(1, 5)), int(rng.choice([1, 3]))
+        d = (Z, Y, X)
+        f = _smooth_noise(torch, (C,) + d, 2.0, gen, dev)
+        g = _smooth_noise(torch, (C,) + d, 2.0, gen, dev)
+        u = _smooth_noise(torch, (3,) + d, 4.0, gen, dev)
+        u *= 5.0 / u.abs().max()
+        phi = (torch.from_numpy(pp.id(dim=(X, Y, Z))).to(dev) + u).contiguous()
+        vols.append(dict(f=f, phi=phi, aux=g, deg=deg, bnd=(1, 1, 1), ext=1, C=C, d=d))
+        descs.append(ops._desc(pp.Settings(deg, 1, 1), d, d, C, 1))
+    chan_vox = sum(v["C"] * int(np.prod(v["d"])) for v in vols)
+    _progress(f"batched: {len(vols)} volumes, {chan_vox / 1e6:.0f} M channel-voxels")
+    plan = ppb.BatchPlan("objective", vols)
+    graph = ppb.Graph(lambda s: plan.run(s))
+    st = graph.stream
+    L = _lib.lib()
+    P = lambda t: ct.c_void_p(t.data_ptr())  # noqa: E731
+    sp = ct.c_void_p(st.cuda_stream)
+    lo = [dict(dphi=torch.empty_like(v["phi"]), df=torch.empty_like(v["f"]),
+               loss=torch.zeros((), dtype=torch.float64, device=dev), f1=torch.empty_like(v["aux"]),
+               r=torch.empty_like(v["aux"])) for v in vols]
+
+    def loop_fused():
+        for v, dd, o in zip(vols, descs, lo):
+            _lib.check(L.pp_objective_grad(ct.byref(dd), P(o["dphi"]), P(o["df"]), P(o["loss"]), P(v["phi"]),
+                                           P(v["f"]), P(v["aux"]), 0, sp), "objective")
+
+    def loop_unfused():
+        for v, dd, o in zip(vols, descs, lo):
+            _lib.check(L.pp_pull(ct.byref(dd), P(o["f1"]), P(v["phi"]), P(v["f"]), sp), "pull")
+            torch.sub(o["f1"], v["aux"], out=o["r"])
+            _lib.check(L.pp_push(ct.byref(dd), P(o["df"]), P(v["phi"]), P(o["r"]), 0, sp), "push")
+            _lib.check(L.pp_pull_backward(ct.byref(dd), P(o["dphi"]), P(v["phi"]), P(v["f"]), P(o["r"]), sp), "bwd")
+
+    with torch.cuda.stream(st):
+        graph.launch()
+        loop_fused()
+        st.synchronize()
+        t_graph = _time_ms(torch, graph.launch, reps, st)
+        t_fused = _time_ms(torch, loop_fused, reps, st)
+        t_unfused = _time_ms(torch, loop_unfused, reps, st)
+        graph.launch()
+        loop_fused()
+        st.synchronize()
+    worst = 0.0
+    for v, o in zip(plan.vols, lo):
+        for a, b in ((v["out"], o["dphi"]), (v["out2"], o["df"])):
+            worst = max(worst, float((a.double() - b.double()).norm() / b.double().norm()))
+        worst = max(worst, abs(float(v["loss"]) - float(o["loss"])) / abs(float(o["loss"])))
+    graph.close()
+    plan.close()
+    gv = lambda ms: round(chan_vox / (ms * 1e-3) / 1e9, 3)  # noqa: E731
+    return {"volumes": 24, "channel_voxels": chan_vox,
+            "batched_graph_ms": round(t_graph, 3), "loop_fused_ms": round(t_fused, 3),
+            "loop_unfused_ms": round(t_unfused, 3),
+            "Gvoxel_s": {"batched_graph": gv(t_graph), "loop_fused": gv(t_fused), "loop_unfused": gv(t_unfused)},
+            "speedup_vs_loop_unfused": round(t_unfused / t_graph, 2),
+            "speedup_vs_loop_fused": round(t_fused / t_graph, 2),
+            "max_rel_diff_batched_vs_loop": worst}
+
+
+def bench_matching(torch, pp, iters=50, alpha=0.02, seed=5, check=True):
+    """8 subjects at 256^3, 4 channels, degree 3: `iters` iterations of objective and
+    gradient (pull, residual, push, pullback) with phi <- phi - alpha * dL/dphi, all
+    subjects batched and each iteration replayed from a HIP graph. The FP64 oracle
+    re-evaluates dL/dphi of subject 0 on a 256x256x8 slab from the GPU's own inputs of
+    iterations 1 and `iters`."""
+    from pushpull import _lib
+    from pushpull import batch as ppb
+
+    dev = torch.device("cuda:0")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    B, C, d = 8, 4, (N, N, N)
+    f = _smooth_noise(torch, (B, C) + d, 2.0, gen, dev)
+    g = _smooth_noise(torch, (B, C) + d, 2.0, gen, dev)
+    u = _smooth_noise(torch, (B, 3) + d, 4.0, gen, dev)
+    u *= 3.0 / u.abs().max()
+    phi = (torch.from_numpy(pp.id(dim=d)).to(dev)[None] + u).contiguous()
+    del u
+    dphi = torch.empty_like(phi)
+    df = torch.empty_like(f)
+    loss = torch.zeros(B, dtype=torch.float64, device=dev)
+    vols = [dict(f=f[b], phi=phi[b], aux=g[b], deg=3, bnd=(1, 1, 1), ext=1, out=dphi[b], out2=df[b], loss=loss[b])
+            for b in range(B)]
+    plan = ppb.BatchPlan("objective", vols)
+    L = _lib.lib()
+
+    def iteration(s):
+        plan.run(s)
+        _lib.check(L.pp_axpy(ct.c_void_p(phi.data_ptr()), ct.c_void_p(dphi.data_ptr()), -alpha, phi.numel(),
+                             ct.c_void_p(s.cuda_stream)), "axpy")
+
+    graph = ppb.Graph(iteration)
+    st = graph.stream
+    _progress("matching loop: captured")
+    errs = {}
+    z0 = 124
+
+    def oracle_check(tag, phi_in):
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+
+        class _S:
+            deg, bnd, ext = (3, 3, 3), [(1, 1, 1)], 1
+
+        f0 = f[0].cpu().numpy()
+        ph = phi_in[:, z0:z0 + 8].cpu().numpy()
+        r = oracle.pull(f0[None], ph[None], _S, prec="f64")[0] - g[0, :, z0:z0 + 8].cpu().double().numpy()
+        ref = oracle.backward_phi(f0[None], ph[None], r[None], _S)[0]
+        got = dphi[0, :, z0:z0 + 8].cpu().double().numpy()
+        errs[tag] = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+
+    losses = []
+    with torch.cuda.stream(st):
+        phi_first = phi[0].clone() if check else None
+        graph.launch()  # iteration 1
+        st.synchronize()
+        losses.append(float(loss.sum()))
+        if check:
+            oracle_check("iteration_1", phi_first)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(iters - 2):
+            graph.launch()
+        e1.record(st)
+        e1.synchronize()
+        t_ms = e0.elapsed_time(e1) / max(1, iters - 2)
+        phi_last = phi[0].clone() if check else None
+        graph.launch()  # iteration `iters`
+        st.synchronize()
+        losses.append(float(loss.sum()))
+        if check:
+            oracle_check(f"iteration_{iters}", phi_last)
+    graph.close()
+    plan.close()
+    chan_vox = B * C * N ** 3
+    return {"subjects": B, "channels": C, "iterations": iters, "ms_per_iteration": round(t_ms, 3),
+            "Gvoxel_s": round(chan_vox / (t_ms * 1e-3) / 1e9, 3),
+            "hbm_frac": round((24 + 12 * C) * B * N ** 3 / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "loss_first_last": [losses[0], losses[-1]], "rel_err_vs_fp64_dphi_slab": errs}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +293,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--tiling", type=int, default=1, help="1: LDS-tiled kernels, 0: one work-item per sample")
     ap.add_argument("--shape", type=int, default=0, help="tile shape index (0: 16x8x8, 1: 32x8x4, 2: 16x16x4)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched-uneven and matching-loop legs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device copy (reference kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -128,6 +309,7 @@ def main():
     from pushpull import _lib, ops
 
     dev = torch.device("cuda:0")
+    _progress("making inputs")
     f, phi, delta = make_inputs(torch, dev)
     n1 = N ** 3
     f1 = torch.empty_like(f)
@@ -176,9 +358,11 @@ def main():
         def add(self, deg, m):
             self.rows.append((deg, m))
 
+    _progress("warm-up")
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    _progress("timed steps")
     ev = Ev()
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
@@ -238,7 +422,16 @@ def main():
         "per_degree": detail,
         "wall_s": round(wall, 3),
     }
+    if not args.no_extras:
+        del f1, df, dphi
+        _progress("batched uneven volumes")
+        res["batched_uneven"] = bench_batched(torch, pp)
+        torch.cuda.empty_cache()
+        _progress("matching loop")
+        res["matching_loop"] = bench_matching(torch, pp)
+        torch.cuda.empty_cache()
     if not args.no_cpu:
+        _progress("cpu baseline")
         res["cpu_baseline"] = cpu_baseline(f, phi, delta)
     print(json.dumps(res))
 

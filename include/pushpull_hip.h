@@ -65,6 +65,15 @@ void push(float *f0, const float *phi, const float *f1, const pp_usize *d0, pp_u
 void pushg(float *f0, const float *phi, const float *g1, const pp_usize *d0, pp_usize n1,
            const int *bnd, const pp_usize *dp, int ext);
 
+/* f1 (grid d1) = pull of f0 at x = A*(i,j,k,1), A the 0-based 3x4 affine (column-major,
+ * 12 floats). Replaces pushpull.c:217 `pull_affine`; the loop runs over d1 (n1 is not used). */
+void pull_affine(float *f1, const float *f0, const pp_usize *d0, pp_usize n1, const pp_usize *d1,
+                 const float *Aff, const int *bnd, const pp_usize *dp, int ext);
+
+/* f0 (grid d0) += push of f1 (grid d1) at x = A*(i,j,k,1). Replaces pushpull.c:254 `push_affine`. */
+void push_affine(float *f0, const float *f1, const pp_usize *d0, pp_usize n1, const pp_usize *d1,
+                 const float *Aff, const int *bnd, const pp_usize *dp, int ext);
+
 /* ---------------------------------------------------------------- Tier B ---- */
 
 enum {
@@ -100,6 +109,13 @@ int pp_pull_hess(const pp_desc *d, float *h1, const float *phi, const float *f0,
 int pp_push(const pp_desc *d, float *f0, const float *phi, const float *f1, int accumulate, void *stream);
 /* Julia: push_grad(g1, phi, d0, sett)   src/pushpull_cpu.jl:220     in (d1,3,C,B) */
 int pp_push_grad(const pp_desc *d, float *f0, const float *phi, const float *g1, int accumulate, void *stream);
+
+/* Julia: affine_pull(f0, Aff, d1, sett)  src/pushpull_cpu.jl:268  out (d1,C,B); `aff` is the
+ * 0-based 3x4 matrix (12 floats, column-major, host memory) that adjust_affine
+ * (src/pushpull_cpu.jl:357-362) produces; d->d1 is the output grid. */
+int pp_affine_pull(const pp_desc *d, float *f1, const float *f0, const float *aff, void *stream);
+/* Julia: affine_push(f1, Aff, d0, sett)  src/pushpull_cpu.jl:309  out (d0,C,B) */
+int pp_affine_push(const pp_desc *d, float *f0, const float *f1, const float *aff, int accumulate, void *stream);
 
 /* Pullback of pull wrt phi (src/pushpull_pb.jl:13-17): dphi(d1,3,B) = sum_c delta_c * pull_grad(f0)_c,
  * NaN where the sample is invalid. Also the phi-pullback of push (src/pushpull_pb.jl:144-147)
@@ -167,6 +183,8 @@ int pp_free(void *p);
 int pp_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
 int pp_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
 int pp_memset_zero(float *p, size_t count, void *stream);
+/* y[i] += a * x[i] (stream-ordered kernel; e.g. the deformation update of a matching loop) */
+int pp_axpy(float *y, const float *x, float a, size_t count, void *stream);
 int pp_stream_sync(void *stream);
 int pp_device_sync(void);
 

@@ -31,6 +31,8 @@ REF_SO = os.path.join(_HERE, "_ref", "libpushpull_ref.so")
 
 _P = ct.c_void_p
 _SIG = [_P, _P, _P, _P, ct.c_ulonglong, _P, _P, ct.c_int]
+# pull_affine / push_affine (artifacts/C/pushpull.h:32-35)
+_SIG_AFF = [_P, _P, _P, ct.c_ulonglong, _P, _P, _P, _P, ct.c_int]
 
 _lib = None
 _ref = None
@@ -47,6 +49,11 @@ def lib():
             for op in ("pull", "pullg", "pullh", "push", "pushg"):
                 fn = getattr(L, f"ppo_{op}_{p}")
                 fn.argtypes = _SIG
+                fn.restype = None
+        for p in ("f32", "f64"):
+            for op in ("pull_affine", "push_affine"):
+                fn = getattr(L, f"ppo_{op}_{p}")
+                fn.argtypes = _SIG_AFF
                 fn.restype = None
         L.ppo_knot_export.argtypes = [ct.c_float, ct.c_int]
         L.ppo_knot_export.restype = ct.c_longlong
@@ -72,6 +79,10 @@ def ref():
         for op in ("pull", "pullg", "pullh", "push", "pushg"):
             fn = getattr(L, op)
             fn.argtypes = _SIG
+            fn.restype = None
+        for op in ("pull_affine", "push_affine"):
+            fn = getattr(L, op)
+            fn.argtypes = _SIG_AFF
             fn.restype = None
         _ref = L
     return _ref
@@ -213,3 +224,64 @@ def backward_phi(f0, phi, delta, sett, prec="f64"):
     d = np.asarray(delta, dtype=g.dtype)
     d = _as5(d, 5)
     return (d[:, :, None] * g).sum(axis=1)  # (B,3,...)
+
+
+def adjust_affine(Aff):
+    """src/pushpull_cpu.jl:357-362: keep rows 1-3 and set column 4 to the row sums minus 1
+    (1-based -> 0-based voxel indices), all in Float32 with the row sum taken in order.
+    Aff is indexed like the Julia matrix (row, column). Returns the 3x4 float32 matrix."""
+    A = np.asarray(Aff, dtype=np.float32)[:3, :4].copy()
+    for r in range(3):
+        s = np.float32(0)
+        for c in range(4):
+            s = np.float32(s + A[r, c])
+        A[r, 3] = np.float32(s - np.float32(1))
+    return A
+
+
+def _aff12(A):
+    """3x4 (row, column) -> the 12 floats of the Julia column-major matrix."""
+    return np.ascontiguousarray(np.asarray(A, np.float32).T.ravel())
+
+
+def affine_pull(f0, A0, d1, sett, prec="f32"):
+    """affine_pull with an already 0-based 3x4 matrix A0 (see adjust_affine):
+    f0 (B,C,Z0,Y0,X0) -> (B,C)+d1, d1 in array order (Z1,Y1,X1). pushpull.c:217-250."""
+    dt = _dtype(prec)
+    f0 = np.ascontiguousarray(_as5(f0, 5), dtype=dt)
+    B, C = f0.shape[:2]
+    d1 = tuple(int(v) for v in d1)
+    out = np.zeros((B, C) + d1, dtype=dt)
+    d0u = np.ascontiguousarray(f0.shape[:1:-1], dtype=np.uint64)
+    d1u = np.ascontiguousarray(d1[::-1], dtype=np.uint64)
+    a12 = _aff12(A0)
+    dp = _dp(sett)
+    fn = ref().pull_affine if prec == "ref" else getattr(lib(), f"ppo_pull_affine_{prec}")
+    n1 = int(np.prod(d1))
+    for b in range(B):
+        for c in range(C):
+            bnd = _bnd_for(sett, c)
+            fn(_ptr(out[b, c]), _ptr(f0[b, c]), _ptr(d0u), n1, _ptr(d1u), _ptr(a12), _ptr(bnd), _ptr(dp),
+               int(sett.ext))
+    return out
+
+
+def affine_push(f1, A0, d0, sett, prec="f32"):
+    """affine_push: f1 (B,C,Z1,Y1,X1) -> (B,C)+d0, pushpull.c:254-285."""
+    dt = _dtype(prec)
+    f1 = np.ascontiguousarray(_as5(f1, 5), dtype=dt)
+    B, C = f1.shape[:2]
+    d0 = tuple(int(v) for v in d0)
+    out = np.zeros((B, C) + d0, dtype=dt)
+    d0u = np.ascontiguousarray(d0[::-1], dtype=np.uint64)
+    d1u = np.ascontiguousarray(f1.shape[:1:-1], dtype=np.uint64)
+    a12 = _aff12(A0)
+    dp = _dp(sett)
+    fn = ref().push_affine if prec == "ref" else getattr(lib(), f"ppo_push_affine_{prec}")
+    n1 = int(np.prod(f1.shape[2:]))
+    for b in range(B):
+        for c in range(C):
+            bnd = _bnd_for(sett, c)
+            fn(_ptr(out[b, c]), _ptr(f1[b, c]), _ptr(d0u), n1, _ptr(d1u), _ptr(a12), _ptr(bnd), _ptr(dp),
+               int(sett.ext))
+    return out

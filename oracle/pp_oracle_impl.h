@@ -371,4 +371,59 @@ void FN(ppo_pushg)(REAL *f0, const float *phi, const REAL *g1, const u64 *d0, u6
     }
 }
 
+/* Affine variants (pushpull.c:217-285): coordinates x = A*(i,j,k,1) over the output
+ * grid d1 from the 0-based 3x4 matrix A (column-major), in the reference's order
+ * xj = (A3 j + A6 k) + A9, x = xj + A0 i, all in FP32. */
+static void FN(affine_coords)(const float *A, int i, int j, int k, float *x)
+{
+    float xj0 = A[3] * j + A[6] * k + A[9];
+    float xj1 = A[4] * j + A[7] * k + A[10];
+    float xj2 = A[5] * j + A[8] * k + A[11];
+    x[0] = xj0 + A[0] * i;
+    x[1] = xj1 + A[1] * i;
+    x[2] = xj2 + A[2] * i;
+}
+
+void FN(ppo_pull_affine)(REAL *f1, const REAL *f0, const u64 *d0, u64 n1, const u64 *d1, const float *A,
+                         const int *bnd, const u64 *dpu, int ext)
+{
+    int dp[3] = {(int)dpu[0], (int)dpu[1], (int)dpu[2]};
+    REAL fp[PPO_MAXD * PPO_MAXD * PPO_MAXD];
+    FN(axis_t) Ax[3];
+    int i, j, k;
+    u64 v = 0;
+    float x[3];
+    (void)n1;
+    for (k = 0; k < (int)d1[2]; k++)
+        for (j = 0; j < (int)d1[1]; j++)
+            for (i = 0; i < (int)d1[0]; i++, v++) {
+                FN(affine_coords)(A, i, j, k, x);
+                if (!ppo_voxok(x, d0, ext)) { f1[v] = (REAL)NAN; continue; }
+                FN(setup)(x, d0, bnd, dp, 0, 0, Ax);
+                FN(gather)(dp, Ax, d0, f0, fp);
+                f1[v] = FN(contract)(dp, fp, Ax[0].w, Ax[1].w, Ax[2].w);
+            }
+}
+
+void FN(ppo_push_affine)(REAL *f0, const REAL *f1, const u64 *d0, u64 n1, const u64 *d1, const float *A,
+                         const int *bnd, const u64 *dpu, int ext)
+{
+    int dp[3] = {(int)dpu[0], (int)dpu[1], (int)dpu[2]};
+    FN(axis_t) Ax[3];
+    int i, j, k;
+    u64 v = 0;
+    float x[3];
+    (void)n1;
+    for (k = 0; k < (int)d1[2]; k++)
+        for (j = 0; j < (int)d1[1]; j++)
+            for (i = 0; i < (int)d1[0]; i++, v++) {
+                REAL val = f1[v];
+                if (!isfinite(val)) continue;
+                FN(affine_coords)(A, i, j, k, x);
+                if (!ppo_voxok(x, d0, ext)) continue;
+                FN(setup)(x, d0, bnd, dp, 0, 0, Ax);
+                FN(scatter)(dp, Ax, d0, f0, Ax[0].w, Ax[1].w, Ax[2].w, val);
+            }
+}
+
 #undef FN

@@ -273,11 +273,32 @@ int tierA_args(const pp_usize* d0, pp_usize n1, const int* bnd, const pp_usize* 
 
 // in_n: floats of the gathered/pushed input; out_n: floats of the output; the output is
 // uploaded first when `upload_out` (push accumulates into the caller's array).
+// aff/d1: affine variant (pushpull.c:217-285): coordinates from the 0-based 3x4 matrix
+// over the output grid d1 instead of from phi.
 int tierA(pp::Op op, float* out, long long out_n, bool upload_out, const float* phi, const float* in, long long in_n,
-          const pp_usize* d0, pp_usize n1, const int* bnd, const pp_usize* dp, int ext) {
+          const pp_usize* d0, pp_usize n1, const int* bnd, const pp_usize* dp, int ext,
+          const float* aff = nullptr, const pp_usize* d1 = nullptr) {
   pp::KArgs a;
   long long n0;
   if (int r = tierA_args(d0, n1, bnd, dp, ext, a, n0)) return r;
+  if (aff) {
+    if (!d1) return fail(PP_EINVAL, "null d1");
+    long long m = 1;
+    for (int q = 0; q < 3; ++q) {
+      if (d1[q] < 1 || d1[q] > (pp_usize)kMaxDim) return fail(PP_EINVAL, "d1[%d] out of range", q);
+      m *= (long long)d1[q];
+    }
+    if (m > kMaxVox) return fail(PP_ERANGE, "output grid larger than 2^31-1 voxels");
+    // the reference loops over d1 (pushpull.c:226-249); n1 only sizes the caller's buffer
+    n1 = (pp_usize)m;
+    a.n1 = (int)m;
+    a.d1x = (int)d1[0];
+    a.d1y = (int)d1[1];
+    a.d1z = (int)d1[2];
+    a.affine = 1;
+    for (int q = 0; q < 12; ++q) a.aff[q] = aff[q];
+    phi = in;  // no deformation array is uploaded
+  }
   if (n1 == 0) return ok();
   if (!out || !phi || !in) return fail(PP_EINVAL, "null array");
   int dev = 0;
@@ -285,7 +306,7 @@ int tierA(pp::Op op, float* out, long long out_n, bool upload_out, const float* 
   if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
   Staging& st = g_stage[dev & 63];
   std::lock_guard<std::mutex> lock(st.mu);
-  const size_t phi_b = 3 * (size_t)n1 * 4, in_b = (size_t)in_n * 4, out_b = (size_t)out_n * 4;
+  const size_t phi_b = aff ? 0 : 3 * (size_t)n1 * 4, in_b = (size_t)in_n * 4, out_b = (size_t)out_n * 4;
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t need = al(phi_b) + al(in_b) + al(out_b);
   if (st.cap < need || st.dev != dev) {
@@ -302,7 +323,8 @@ int tierA(pp::Op op, float* out, long long out_n, bool upload_out, const float* 
   float* din = reinterpret_cast<float*>(base + al(phi_b));
   float* dout = reinterpret_cast<float*>(base + al(phi_b) + al(in_b));
   hipStream_t s = nullptr;
-  if ((e = hipMemcpyAsync(dphi, phi, phi_b, hipMemcpyHostToDevice, s)) != hipSuccess) return hip_fail(e, "upload phi");
+  if (phi_b && (e = hipMemcpyAsync(dphi, phi, phi_b, hipMemcpyHostToDevice, s)) != hipSuccess)
+    return hip_fail(e, "upload phi");
   if ((e = hipMemcpyAsync(din, in, in_b, hipMemcpyHostToDevice, s)) != hipSuccess) return hip_fail(e, "upload input");
   if (upload_out) {
     if ((e = hipMemcpyAsync(dout, out, out_b, hipMemcpyHostToDevice, s)) != hipSuccess) return hip_fail(e, "upload output");
@@ -313,7 +335,8 @@ int tierA(pp::Op op, float* out, long long out_n, bool upload_out, const float* 
   // one channel, one batch: channel/batch strides are never applied
   a.f_cs = in_n;
   a.out_cs = out_n;
-  if ((e = pp::launch_op(op, a, 1, s, kBlock)) != hipSuccess) return hip_fail(e, "kernel launch");
+  e = aff ? launch(op, a, 1, s) : pp::launch_op(op, a, 1, s, kBlock);
+  if (e != hipSuccess) return hip_fail(e, "kernel launch");
   if ((e = hipMemcpyAsync(out, dout, out_b, hipMemcpyDeviceToHost, s)) != hipSuccess) return hip_fail(e, "download");
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "synchronize");
   return ok();
@@ -359,6 +382,17 @@ void pushg(float* f0, const float* phi, const float* g1, const pp_usize* d0, pp_
   (void)tierA(pp::OP_PUSHG, f0, prod3(d0), true, phi, g1, 3 * (long long)n1, d0, n1, bnd, dp, ext);
 }
 
+void pull_affine(float* f1, const float* f0, const pp_usize* d0, pp_usize n1, const pp_usize* d1, const float* Aff,
+                 const int* bnd, const pp_usize* dp, int ext) {
+  const long long m = prod3(d1);
+  if (tierA(pp::OP_PULL, f1, m, false, nullptr, f0, prod3(d0), d0, n1, bnd, dp, ext, Aff, d1)) nan_fill(f1, m);
+}
+
+void push_affine(float* f0, const float* f1, const pp_usize* d0, pp_usize n1, const pp_usize* d1, const float* Aff,
+                 const int* bnd, const pp_usize* dp, int ext) {
+  (void)tierA(pp::OP_PUSH, f0, prod3(d0), true, nullptr, f1, prod3(d1), d0, n1, bnd, dp, ext, Aff, d1);
+}
+
 // ---- Tier B --------------------------------------------------------------------
 int pp_pull(const pp_desc* d, float* f1, const float* phi, const float* f0, void* s) {
   return pull_family(pp::OP_PULL, 1, d, f1, phi, f0, s);
@@ -375,6 +409,48 @@ int pp_push(const pp_desc* d, float* f0, const float* phi, const float* f1, int 
 int pp_push_grad(const pp_desc* d, float* f0, const float* phi, const float* g1, int accumulate, void* s) {
   return push_family(pp::OP_PUSHG, 3, d, f0, phi, g1, accumulate, s);
 }
+int pp_affine_pull(const pp_desc* d, float* f1, const float* f0, const float* aff, void* s) {
+  Geo g;
+  if (int r = check_desc(d, g)) return r;
+  if (!f1 || !f0 || !aff) return fail(PP_EINVAL, "null array");
+  pp::KArgs a;
+  base_args(d, a, g);
+  a.affine = 1;
+  for (int q = 0; q < 12; ++q) a.aff[q] = aff[q];
+  a.phi = f0;  // unused: coordinates come from the affine
+  a.phi_bs = 0;
+  a.f = f0;
+  a.f_cs = g.n0;
+  a.f_bs = g.n0 * d->nchan;
+  a.out = f1;
+  a.out_cs = g.n1;
+  a.out_bs = g.n1 * d->nchan;
+  return run(pp::OP_PULL, d, g, a, S(s));
+}
+
+int pp_affine_push(const pp_desc* d, float* f0, const float* f1, const float* aff, int accumulate, void* s) {
+  Geo g;
+  if (int r = check_desc(d, g)) return r;
+  if (!f0 || !f1 || !aff) return fail(PP_EINVAL, "null array");
+  if (!accumulate) {
+    hipError_t e = pp::launch_fill(f0, g.n0 * d->nchan * d->nbatch, 0.0f, S(s));
+    if (e != hipSuccess) return hip_fail(e, "zero fill");
+  }
+  pp::KArgs a;
+  base_args(d, a, g);
+  a.affine = 1;
+  for (int q = 0; q < 12; ++q) a.aff[q] = aff[q];
+  a.phi = f1;
+  a.phi_bs = 0;
+  a.f = f1;
+  a.f_cs = g.n1;
+  a.f_bs = g.n1 * d->nchan;
+  a.out = f0;
+  a.out_cs = g.n0;
+  a.out_bs = g.n0 * d->nchan;
+  return run(pp::OP_PUSH, d, g, a, S(s));
+}
+
 int pp_pull_backward(const pp_desc* d, float* dphi, const float* phi, const float* f0, const float* delta, void* s) {
   return backward_family(pp::OP_PULL_BWD, 1, d, dphi, phi, f0, delta, s);
 }
@@ -567,6 +643,11 @@ int pp_memcpy_d2h(void* dst, const void* src, size_t bytes, void* s) {
 int pp_memset_zero(float* p, size_t count, void* s) {
   hipError_t e = pp::launch_fill(p, (long long)count, 0.0f, S(s));
   return e == hipSuccess ? ok() : hip_fail(e, "zero fill");
+}
+int pp_axpy(float* y, const float* x, float a, size_t count, void* s) {
+  if (!y || !x) return fail(PP_EINVAL, "null array");
+  hipError_t e = pp::launch_axpy(y, x, a, (long long)count, S(s));
+  return e == hipSuccess ? ok() : hip_fail(e, "axpy");
 }
 int pp_stream_sync(void* s) {
   hipError_t e = hipStreamSynchronize(S(s));

@@ -208,6 +208,12 @@ __global__ void k_fill(float* p, long long n, float v) {
   for (; i < n; i += stride) p[i] = v;
 }
 
+__global__ void k_axpy(float* __restrict__ y, const float* __restrict__ x, float a, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) y[i] += a * x[i];
+}
+
 __global__ void k_fill_d(double* p, long long n, double v) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long stride = (long long)gridDim.x * blockDim.x;
@@ -251,6 +257,14 @@ hipError_t launch_fill(float* p, long long n, float v, hipStream_t s) {
   long long nb = (n + 255) / 256;
   if (nb > 65536) nb = 65536;
   hipLaunchKernelGGL(k_fill, dim3((unsigned)nb), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_axpy(float* y, const float* x, float a, long long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  long long nb = (n + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  hipLaunchKernelGGL(k_axpy, dim3((unsigned)nb), dim3(256), 0, s, y, x, a, n);
   return hipGetLastError();
 }
 

@@ -41,6 +41,8 @@ struct KArgs {
   int nchan;                 // channels processed per thread
   int accum;                 // channel-reducing ops: add to `out` instead of overwriting
   float lx, ly, lz;          // d0 - 0.99f per axis (VOXOK upper bounds)
+  int affine;                // 1: sample coordinates come from aff, not from phi
+  float aff[12];             // 0-based 3x4 affine, column-major (artifacts/C/pushpull.c:226-242)
 };
 
 // Launch op on `stream`; dp is (DP,DP,DP) when isotropic, else the runtime path is used.
@@ -70,5 +72,6 @@ hipError_t launch_batch(Op op, const BatchVol* dvols, int nvol, int total_tiles,
 // Fill n floats with v (graph-capturable; no hipMemsetAsync nodes).
 hipError_t launch_fill(float* p, long long n, float v, hipStream_t stream);
 hipError_t launch_fill_d(double* p, long long n, double v, hipStream_t stream);
+hipError_t launch_axpy(float* y, const float* x, float a, long long n, hipStream_t stream);
 
 }  // namespace pp

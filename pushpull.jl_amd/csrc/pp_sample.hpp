@@ -15,12 +15,35 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
   return (x < rem) ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
 }
 
+// Source coordinate of output voxel (i,j,k) under a 0-based affine, in the reference
+// CPU order xj = (A3 j + A6 k) + A9, x = xj + A0 i (artifacts/C/pushpull.c:231-241),
+// rounded per operation so knots and masks match it exactly.
+#pragma clang fp contract(off)
+__device__ __forceinline__ void affine_x(const float* A, int i, int j, int k, float& x, float& y, float& z) {
+  const float fi = (float)i, fj = (float)j, fk = (float)k;
+  x = (A[3] * fj + A[6] * fk + A[9]) + A[0] * fi;
+  y = (A[4] * fj + A[7] * fk + A[10]) + A[1] * fi;
+  z = (A[5] * fj + A[8] * fk + A[11]) + A[2] * fi;
+}
+#pragma clang fp contract(fast)
+
+// Coordinates (0-based) of sample v: phi - 1 (pushpull.c:65-67) or the affine map.
+__device__ __forceinline__ void sample_coords(const KArgs& a, const float* phi, int v, float& x, float& y, float& z) {
+  if (a.affine) {
+    const int i = v % a.d1x, jk = v / a.d1x;
+    affine_x(a.aff, i, jk % a.d1y, jk / a.d1y, x, y, z);
+    return;
+  }
+  x = phi[v] - 1.0f;
+  y = phi[v + a.n1] - 1.0f;
+  z = phi[v + 2 * (long long)a.n1] - 1.0f;
+}
+
 template <int DP, int NEED>
 __device__ __forceinline__ bool setup(const KArgs& a, const float* phi, int v, Axis<DP>& ax, Axis<DP>& ay,
                                       Axis<DP>& az) {
-  const float x = phi[v] - 1.0f;
-  const float y = phi[v + a.n1] - 1.0f;
-  const float z = phi[v + 2 * (long long)a.n1] - 1.0f;
+  float x, y, z;
+  sample_coords(a, phi, v, x, y, z);
   if (!voxok(x, y, z, a.ext, a.lx, a.ly, a.lz)) return false;
   make_axis<DP, NEED>(x, a.dpx, a.d0x, 1, a.bx, ax);
   make_axis<DP, NEED>(y, a.dpy, a.d0y, a.d0x, a.by, ay);

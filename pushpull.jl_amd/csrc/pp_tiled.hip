@@ -102,9 +102,7 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
     bool ok = false;
     if (in) {
       V.inside |= 1u << q;
-      x = phi[v] - 1.0f;
-      y = phi[v + a.n1] - 1.0f;
-      z = phi[v + 2 * (long long)a.n1] - 1.0f;
+      sample_coords(a, phi, v, x, y, z);
       ok = voxok(x, y, z, a.ext, a.lx, a.ly, a.lz);
     }
     V.x[q] = x;
@@ -237,9 +235,7 @@ __device__ __forceinline__ int tile_voxel(const KArgs& a, const TileState& S, in
 
 __device__ __forceinline__ void load_x(const KArgs& a, const float* __restrict__ phi, int v, float& x, float& y,
                                        float& z) {
-  x = phi[v] - 1.0f;
-  y = phi[v + a.n1] - 1.0f;
-  z = phi[v + 2 * (long long)a.n1] - 1.0f;
+  sample_coords(a, phi, v, x, y, z);
 }
 
 // ---- push / push_grad -------------------------------------------------------------

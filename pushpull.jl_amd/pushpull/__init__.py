@@ -8,11 +8,12 @@ there is no CPU fallback.
 from .settings import Settings, id  # noqa: A001
 from .autograd import pull, pull_grad, push
 from .ops import push_grad, pull_hess, pull_backward, pull_grad_backward, objective_grad
+from .affine import affine_pull, affine_push, adjust_affine
 from . import _lib
 
 __all__ = [
     "Settings", "id", "pull", "push", "pull_grad", "push_grad", "pull_hess",
-    "pull_backward", "pull_grad_backward", "objective_grad",
+    "pull_backward", "pull_grad_backward", "objective_grad", "affine_pull", "affine_push", "adjust_affine",
 ]
 
 

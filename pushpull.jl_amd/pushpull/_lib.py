@@ -45,6 +45,10 @@ _TIER_A = ["pull", "pullg", "pullh", "push", "pushg"]
 # name -> (argtypes, restype)
 _SIGS = {
     **{n: ([_P, _P, _P, _P, _U, _P, _P, ct.c_int], None) for n in _TIER_A},
+    "pull_affine": ([_P, _P, _P, _U, _P, _P, _P, _P, ct.c_int], None),
+    "push_affine": ([_P, _P, _P, _U, _P, _P, _P, _P, ct.c_int], None),
+    "pp_affine_pull": ([_P, _P, _P, _P, _P], ct.c_int),
+    "pp_affine_push": ([_P, _P, _P, _P, ct.c_int, _P], ct.c_int),
     "pp_pull": ([_P, _P, _P, _P, _P], ct.c_int),
     "pp_pull_grad": ([_P, _P, _P, _P, _P], ct.c_int),
     "pp_pull_hess": ([_P, _P, _P, _P, _P], ct.c_int),
@@ -58,6 +62,7 @@ _SIGS = {
     "pp_memcpy_h2d": ([_P, _P, ct.c_size_t, _P], ct.c_int),
     "pp_memcpy_d2h": ([_P, _P, ct.c_size_t, _P], ct.c_int),
     "pp_memset_zero": ([_P, ct.c_size_t, _P], ct.c_int),
+    "pp_axpy": ([_P, _P, ct.c_float, ct.c_size_t, _P], ct.c_int),
     "pp_stream_sync": ([_P], ct.c_int),
     "pp_device_sync": ([], ct.c_int),
     "pp_set_option": ([ct.c_int, ct.c_int], ct.c_int),

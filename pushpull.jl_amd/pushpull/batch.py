@@ -77,13 +77,22 @@ class BatchPlan:
             else:
                 d0 = tuple(v["dim"])
             if op == "pull":
-                out = torch.empty((C,) + d1, dtype=torch.float32, device=f.device)
+                shape = (C,) + d1
             elif op in ("push", "push_grad"):
-                out = torch.empty((C,) + d0, dtype=torch.float32, device=f.device)
+                shape = (C,) + d0
             else:
-                out = torch.empty((3,) + d1, dtype=torch.float32, device=f.device)
-            out2 = torch.empty_like(f) if op == "objective" else None
-            loss = torch.zeros((), dtype=torch.float64, device=f.device) if op == "objective" else None
+                shape = (3,) + d1
+            out = v.get("out")
+            if out is None:
+                out = torch.empty(shape, dtype=torch.float32, device=f.device)
+            elif tuple(out.shape) != shape or not out.is_contiguous():
+                raise AssertionError(f"`out` must be a contiguous {shape} tensor")
+            out2 = loss = None
+            if op == "objective":
+                out2 = v.get("out2")
+                out2 = torch.empty_like(f) if out2 is None else out2
+                loss = v.get("loss")
+                loss = torch.zeros((), dtype=torch.float64, device=f.device) if loss is None else loss
             rec = dict(f=f, phi=phi, aux=aux, out=out, out2=out2, loss=loss)
             self.vols.append(rec)
             s = arr[i]

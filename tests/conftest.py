@@ -37,9 +37,16 @@ def pytest_collection_modifyitems(config, items):
 @pytest.fixture(scope="session")
 def golden():
     z = np.load(GOLDEN, allow_pickle=False)
-    n = 1 + max(int(k[1:4]) for k in z.files)
+    n = 1 + max(int(k[1:4]) for k in z.files if k.startswith("c"))
     cases = []
     for i in range(n):
         p = f"c{i:03d}_"
         cases.append({k[len(p):]: z[k] for k in z.files if k.startswith(p)})
     return cases
+
+
+@pytest.fixture(scope="session")
+def golden_affine():
+    z = np.load(GOLDEN, allow_pickle=False)
+    ids = sorted({k[:4] for k in z.files if k.startswith("a")})
+    return [{k[5:]: z[k] for k in z.files if k.startswith(i + "_")} for i in ids]

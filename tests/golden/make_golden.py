@@ -142,6 +142,36 @@ def cases():
     return cs
 
 
+def affine_cases():
+    """Affine pull/push (pushpull.c:217-285): rotations, shears, scalings and shifts of
+    small grids, every degree, mixed boundaries, ext 0/1."""
+    rng = np.random.default_rng(20240602)
+    cs = []
+    for i in range(20):
+        d0 = tuple(int(v) for v in rng.integers(3, 10, size=3))
+        d1 = tuple(int(v) for v in rng.integers(3, 10, size=3))
+        deg = int(i % 5)
+        th = rng.uniform(-0.6, 0.6)
+        R = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
+        M = R @ np.diag(rng.uniform(0.6, 1.6, size=3)) + rng.uniform(-0.2, 0.2, size=(3, 3))
+        Aff = np.zeros((4, 4))
+        Aff[:3, :3] = M
+        Aff[:3, 3] = rng.uniform(-4, 4, size=3)
+        Aff[3, 3] = 1
+        A0 = oracle.adjust_affine(Aff.astype(np.float32))
+        bnd = tuple(int(v) for v in rng.integers(0, 3, size=3))
+        ext = int(i % 2)
+        s = _S((deg,) * 3, bnd, ext)
+        f0 = rng.standard_normal(d0).astype(np.float32)
+        f1 = rng.standard_normal(d1).astype(np.float32)
+        f1.reshape(-1)[rng.choice(f1.size, size=2, replace=False)] = np.nan
+        cs.append(dict(aff=Aff.astype(np.float32), aff0=A0, f0=f0, f1=f1, d1=np.array(d1, np.int64),
+                       deg=np.array((deg,) * 3, np.int32), bnd=np.array(bnd, np.int32), ext=np.array(ext, np.int32),
+                       pull_affine=oracle.affine_pull(f0, A0, d1, s, prec="ref")[0, 0],
+                       push_affine=oracle.affine_push(f1, A0, d0, s, prec="ref")[0, 0]))
+    return cs
+
+
 def main():
     if not oracle.ref_available():
         raise SystemExit("oracle/_ref/libpushpull_ref.so is missing: run `make -C oracle ref` first")
@@ -153,6 +183,9 @@ def main():
             arrays[f"c{i:03d}_{k}"] = np.asarray(v)
         manifest.append(dict(id=i, d0=list(c["f0"].shape), d1=list(c["phi"].shape[1:]), deg=c["deg"].tolist(),
                              bnd=c["bnd"].tolist(), ext=int(c["ext"])))
+    for j, c in enumerate(affine_cases()):
+        for k, v in c.items():
+            arrays[f"a{j:03d}_{k}"] = np.asarray(v)
     path = os.path.join(HERE, "pushpull_golden.npz")
     np.savez_compressed(path, **arrays)
     with open(os.path.join(HERE, "pushpull_golden.json"), "w") as fh:

@@ -94,3 +94,18 @@ def test_live_random_sweep_against_reference():
         for fn in (oracle.pull, oracle.pull_grad, oracle.pull_hess):
             assert _eq(fn(f0, phi, s), fn(f0, phi, s, prec="ref")), (trial, fn.__name__)
         assert _eq(oracle.push(f1, phi, d0, s), oracle.push(f1, phi, d0, s, prec="ref")), trial
+
+
+@pytest.mark.parametrize("op", ["pull_affine", "push_affine"])
+def test_affine_restatement_bit_exact(golden_affine, op):
+    assert len(golden_affine) >= 20
+    bad = []
+    for i, c in enumerate(golden_affine):
+        s = case_settings(c)
+        if op == "pull_affine":
+            got = oracle.affine_pull(c["f0"], c["aff0"], tuple(c["d1"]), s)[0, 0]
+        else:
+            got = oracle.affine_push(c["f1"], c["aff0"], c["f0"].shape, s)[0, 0]
+        if not _eq(got, c[op]):
+            bad.append(i)
+    assert not bad, bad

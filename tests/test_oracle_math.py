@@ -266,3 +266,55 @@ def test_invalid_samples_are_nan_and_skipped():
     g[2, 2, 2] = np.inf           # non-finite pushed values are skipped
     p = oracle.push(g, identity(d), d, S(1, 1, 0), prec="f64")[0, 0]
     assert np.isfinite(p).all() and abs(p.sum() - 63) < 1e-12
+
+
+# ---- affine (src/pushpull_cpu.jl:268-362, pushpull.c:217-285) ----------------------
+def test_adjust_affine_known_answers():
+    import pushpull as pp
+
+    I = np.eye(4, dtype=np.float32)
+    np.testing.assert_array_equal(pp.adjust_affine(I), np.eye(4, dtype=np.float32)[:3])
+    T = np.eye(4, dtype=np.float32)
+    T[:3, 3] = (2.5, -1.0, 4.0)           # 1-based shift -> same 0-based shift
+    np.testing.assert_array_equal(pp.adjust_affine(T)[:, 3], np.float32([2.5, -1.0, 4.0]))
+    S2 = np.diag(np.float32([2, 3, 0.5, 1]))  # x1 = 2 i1  ->  x0 = 2 i0 + 1
+    np.testing.assert_array_equal(pp.adjust_affine(S2)[:, 3], np.float32([1.0, 2.0, -0.5]))
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        A = rng.standard_normal((3, 4)).astype(np.float32)
+        np.testing.assert_array_equal(pp.adjust_affine(A), oracle.adjust_affine(A))
+
+
+@pytest.mark.parametrize("deg", [0, 1, 2, 3, 4])
+def test_affine_identity_and_shift(deg):
+    import pushpull as pp
+
+    rng = np.random.default_rng(deg)
+    d = (6, 7, 8)
+    f = rng.standard_normal(d).astype(np.float32)
+    A0 = pp.adjust_affine(np.eye(4, dtype=np.float32))
+    a = oracle.affine_pull(f, A0, d, S(deg, 0, 1), prec="f64")[0, 0]
+    b = oracle.pull(f, identity(d), S(deg, 0, 1), prec="f64")[0, 0]
+    np.testing.assert_allclose(a, b, atol=1e-13)
+    T = np.eye(4, dtype=np.float32)
+    T[:3, 3] = (2, -1, 3)  # (x, y, z) shift, circulant -> roll
+    a = oracle.affine_pull(f, pp.adjust_affine(T), d, S(deg, 0, 1), prec="f64")[0, 0]
+    np.testing.assert_allclose(a, np.roll(b, (-3, 1, -2), axis=(0, 1, 2)), atol=1e-12)
+
+
+@pytest.mark.parametrize("deg", [1, 2, 3])
+def test_affine_push_is_adjoint(deg):
+    import pushpull as pp
+
+    rng = np.random.default_rng(10 + deg)
+    d0, d1 = (7, 6, 9), (5, 8, 6)
+    A = np.eye(4)
+    A[:3, :3] += rng.uniform(-0.3, 0.3, size=(3, 3))
+    A[:3, 3] = rng.uniform(-2, 2, size=3)
+    A0 = pp.adjust_affine(A.astype(np.float32))
+    f = rng.standard_normal(d0)
+    g = rng.standard_normal(d1)
+    s = S(deg, (0, 1, 2), 1)
+    lhs = np.sum(oracle.affine_pull(f, A0, d1, s, prec="f64") * g)
+    rhs = np.sum(f * oracle.affine_push(g, A0, d0, s, prec="f64"))
+    assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + abs(rhs))
