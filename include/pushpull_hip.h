@@ -139,8 +139,9 @@ int pp_objective_grad(const pp_desc *d, float *dphi, float *df0, double *loss, c
  * No counterpart in the reference, whose wrappers loop over same-size (channel, batch)
  * slices with one native call each (src/pushpull_cpu.jl:42-51, ext/pushpull_gpu.jl:37-48).
  * A plan describes many volumes of different sizes, channel counts and degrees; running it
- * is one persistent launch over all their tiles (plus one zeroing launch for the
- * accumulating ops), on kernels only, so it can be captured into a HIP graph. */
+ * is one persistent launch over all the tiles of the volumes that share a degree (one per
+ * degree present; plus one zeroing launch for the accumulating ops), on kernels only, so
+ * it can be captured into a HIP graph. */
 enum {
   PP_OP_PULL = 0,          /* out = pull(f, phi)                                   */
   PP_OP_PUSH = 3,          /* out = push(f, phi) (zeroed first)                    */

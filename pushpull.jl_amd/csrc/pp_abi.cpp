@@ -2,6 +2,7 @@
 // include/pushpull_hip.h): argument checking, stride set-up, Tier A host staging.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdarg>
@@ -488,11 +489,18 @@ int pp_objective_grad(const pp_desc* d, float* dphi, float* df0, double* loss, c
 }
 
 // ---- batches of uneven volumes -----------------------------------------------------
+struct pp_batch_group {
+  int dp;      // degree + 1 of every volume in the group
+  int first;   // index of the group's first volume in the device array
+  int nvol;
+  int total;   // tiles over the group's volumes
+  int grid;    // persistent workgroups for this degree's kernel
+};
+
 struct pp_batch_plan {
   pp::Op op;
   int nvol;
-  int total;  // tiles over all volumes
-  int grid;   // persistent workgroups
+  std::vector<pp_batch_group> groups;  // volumes ordered by degree, one launch per group
   pp::BatchVol* dvols;
 };
 
@@ -560,9 +568,21 @@ int pp_batch_plan_create(int op, const pp_volume* vols, int nvol, pp_batch_plan*
         break;
     }
     hv[i].dp = v.deg + 1;
-    hv[i].tile0 = (int)total;
     total += pp::batch_tiles(a);
     if (total > kMaxVox) return fail(PP_ERANGE, "too many tiles in one batch");
+  }
+  // order by degree (stable), number tiles within each degree group
+  std::stable_sort(hv.begin(), hv.end(), [](const pp::BatchVol& x, const pp::BatchVol& y) { return x.dp < y.dp; });
+  std::vector<pp_batch_group> groups;
+  for (int i = 0; i < nvol; ++i) {
+    if (groups.empty() || groups.back().dp != hv[i].dp) {
+      const pp::Op o = static_cast<pp::Op>(op);
+      groups.push_back(pp_batch_group{hv[i].dp, i, 0, 0, pp::batch_grid(o, hv[i].dp)});
+    }
+    pp_batch_group& g = groups.back();
+    hv[i].tile0 = g.total;
+    g.total += pp::batch_tiles(hv[i].a);
+    g.nvol += 1;
   }
   pp::BatchVol* d = nullptr;
   hipError_t e = hipMalloc((void**)&d, sizeof(pp::BatchVol) * (size_t)nvol);
@@ -572,13 +592,17 @@ int pp_batch_plan_create(int op, const pp_volume* vols, int nvol, pp_batch_plan*
     (void)hipFree(d);
     return hip_fail(e, "upload batch descriptors");
   }
-  *plan = new pp_batch_plan{static_cast<pp::Op>(op), nvol, (int)total, pp::batch_grid(static_cast<pp::Op>(op)), d};
+  *plan = new pp_batch_plan{static_cast<pp::Op>(op), nvol, std::move(groups), d};
   return ok();
 }
 
 int pp_batch_run(const pp_batch_plan* plan, void* s) {
   if (!plan) return fail(PP_EINVAL, "null plan");
-  hipError_t e = pp::launch_batch(plan->op, plan->dvols, plan->nvol, plan->total, plan->grid, true, S(s));
+  hipError_t e = pp::launch_batch_zero(plan->op, plan->dvols, plan->nvol, S(s));
+  for (const pp_batch_group& g : plan->groups) {
+    if (e != hipSuccess) break;
+    e = pp::launch_batch_group(plan->op, g.dp, plan->dvols + g.first, g.nvol, g.total, g.grid, S(s));
+  }
   return e == hipSuccess ? ok() : hip_fail(e, "batch launch");
 }
 
@@ -708,10 +732,13 @@ int pp_tune(int op, const pp_desc* d, float* out, float* out2, double* loss, con
     t_force = v;
     rc = call();  // warm-up (also compiles nothing: all variants are prebuilt)
     if (rc != PP_OK) break;
-    hipEventRecord(e0, S(s));
+    if (hipEventRecord(e0, S(s)) != hipSuccess) {
+      rc = fail(PP_EDEVICE, "hipEventRecord");
+      break;
+    }
     for (int i = 0; i < reps && rc == PP_OK; ++i) rc = call();
-    hipEventRecord(e1, S(s));
-    if (hipEventSynchronize(e1) != hipSuccess) rc = fail(PP_EDEVICE, "hipEventSynchronize");
+    if (rc == PP_OK && (hipEventRecord(e1, S(s)) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+      rc = fail(PP_EDEVICE, "hipEventRecord/hipEventSynchronize");
     float ms = 0.f;
     if (rc == PP_OK && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && (best < 0 || ms < best_ms)) {
       best_ms = ms;

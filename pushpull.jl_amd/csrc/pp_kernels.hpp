@@ -64,10 +64,12 @@ struct BatchVol {
   int tile0;        // first tile number of this volume
   long long zero_n; // floats zeroed before a push / objective launch
 };
-int batch_tiles(const KArgs& a);  // tiles of one volume (16x8x8 tiles of the phi grid)
-int batch_grid(Op op);            // workgroups that fit on the device at once (multiple of 8)
-hipError_t launch_batch(Op op, const BatchVol* dvols, int nvol, int total_tiles, int grid, bool zero,
-                        hipStream_t stream);
+int batch_tiles(const KArgs& a);   // tiles of one volume (16x8x8 tiles of the phi grid)
+int batch_grid(Op op, int dp);     // workgroups of one degree's kernel that fit at once (multiple of 8)
+hipError_t launch_batch_zero(Op op, const BatchVol* dvols, int nvol, hipStream_t stream);
+// One degree group: dvols[0..nvol) all have this dp; tile0 counts from the group start.
+hipError_t launch_batch_group(Op op, int dp, const BatchVol* dvols, int nvol, int total_tiles, int grid,
+                              hipStream_t stream);
 
 // Fill n floats with v (graph-capturable; no hipMemsetAsync nodes).
 hipError_t launch_fill(float* p, long long n, float v, hipStream_t stream);

@@ -180,9 +180,44 @@ __device__ __forceinline__ void stage_box(float* box, const TileState& S, const 
   }
 }
 
-// Flush the accumulated box into image channel `dst` with one global atomic per
-// non-zero cell (a cell nobody touched holds exactly +0 and adds nothing).
-__device__ __forceinline__ void flush_box(const float* box, const TileState& S, float* __restrict__ dst,
+// ---- fixed-point scatter accumulation ---------------------------------------------
+// gfx950 executes float atomics (LDS ds_add_f32 and global alike) at a small fraction
+// of the rate of integer LDS atomics, so scatters accumulate into an int64 LDS box
+// with ds_add_u64: every contribution c is stored as round(c * 2^scale). Per channel
+// and tile, scale = 52 - e where 2^e bounds the largest contribution: no cell can
+// overflow (at most 2^10 samples per tile touch a cell), integer sums are exact and
+// order-independent, and each cell is rounded to float once, when it is flushed.
+typedef unsigned long long acc_t;
+
+__device__ __forceinline__ int fixed_scale(float vmax) {
+  if (!(vmax > 0.0f)) return 0;
+  int e;
+  (void)frexpf(vmax, &e);  // vmax < 2^e
+  return 52 - e;
+}
+
+__device__ __forceinline__ void acc_add(acc_t* cell, float c, int scale) {
+  atomicAdd(cell, (acc_t)__float2ll_rn(ldexpf(c, scale)));
+}
+
+__device__ __forceinline__ void zero_acc(acc_t* box, int cells) {
+  for (int i = threadIdx.x; i < cells; i += kTB) box[i] = 0ull;
+}
+
+// Largest value over the workgroup (v >= 0); uses and releases wsum.
+__device__ __forceinline__ float block_max(float v, float* wsum) {
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+  __syncthreads();
+  float m = wsum[0];
+  for (int w = 1; w < kTB / 64; ++w) m = fmaxf(m, wsum[w]);
+  __syncthreads();
+  return m;
+}
+
+// Flush the int64 box into image channel `dst`: one folded global float atomic per
+// non-zero cell.
+__device__ __forceinline__ void flush_acc(const acc_t* box, const TileState& S, int scale, float* __restrict__ dst,
                                           const KArgs& a) {
   const int BX = S.b[0], BY = S.b[1], rows = S.b[1] * S.b[2];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -191,20 +226,17 @@ __device__ __forceinline__ void flush_box(const float* box, const TileState& S, 
     const int fy = fold32(a.by, S.lo[1] + r % BY, a.d0y, ny);
     const int fz = fold32(a.bz, S.lo[2] + r / BY, a.d0z, nz);
     float* drow = dst + a.d0x * (fy + a.d0y * fz);
-    const float* brow = box + r * BX;
+    const acc_t* brow = box + r * BX;
     for (int c = lane; c < BX; c += 64) {
-      const float v = brow[c];
-      if (v != 0.0f) {
+      const long long q = (long long)brow[c];
+      if (q != 0) {
         bool nx;
         const int fx = fold32(a.bx, S.lo[0] + c, a.d0x, nx);
+        const float v = ldexpf((float)q, -scale);
         unsafeAtomicAdd(drow + fx, (nx ^ ny ^ nz) ? -v : v);
       }
     }
   }
-}
-
-__device__ __forceinline__ void zero_box(float* box, int cells) {
-  for (int i = threadIdx.x; i < cells; i += kTB) box[i] = 0.0f;
 }
 
 // Unsigned, unfolded weights of one in-window sample and its box offset
@@ -239,10 +271,10 @@ __device__ __forceinline__ void load_x(const KArgs& a, const float* __restrict__
 }
 
 // ---- push / push_grad -------------------------------------------------------------
-// One tile of push (GRAD=false) or push_grad (GRAD=true) for batch index bb. The
-// caller provides the LDS box (CAP floats) and the tile state.
+// One tile of push (GRAD=false) or push_grad (GRAD=true) for batch index bb. The caller
+// provides the int64 LDS box (CAP cells), wsum (kTB/64 floats) and the tile state.
 template <int DP, int TX, int TY, int TZ, int CAP, bool GRAD>
-__device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, float* box, TileState& S) {
+__device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, acc_t* box, float* wsum, TileState& S) {
   constexpr int VPT = Shape<TX, TY, TZ>::VPT;
   const float* phi = a.phi + bb * a.phi_bs;
   unsigned valid, inl;
@@ -258,8 +290,22 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, floa
   for (int c = 0; c < a.nchan; ++c) {
     const float* src = a.f + bb * a.f_bs + c * a.f_cs;
     float* dst = a.out + bb * a.out_bs + c * a.out_cs;
+    unsigned use = lds ? inl : 0u;  // samples accumulated in LDS for this channel
+    int scale = 0;
     if (lds) {
-      zero_box(box, S.cells);
+      // bound on |contribution| (|w|, |dw| <= 1); non-finite values take the global path
+      float vmax = 0.f;
+#pragma unroll 1
+      for (int q = 0; q < VPT; ++q) {
+        if (!((use >> q) & 1u)) continue;
+        bool in;
+        const int v = tile_voxel<TX, TY, TZ>(a, S, q, in);
+        const float b = GRAD ? fabsf(src[v]) + fabsf(src[v + n1]) + fabsf(src[v + 2 * n1]) : fabsf(src[v]);
+        if (isfinite(b)) vmax = fmaxf(vmax, b);
+        else use &= ~(1u << q);
+      }
+      scale = fixed_scale(block_max(vmax, wsum));
+      zero_acc(box, S.cells);
       __syncthreads();
     }
 #pragma unroll 1
@@ -273,23 +319,23 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, floa
       if (!GRAD && !isfinite(g0)) continue;  // push skips non-finite values (pushpull.c:155)
       float x, y, z;
       load_x(a, phi, v, x, y, z);
-      if (lds && ((inl >> q) & 1u)) {
+      if ((use >> q) & 1u) {
         LSample<DP> L;
         lsample<DP, GRAD>(x, y, z, S, L);
 #pragma unroll
         for (int k = 0; k < DP; ++k)
 #pragma unroll
           for (int j = 0; j < DP; ++j) {
-            float* row = box + L.base + BX * j + BXY * k;
+            acc_t* row = box + L.base + BX * j + BXY * k;
             if (GRAD) {
               const float cx = (L.wz[k] * L.wy[j]) * g0;
               const float cyz = (L.wz[k] * L.dy[j]) * g1 + (L.dz[k] * L.wy[j]) * g2;
 #pragma unroll
-              for (int i = 0; i < DP; ++i) atomicAdd(row + i, L.dx[i] * cx + L.wx[i] * cyz);
+              for (int i = 0; i < DP; ++i) acc_add(row + i, L.dx[i] * cx + L.wx[i] * cyz, scale);
             } else {
               const float wv = (L.wz[k] * L.wy[j]) * g0;
 #pragma unroll
-              for (int i = 0; i < DP; ++i) atomicAdd(row + i, L.wx[i] * wv);
+              for (int i = 0; i < DP; ++i) acc_add(row + i, L.wx[i] * wv, scale);
             }
           }
       } else {
@@ -304,7 +350,7 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, floa
     }
     if (lds) {
       __syncthreads();
-      flush_box(box, S, dst, a);
+      flush_acc(box, S, scale, dst, a);
       __syncthreads();
     }
   }
@@ -313,9 +359,10 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, floa
 
 template <int DP, int TX, int TY, int TZ, int CAP, bool GRAD>
 __global__ __launch_bounds__(kTB) void k_push_tile(KArgs a) {
-  __shared__ float box[CAP];
+  __shared__ acc_t box[CAP];
   __shared__ TileState S;
-  push_tile<DP, TX, TY, TZ, CAP, GRAD>(a, xcd_block(blockIdx.x, gridDim.x), blockIdx.y, box, S);
+  __shared__ float wsum[kTB / 64];
+  push_tile<DP, TX, TY, TZ, CAP, GRAD>(a, xcd_block(blockIdx.x, gridDim.x), blockIdx.y, box, wsum, S);
 }
 
 // ---- gathers: pull (KIND 0), phi pullback of pull (1), fused objective (2) ----------
@@ -359,18 +406,20 @@ __device__ __forceinline__ void lds_grad(const float* box, const LSample<DP>& L,
     }
 }
 
-// LDS a gather kind needs, in floats: box, second box (objective), per-sample pullback
+// LDS a gather kind needs, in floats. The objective's box is twice as large: it first
+// holds the staged image (float) and then, reused, the int64 scatter accumulator.
 template <int TX, int TY, int TZ, int CAP, int KIND>
 struct GatherLds {
-  static constexpr int CAP2 = KIND == 2 ? CAP : 1;
-  static constexpr int NACC = KIND == 0 ? 1 : 3 * Shape<TX, TY, TZ>::N;
+  static constexpr int BOXF = KIND == 2 ? 2 * CAP : CAP;         // box
+  static constexpr int NACC = KIND == 0 ? 1 : 3 * Shape<TX, TY, TZ>::N;  // per-sample pullback
+  static constexpr int NR = KIND == 2 ? Shape<TX, TY, TZ>::N : 1;         // per-sample residual
 };
 
 // One tile of pull (KIND 0), the pull pullback (1) or the fused objective (2).
-// box: CAP floats; box2: CAP floats for KIND 2; acc: 3*N floats for KIND 1/2;
-// wsum: kTB/64 floats.
+// box: GatherLds::BOXF floats (8-byte aligned); acc: 3*N floats for KIND 1/2;
+// rbuf: N floats for KIND 2; wsum: kTB/64 floats.
 template <int DP, int TX, int TY, int TZ, int CAP, int KIND>
-__device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, float* box, float* box2, float* acc,
+__device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, float* box, float* acc, float* rbuf,
                                             float* wsum, TileState& S) {
   constexpr int N = Shape<TX, TY, TZ>::N;
   constexpr int VPT = Shape<TX, TY, TZ>::VPT;
@@ -396,9 +445,9 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
   float lsum = 0.f;
   for (int c = 0; c < a.nchan; ++c) {
     const float* src = a.f + bb * a.f_bs + c * a.f_cs;
+    unsigned rfin = 0u;  // KIND 2: in-window samples with a finite residual (scattered via LDS)
     if (lds) {
       stage_box(box, S, src, a);
-      if (KIND == 2) zero_box(box2, S.cells);
       __syncthreads();
     }
 #pragma unroll 1
@@ -437,15 +486,8 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
         lds_grad<DP, KIND == 2>(box, L, BX, BXY, val, gx, gy, gz);
         w = (KIND == 1) ? aux : val - aux;
         if (KIND == 2 && isfinite(w)) {
-#pragma unroll
-          for (int k = 0; k < DP; ++k)
-#pragma unroll
-            for (int j = 0; j < DP; ++j) {
-              float* row = box2 + L.base + BX * j + BXY * k;
-              const float wv = (L.wz[k] * L.wy[j]) * w;
-#pragma unroll
-              for (int i = 0; i < DP; ++i) atomicAdd(row + i, L.wx[i] * wv);
-            }
+          rbuf[threadIdx.x + kTB * q] = w;
+          rfin |= 1u << q;
         }
       } else {
         Axis<DP> ax, ay, az;
@@ -465,8 +507,37 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
     }
     if (lds) {
       __syncthreads();
-      if (KIND == 2) {
-        flush_box(box2, S, a.out2 + bb * a.out2_bs + c * a.out2_cs, a);
+      if (KIND == 2) {  // scatter the residuals of the in-window samples through the int64 box
+        float vmax = 0.f;
+#pragma unroll 1
+        for (int q = 0; q < VPT; ++q)
+          if ((rfin >> q) & 1u) vmax = fmaxf(vmax, fabsf(rbuf[threadIdx.x + kTB * q]));
+        const int scale = fixed_scale(block_max(vmax, wsum));  // also orders the staging reads
+        acc_t* box64 = reinterpret_cast<acc_t*>(box);
+        zero_acc(box64, S.cells);
+        __syncthreads();
+#pragma unroll 1
+        for (int q = 0; q < VPT; ++q) {
+          if (!((rfin >> q) & 1u)) continue;
+          bool in;
+          const int v = tile_voxel<TX, TY, TZ>(a, S, q, in);
+          float x, y, z;
+          load_x(a, phi, v, x, y, z);
+          LSample<DP> L;
+          lsample<DP, false>(x, y, z, S, L);
+          const float w = rbuf[threadIdx.x + kTB * q];
+#pragma unroll
+          for (int k = 0; k < DP; ++k)
+#pragma unroll
+            for (int j = 0; j < DP; ++j) {
+              acc_t* row = box64 + L.base + BX * j + BXY * k;
+              const float wv = (L.wz[k] * L.wy[j]) * w;
+#pragma unroll
+              for (int i = 0; i < DP; ++i) acc_add(row + i, L.wx[i] * wv, scale);
+            }
+        }
+        __syncthreads();
+        flush_acc(box64, S, scale, a.out2 + bb * a.out2_bs + c * a.out2_cs, a);
         __syncthreads();
       }
     }
@@ -499,17 +570,24 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
 template <int DP, int TX, int TY, int TZ, int CAP, int KIND>
 __global__ __launch_bounds__(kTB) void k_gather_tile(KArgs a) {
   using G = GatherLds<TX, TY, TZ, CAP, KIND>;
-  __shared__ float box[CAP];
-  __shared__ float box2[G::CAP2];
+  __shared__ __align__(8) float box[G::BOXF];
   __shared__ float acc[G::NACC];  // per-sample phi pullback, summed over channels
+  __shared__ float rbuf[G::NR];
   __shared__ TileState S;
   __shared__ float wsum[kTB / 64];
-  gather_tile<DP, TX, TY, TZ, CAP, KIND>(a, xcd_block(blockIdx.x, gridDim.x), blockIdx.y, box, box2, acc, wsum, S);
+  gather_tile<DP, TX, TY, TZ, CAP, KIND>(a, xcd_block(blockIdx.x, gridDim.x), blockIdx.y, box, acc, rbuf, wsum, S);
 }
 
 // ---- launcher ------------------------------------------------------------------
-// Box budget: CAP cells per box (32 KiB); the fused objective holds two boxes.
+// Box budgets in cells: gathers stage 8192 floats (32 KiB); scatters accumulate 6144
+// int64 cells (48 KiB); the objective stages and accumulates in one 6144-cell region.
 constexpr int kCap = 8192;
+constexpr int kCapAcc = 6144;
+
+template <int KIND>
+constexpr int cap_for() {
+  return KIND == 0 || KIND == 1 ? kCap : kCapAcc;
+}
 
 template <int TX, int TY, int TZ>
 static int tiles_of(const KArgs& a) {
@@ -522,9 +600,9 @@ static hipError_t launch_shape(Op op, const KArgs& a, int nbatch, hipStream_t s)
   switch (op) {
     case OP_PULL: hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, kCap, 0>), grid, block, 0, s, a); break;
     case OP_PULL_BWD: hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, kCap, 1>), grid, block, 0, s, a); break;
-    case OP_OBJGRAD: hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, kCap, 2>), grid, block, 0, s, a); break;
-    case OP_PUSH: hipLaunchKernelGGL((k_push_tile<DP, TX, TY, TZ, kCap, false>), grid, block, 0, s, a); break;
-    case OP_PUSHG: hipLaunchKernelGGL((k_push_tile<DP, TX, TY, TZ, kCap, true>), grid, block, 0, s, a); break;
+    case OP_OBJGRAD: hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, kCapAcc, 2>), grid, block, 0, s, a); break;
+    case OP_PUSH: hipLaunchKernelGGL((k_push_tile<DP, TX, TY, TZ, kCapAcc, false>), grid, block, 0, s, a); break;
+    case OP_PUSHG: hipLaunchKernelGGL((k_push_tile<DP, TX, TY, TZ, kCapAcc, true>), grid, block, 0, s, a); break;
     default: return hipErrorNotSupported;
   }
   return hipGetLastError();
@@ -539,28 +617,26 @@ static hipError_t launch_deg(Op op, const KArgs& a, int nbatch, hipStream_t s, T
   }
 }
 
-// ---- batched persistent launch over volumes of uneven size and degree -------------
-// Tiles of all volumes are numbered consecutively (BatchVol::tile0 is the prefix sum).
-// A grid that fits on the device at once walks them; each XCD takes a contiguous
-// range of tile numbers so neighbouring tiles share its L2. Degrees 1-3 use the LDS
-// tile bodies; degrees 0 and 4 run the same bodies with an empty box budget, i.e. per
-// sample through global memory.
+// ---- batched persistent launch over volumes of uneven size --------------------------
+// A plan orders its volumes by degree; each degree present gets one persistent launch of
+// a kernel compiled for that degree only (a single kernel switching over five degrees
+// would carry the register footprint of all of them). Within a launch the tiles of the
+// group's volumes are numbered consecutively (BatchVol::tile0, relative to the group);
+// as many workgroups as fit on the device walk them, each XCD a contiguous range of
+// tile numbers so neighbouring tiles share its L2. Degrees 1-3 use the LDS tile bodies;
+// degrees 0 and 4 run the same bodies with an empty box budget, i.e. per sample through
+// global memory.
 constexpr int kBTX = 16, kBTY = 8, kBTZ = 8;
 
 template <int DP, int KIND>
-__device__ __forceinline__ void batch_tile(const KArgs& a, int tile, float* box, float* box2, float* acc, float* wsum,
-                                           TileState& S) {
-  constexpr int CAPE = (DP >= 2 && DP <= 4) ? kCap : 0;
-  if (KIND <= 2) gather_tile<DP, kBTX, kBTY, kBTZ, CAPE, KIND>(a, tile, 0, box, box2, acc, wsum, S);
-  else push_tile<DP, kBTX, kBTY, kBTZ, CAPE, KIND == 4>(a, tile, 0, box, S);
-}
-
-template <int KIND>
 __global__ __launch_bounds__(kTB) void k_batch(const BatchVol* __restrict__ vols, int nvol, int total) {
-  using G = GatherLds<kBTX, kBTY, kBTZ, kCap, KIND <= 2 ? KIND : 0>;
-  __shared__ float box[kCap];
-  __shared__ float box2[G::CAP2];
+  constexpr int KG = KIND <= 2 ? KIND : 0;
+  constexpr int CAPE = (DP >= 2 && DP <= 4) ? cap_for<KIND>() : 0;
+  using G = GatherLds<kBTX, kBTY, kBTZ, CAPE, KG>;
+  constexpr int BOXF = KIND <= 2 ? G::BOXF : 2 * CAPE;  // push: CAPE int64 cells
+  __shared__ __align__(8) float box[BOXF > 0 ? BOXF : 2];
   __shared__ float acc[G::NACC];
+  __shared__ float rbuf[G::NR];
   __shared__ TileState S;
   __shared__ float wsum[kTB / 64];
   const int nb = gridDim.x / kXcds;  // the host launches a multiple of 8 workgroups
@@ -576,13 +652,8 @@ __global__ __launch_bounds__(kTB) void k_batch(const BatchVol* __restrict__ vols
     }
     const KArgs a = vols[lo].a;
     const int tile = t - vols[lo].tile0;
-    switch (vols[lo].dp) {
-      case 1: batch_tile<1, KIND>(a, tile, box, box2, acc, wsum, S); break;
-      case 2: batch_tile<2, KIND>(a, tile, box, box2, acc, wsum, S); break;
-      case 3: batch_tile<3, KIND>(a, tile, box, box2, acc, wsum, S); break;
-      case 4: batch_tile<4, KIND>(a, tile, box, box2, acc, wsum, S); break;
-      default: batch_tile<5, KIND>(a, tile, box, box2, acc, wsum, S); break;
-    }
+    if (KIND <= 2) gather_tile<DP, kBTX, kBTY, kBTZ, CAPE, KG>(a, tile, 0, box, acc, rbuf, wsum, S);
+    else push_tile<DP, kBTX, kBTY, kBTZ, CAPE, KIND == 4>(a, tile, 0, reinterpret_cast<acc_t*>(box), wsum, S);
   }
 }
 
@@ -598,36 +669,46 @@ __global__ void k_batch_zero(const BatchVol* __restrict__ vols, int kind) {
 
 int batch_tiles(const KArgs& a) { return tiles_of<kBTX, kBTY, kBTZ>(a); }
 
-hipError_t launch_batch(Op op, const BatchVol* dvols, int nvol, int total, int grid, bool zero, hipStream_t s) {
-  if (nvol <= 0 || total <= 0) return hipSuccess;
-  if (zero && (op == OP_PUSH || op == OP_PUSHG || op == OP_OBJGRAD)) {
-    hipLaunchKernelGGL(k_batch_zero, dim3(512, (unsigned)nvol), dim3(256), 0, s, dvols, (int)op);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+template <int KIND>
+static const void* batch_kernel(int dp) {
+  switch (dp) {
+    case 1: return reinterpret_cast<const void*>(&k_batch<1, KIND>);
+    case 2: return reinterpret_cast<const void*>(&k_batch<2, KIND>);
+    case 3: return reinterpret_cast<const void*>(&k_batch<3, KIND>);
+    case 4: return reinterpret_cast<const void*>(&k_batch<4, KIND>);
+    default: return reinterpret_cast<const void*>(&k_batch<5, KIND>);
   }
-  const dim3 g((unsigned)grid), b(kTB);
+}
+
+static const void* batch_kernel_for(Op op, int dp) {
   switch (op) {
-    case OP_PULL: hipLaunchKernelGGL(k_batch<0>, g, b, 0, s, dvols, nvol, total); break;
-    case OP_PULL_BWD: hipLaunchKernelGGL(k_batch<1>, g, b, 0, s, dvols, nvol, total); break;
-    case OP_OBJGRAD: hipLaunchKernelGGL(k_batch<2>, g, b, 0, s, dvols, nvol, total); break;
-    case OP_PUSH: hipLaunchKernelGGL(k_batch<3>, g, b, 0, s, dvols, nvol, total); break;
-    case OP_PUSHG: hipLaunchKernelGGL(k_batch<4>, g, b, 0, s, dvols, nvol, total); break;
-    default: return hipErrorNotSupported;
+    case OP_PULL: return batch_kernel<0>(dp);
+    case OP_PULL_BWD: return batch_kernel<1>(dp);
+    case OP_OBJGRAD: return batch_kernel<2>(dp);
+    case OP_PUSH: return batch_kernel<3>(dp);
+    case OP_PUSHG: return batch_kernel<4>(dp);
+    default: return nullptr;
   }
+}
+
+hipError_t launch_batch_zero(Op op, const BatchVol* dvols, int nvol, hipStream_t s) {
+  if (nvol <= 0 || !(op == OP_PUSH || op == OP_PUSHG || op == OP_OBJGRAD)) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_zero, dim3(512, (unsigned)nvol), dim3(256), 0, s, dvols, (int)op);
   return hipGetLastError();
 }
 
-int batch_grid(Op op) {
+hipError_t launch_batch_group(Op op, int dp, const BatchVol* dvols, int nvol, int total, int grid, hipStream_t s) {
+  if (nvol <= 0 || total <= 0) return hipSuccess;
+  const void* k = batch_kernel_for(op, dp);
+  if (!k) return hipErrorNotSupported;
+  void* args[] = {(void*)&dvols, (void*)&nvol, (void*)&total};
+  return hipLaunchKernel(k, dim3((unsigned)grid), dim3(kTB), args, 0, s);
+}
+
+int batch_grid(Op op, int dp) {
+  const void* k = batch_kernel_for(op, dp);
   int per_cu = 0;
-  hipError_t e = hipErrorNotSupported;
-  switch (op) {
-    case OP_PULL: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<0>, kTB, 0); break;
-    case OP_PULL_BWD: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<1>, kTB, 0); break;
-    case OP_OBJGRAD: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<2>, kTB, 0); break;
-    case OP_PUSH: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<3>, kTB, 0); break;
-    case OP_PUSHG: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<4>, kTB, 0); break;
-    default: break;
-  }
+  hipError_t e = k ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kTB, 0) : hipErrorNotSupported;
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess || per_cu < 1) per_cu = 1;
