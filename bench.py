@@ -31,6 +31,23 @@ N = 256
 DEGS = (1, 2, 3)
 HBM_PEAK_GBS = 8000.0
 BYTES = {"pull": 20, "push": 20, "backward": 32}  # per voxel at C = 1
+KERNEL_NAMES = {"pull": "k_gather_tile<pull>", "push": "k_push_tile", "backward": "k_gather_tile<pullback>"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_256cubed.txt")
+
+
+def _traffic(kernel, deg):
+    """Calibrated HBM bytes per launch of `kernel` at `deg` on this workload, from the
+    committed rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE runs of this script),
+    or None when it has no line for the kernel."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                p = line.split()
+                if len(p) >= 5 and not line.startswith("#") and p[0] == kernel and p[1] == f"deg{deg}":
+                    return int(p[4])
+    except OSError:
+        pass
+    return None
 
 
 def _progress(msg):
@@ -296,7 +313,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched-uneven and matching-loop legs")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run a 256 MiB device copy (reference kernel for HBM counter scaling)")
+                    help="also run k_fill and k_axpy over 64 Mi floats (reference kernels for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -319,10 +336,15 @@ def main():
     _lib.set_option(_lib.PP_OPT_TILING, args.tiling)
     _lib.set_option(_lib.PP_OPT_TILE_SHAPE, args.shape)
     if args.copy_calibration:
-        src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev).normal_()
+        # counter calibration: y += a*x over 64 Mi floats reads 512 MiB and writes 256 MiB
+        # (k_axpy), and the zero fill of 64 Mi floats writes 256 MiB (k_fill)
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
+        cs = ct.c_void_p(torch.cuda.current_stream().cuda_stream)
         for _ in range(3):
-            dst.copy_(src)
+            _lib.check(L.pp_memset_zero(ct.c_void_p(dst.data_ptr()), dst.numel(), cs), "fill")
+            _lib.check(L.pp_axpy(ct.c_void_p(dst.data_ptr()), ct.c_void_p(src.data_ptr()), 0.5, src.numel(), cs),
+                       "axpy")
         torch.cuda.synchronize()
         del src, dst
     descs = {deg: ops._desc(pp.Settings(deg, 1, 1), (N, N, N), (N, N, N), 1, 1) for deg in DEGS}
@@ -399,7 +421,17 @@ def main():
         detail[f"deg{deg}"] = row
 
     value = len(DEGS) * n1 / (ms_step * 1e-3) / 1e9
-    deg, op, ms, gbs = dominant
+    deg, op, ms_op, _ = dominant
+    # The dominant op's kernel alone (push without its zero fill), timed with events on the
+    # stream it is launched on, as rocprofv3 --kernel-trace reports the same kernel.
+    d = ct.byref(descs[deg])
+    kcall = {
+        "pull": lambda: L.pp_pull(d, P(f1), P(phi), P(f), stream),
+        "push": lambda: L.pp_push(d, P(df), P(phi), P(delta), 1, stream),
+        "backward": lambda: L.pp_pull_backward(d, P(dphi), P(phi), P(f), P(delta), stream),
+    }[op]
+    ms = _time_ms(torch, lambda: _lib.check(kcall(), op), 20, None)
+    gbs = BYTES[op] * n1 / (ms * 1e-3) / 1e9
     res = {
         "metric": "Gvoxel/s pull+push+backward at 256^3 FP32 deg1-3",
         "value": round(value, 3),
@@ -416,9 +448,10 @@ def main():
         "config": {"workload": "pull+push+backward, 256^3 x 1 channel, degrees 1,2,3 (one triple each per step), "
                                "reflect boundary, ext=1", "volume": [N, N, N], "channels": 1, "degrees": list(DEGS),
                    "kernels": "tiled" if args.tiling else "per-sample", "tile_shape": args.shape},
-        "roofline": {"bound": "hbm", "kernel": f"{op} deg{deg}", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_launch": BYTES[op] * n1, "avg_ms": round(ms, 4)},
+        "roofline": {"bound": "hbm", "kernel": f"{KERNEL_NAMES[op]} deg{deg}", "achieved": round(gbs, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                     "traffic": _traffic(KERNEL_NAMES[op], deg), "bytes_per_launch": BYTES[op] * n1,
+                     "avg_ms": round(ms, 4), "op_ms_with_fill": round(ms_op, 4)},
         "per_degree": detail,
         "wall_s": round(wall, 3),
     }

@@ -54,12 +54,12 @@ __device__ __forceinline__ bool setup(const KArgs& a, const float* phi, int v, A
 #define PP_TAP(DP, ax, i) if (DP > 0 || (i) < (ax).n)
 
 // Value: sum_ijk wx_i wy_j wz_k f[...]
-template <int DP>
+template <int DP, bool LEAN = false>
 __device__ __forceinline__ float gather_val(const float* __restrict__ f, const Axis<DP>& ax, const Axis<DP>& ay,
                                             const Axis<DP>& az) {
   constexpr int NP = Axis<DP>::NP;
   float acc = 0.f;
-#pragma unroll
+#pragma unroll(LEAN ? 1 : NP)
   for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
 #pragma unroll
       for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
@@ -74,12 +74,12 @@ __device__ __forceinline__ float gather_val(const float* __restrict__ f, const A
 }
 
 // Gradient: (d/dx, d/dy, d/dz)
-template <int DP>
+template <int DP, bool LEAN = false>
 __device__ __forceinline__ void gather_grad(const float* __restrict__ f, const Axis<DP>& ax, const Axis<DP>& ay,
                                             const Axis<DP>& az, float& gx, float& gy, float& gz) {
   constexpr int NP = Axis<DP>::NP;
   gx = gy = gz = 0.f;
-#pragma unroll
+#pragma unroll(LEAN ? 1 : NP)
   for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
 #pragma unroll
       for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
@@ -99,12 +99,12 @@ __device__ __forceinline__ void gather_grad(const float* __restrict__ f, const A
 }
 
 // Value and gradient from one footprint
-template <int DP>
+template <int DP, bool LEAN = false>
 __device__ __forceinline__ void gather_val_grad(const float* __restrict__ f, const Axis<DP>& ax, const Axis<DP>& ay,
                                                 const Axis<DP>& az, float& val, float& gx, float& gy, float& gz) {
   constexpr int NP = Axis<DP>::NP;
   val = gx = gy = gz = 0.f;
-#pragma unroll
+#pragma unroll(LEAN ? 1 : NP)
   for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
 #pragma unroll
       for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
@@ -166,6 +166,7 @@ __device__ __forceinline__ float dot_ref(const float* __restrict__ f, const Axis
                                          const Axis<DP>& az, const float* a, const float* b, const float* c) {
   constexpr int NP = Axis<DP>::NP;
   float acc = 0.f;
+#pragma unroll 1
   for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
       for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
           const float wjj = c[k] * b[j];
@@ -181,6 +182,7 @@ __device__ __forceinline__ void grad_ref(const float* __restrict__ f, const Axis
                                          const Axis<DP>& az, float& g0, float& g1, float& g2) {
   constexpr int NP = Axis<DP>::NP;
   g0 = g1 = g2 = 0.f;
+#pragma unroll 1
   for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
       for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
           const float wx = az.w[k] * ay.w[j], wy = az.w[k] * ay.dw[j], wz = az.dw[k] * ay.w[j];
@@ -214,26 +216,28 @@ __device__ __forceinline__ bool is_slow(const Axis<DP>& ax, const Axis<DP>& ay, 
 }
 
 // Value / gradient / value+gradient / Hessian of one sample, picking the contraction
-template <int DP>
+// LEAN: one z-plane of taps in flight at a time (for rarely taken paths that must not
+// set a kernel's register budget).
+template <int DP, bool LEAN = false>
 __device__ __forceinline__ float sample_val(const float* f, const Axis<DP>& ax, const Axis<DP>& ay,
                                             const Axis<DP>& az) {
   if (is_slow(ax, ay, az)) return dot_ref<DP>(f, ax, ay, az, ax.w, ay.w, az.w);
-  return gather_val<DP>(f, ax, ay, az);
+  return gather_val<DP, LEAN>(f, ax, ay, az);
 }
-template <int DP>
+template <int DP, bool LEAN = false>
 __device__ __forceinline__ void sample_grad(const float* f, const Axis<DP>& ax, const Axis<DP>& ay,
                                             const Axis<DP>& az, float& gx, float& gy, float& gz) {
   if (is_slow(ax, ay, az)) grad_ref<DP>(f, ax, ay, az, gx, gy, gz);
-  else gather_grad<DP>(f, ax, ay, az, gx, gy, gz);
+  else gather_grad<DP, LEAN>(f, ax, ay, az, gx, gy, gz);
 }
-template <int DP>
+template <int DP, bool LEAN = false>
 __device__ __forceinline__ void sample_val_grad(const float* f, const Axis<DP>& ax, const Axis<DP>& ay,
                                                 const Axis<DP>& az, float& v, float& gx, float& gy, float& gz) {
   if (is_slow(ax, ay, az)) {
     v = dot_ref<DP>(f, ax, ay, az, ax.w, ay.w, az.w);
     grad_ref<DP>(f, ax, ay, az, gx, gy, gz);
   } else {
-    gather_val_grad<DP>(f, ax, ay, az, v, gx, gy, gz);
+    gather_val_grad<DP, LEAN>(f, ax, ay, az, v, gx, gy, gz);
   }
 }
 template <int DP>

@@ -39,15 +39,40 @@ struct Shape {
   static_assert(N % kTB == 0, "a tile is a whole number of workgroup passes");
   // window half-width: a knot farther than this from the centre sample's is an outlier
   static constexpr int WX = TX + 8, WY = TY + 8, WZ = TZ + 8;
+  // a box row (<= 2 * WX + dp cells) fits in one workgroup pass (box_walk relies on it)
+  static_assert(2 * WX + 5 < kTB, "box rows must be shorter than the workgroup");
   // local index of the centre sample
   static constexpr int LC = TX / 2 + TX * (TY / 2 + TY * (TZ / 2));
 };
+
+// Cyclic rotation by one element. Loops over a thread's samples that are not unrolled
+// read element 0 and rotate once per iteration (VPT rotations restore the order), so the
+// per-sample arrays stay in registers: a runtime index would put them in scratch memory.
+template <typename T, int N>
+__device__ __forceinline__ void rot(T (&a)[N]) {
+  const T t = a[0];
+#pragma unroll
+  for (int i = 0; i < N - 1; ++i) a[i] = a[i + 1];
+  a[N - 1] = t;
+}
+
+template <typename T, int M, int N>
+__device__ __forceinline__ void rot_rows(T (&a)[M][N]) {
+#pragma unroll
+  for (int m = 0; m < M; ++m) rot(a[m]);
+}
 
 template <int VPT>
 struct Vox {
   float x[VPT], y[VPT], z[VPT];
   int v[VPT];
   unsigned inside, valid, inl;  // bit q: position inside the phi grid / valid sample / in-window
+  __device__ __forceinline__ void rotate() {
+    rot(x);
+    rot(y);
+    rot(z);
+    rot(v);
+  }
 };
 
 __device__ __forceinline__ int wave_min(int v) {
@@ -160,22 +185,58 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
   __syncthreads();
 }
 
+// The box is walked by the whole workgroup: thread t owns column t % BX of rows
+// t / BX, t / BX + RP, ... (RP = kTB / BX rows per pass; BX <= 2 * WX + DP < kTB), and
+// issues kBatch rows' accesses before it uses any of them, so a wave has that many
+// global loads in flight instead of waiting out each one in turn.
+constexpr int kBatch = 16;
+
+struct BoxWalk {
+  int c, r0, rp;  // column, first row, rows per pass; r0 >= rp: this thread idles
+};
+
+__device__ __forceinline__ BoxWalk box_walk(const TileState& S) {
+  const int BX = S.b[0];
+  BoxWalk w;
+  w.c = threadIdx.x % BX;
+  w.r0 = threadIdx.x / BX;
+  w.rp = kTB / BX;
+  return w;
+}
+
+// Folded image offset of box row r (rows run over y, then z) and its boundary sign.
+__device__ __forceinline__ int row_offset(const TileState& S, const KArgs& a, int r, bool& neg) {
+  bool ny, nz;
+  const int fy = fold32(a.by, S.lo[1] + r % S.b[1], a.d0y, ny);
+  const int fz = fold32(a.bz, S.lo[2] + r / S.b[1], a.d0z, nz);
+  neg = ny ^ nz;
+  return a.d0x * (fy + a.d0y * fz);
+}
+
 // Stage the box from image channel `src` (fold + boundary sign applied here).
 __device__ __forceinline__ void stage_box(float* box, const TileState& S, const float* __restrict__ src,
                                           const KArgs& a) {
-  const int BX = S.b[0], BY = S.b[1], rows = S.b[1] * S.b[2];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int r = wave; r < rows; r += kTB / 64) {
-    bool ny, nz;
-    const int fy = fold32(a.by, S.lo[1] + r % BY, a.d0y, ny);
-    const int fz = fold32(a.bz, S.lo[2] + r / BY, a.d0z, nz);
-    const float* srow = src + a.d0x * (fy + a.d0y * fz);
-    float* brow = box + r * BX;
-    for (int c = lane; c < BX; c += 64) {
-      bool nx;
-      const int fx = fold32(a.bx, S.lo[0] + c, a.d0x, nx);
-      const float v = srow[fx];
-      brow[c] = (nx ^ ny ^ nz) ? -v : v;
+  const BoxWalk w = box_walk(S);
+  if (w.r0 >= w.rp) return;
+  const int BX = S.b[0], rows = S.b[1] * S.b[2];
+  bool nx;
+  const int fx = fold32(a.bx, S.lo[0] + w.c, a.d0x, nx);
+  for (int rb = w.r0; rb < rows; rb += kBatch * w.rp) {
+    float v[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int r = rb + u * w.rp;
+      v[u] = 0.f;
+      if (r < rows) {
+        bool neg;
+        const float t = src[fx + row_offset(S, a, r, neg)];
+        v[u] = (neg ^ nx) ? -t : t;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int r = rb + u * w.rp;
+      if (r < rows) box[r * BX + w.c] = v[u];
     }
   }
 }
@@ -219,22 +280,18 @@ __device__ __forceinline__ float block_max(float v, float* wsum) {
 // non-zero cell.
 __device__ __forceinline__ void flush_acc(const acc_t* box, const TileState& S, int scale, float* __restrict__ dst,
                                           const KArgs& a) {
-  const int BX = S.b[0], BY = S.b[1], rows = S.b[1] * S.b[2];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int r = wave; r < rows; r += kTB / 64) {
-    bool ny, nz;
-    const int fy = fold32(a.by, S.lo[1] + r % BY, a.d0y, ny);
-    const int fz = fold32(a.bz, S.lo[2] + r / BY, a.d0z, nz);
-    float* drow = dst + a.d0x * (fy + a.d0y * fz);
-    const acc_t* brow = box + r * BX;
-    for (int c = lane; c < BX; c += 64) {
-      const long long q = (long long)brow[c];
-      if (q != 0) {
-        bool nx;
-        const int fx = fold32(a.bx, S.lo[0] + c, a.d0x, nx);
-        const float v = ldexpf((float)q, -scale);
-        unsafeAtomicAdd(drow + fx, (nx ^ ny ^ nz) ? -v : v);
-      }
+  const BoxWalk w = box_walk(S);
+  if (w.r0 >= w.rp) return;
+  const int BX = S.b[0], rows = S.b[1] * S.b[2];
+  bool nx;
+  const int fx = fold32(a.bx, S.lo[0] + w.c, a.d0x, nx);
+  for (int r = w.r0; r < rows; r += w.rp) {
+    const long long q = (long long)box[r * BX + w.c];
+    if (q != 0) {
+      bool neg;
+      const int off = row_offset(S, a, r, neg);
+      const float v = ldexpf((float)q, -scale);
+      unsafeAtomicAdd(dst + fx + off, (neg ^ nx) ? -v : v);
     }
   }
 }
@@ -256,51 +313,38 @@ __device__ __forceinline__ void lsample(float x, float y, float z, const TileSta
   L.base = (ox - S.lo[0]) + S.b[0] * ((oy - S.lo[1]) + S.b[1] * (oz - S.lo[2]));
 }
 
-// Phi-grid index of sample q of this thread (in: inside the grid)
-template <int TX, int TY, int TZ>
-__device__ __forceinline__ int tile_voxel(const KArgs& a, const TileState& S, int q, bool& in) {
-  const int l = threadIdx.x + kTB * q;
-  const int gx = S.org[0] + l % TX, gy = S.org[1] + (l / TX) % TY, gz = S.org[2] + l / (TX * TY);
-  in = gx < a.d1x && gy < a.d1y && gz < a.d1z;
-  return in ? gx + a.d1x * (gy + a.d1y * gz) : 0;
-}
-
-__device__ __forceinline__ void load_x(const KArgs& a, const float* __restrict__ phi, int v, float& x, float& y,
-                                       float& z) {
-  sample_coords(a, phi, v, x, y, z);
-}
-
 // ---- push / push_grad -------------------------------------------------------------
 // One tile of push (GRAD=false) or push_grad (GRAD=true) for batch index bb. The caller
 // provides the int64 LDS box (CAP cells), wsum (kTB/64 floats) and the tile state.
 template <int DP, int TX, int TY, int TZ, int CAP, bool GRAD>
 __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, acc_t* box, float* wsum, TileState& S) {
   constexpr int VPT = Shape<TX, TY, TZ>::VPT;
+  constexpr int NG = GRAD ? 3 : 1;
   const float* phi = a.phi + bb * a.phi_bs;
-  unsigned valid, inl;
-  {
-    Vox<VPT> V;
-    tile_setup<DP, TX, TY, TZ, CAP>(a, phi, tile, V, S);
-    valid = V.valid;
-    inl = V.inl;
-  }
+  Vox<VPT> V;
+  tile_setup<DP, TX, TY, TZ, CAP>(a, phi, tile, V, S);
   const bool lds = S.lds != 0;
   const int BX = S.b[0], BXY = S.b[0] * S.b[1];
   const long long n1 = a.n1;
   for (int c = 0; c < a.nchan; ++c) {
     const float* src = a.f + bb * a.f_bs + c * a.f_cs;
     float* dst = a.out + bb * a.out_bs + c * a.out_cs;
-    unsigned use = lds ? inl : 0u;  // samples accumulated in LDS for this channel
+    // this channel's values of all the thread's valid samples, loaded together
+    float g[NG][VPT];
+#pragma unroll
+    for (int q = 0; q < VPT; ++q)
+#pragma unroll
+      for (int e = 0; e < NG; ++e) g[e][q] = ((V.valid >> q) & 1u) ? src[V.v[q] + e * n1] : 0.f;
+    unsigned use = lds ? V.inl : 0u;  // samples accumulated in LDS for this channel
     int scale = 0;
     if (lds) {
       // bound on |contribution| (|w|, |dw| <= 1); non-finite values take the global path
       float vmax = 0.f;
-#pragma unroll 1
+#pragma unroll
       for (int q = 0; q < VPT; ++q) {
+        float b = fabsf(g[0][q]);
+        if (GRAD) b += fabsf(g[NG > 1 ? 1 : 0][q]) + fabsf(g[NG > 2 ? 2 : 0][q]);
         if (!((use >> q) & 1u)) continue;
-        bool in;
-        const int v = tile_voxel<TX, TY, TZ>(a, S, q, in);
-        const float b = GRAD ? fabsf(src[v]) + fabsf(src[v + n1]) + fabsf(src[v + 2 * n1]) : fabsf(src[v]);
         if (isfinite(b)) vmax = fmaxf(vmax, b);
         else use &= ~(1u << q);
       }
@@ -309,16 +353,13 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, acc_
       __syncthreads();
     }
 #pragma unroll 1
-    for (int q = 0; q < VPT; ++q) {
-      if (!((valid >> q) & 1u)) continue;
-      bool in;
-      const int v = tile_voxel<TX, TY, TZ>(a, S, q, in);
-      const float g0 = src[v];
-      const float g1 = GRAD ? src[v + n1] : 0.f;
-      const float g2 = GRAD ? src[v + 2 * n1] : 0.f;
+    for (int q = 0; q < VPT; ++q, V.rotate(), rot_rows(g)) {
+      if (!((V.valid >> q) & 1u)) continue;
+      const float g0 = g[0][0];
+      const float g1 = GRAD ? g[NG > 1 ? 1 : 0][0] : 0.f;
+      const float g2 = GRAD ? g[NG > 2 ? 2 : 0][0] : 0.f;
       if (!GRAD && !isfinite(g0)) continue;  // push skips non-finite values (pushpull.c:155)
-      float x, y, z;
-      load_x(a, phi, v, x, y, z);
+      const float x = V.x[0], y = V.y[0], z = V.z[0];
       if ((use >> q) & 1u) {
         LSample<DP> L;
         lsample<DP, GRAD>(x, y, z, S, L);
@@ -406,62 +447,74 @@ __device__ __forceinline__ void lds_grad(const float* box, const LSample<DP>& L,
     }
 }
 
-// LDS a gather kind needs, in floats. The objective's box is twice as large: it first
+// Gradient gather of one sample outside the tile's box, through global memory; for the
+// objective (KIND 2) it also scatters the residual. Returns the weight w of the pullback
+// (the upstream gradient, or the residual). Rarely taken, so it gathers lean: it must not
+// set the register budget of the whole kernel.
+template <int DP, int KIND>
+__device__ __forceinline__ float outlier_grad(const KArgs& a, const float* __restrict__ src, float* __restrict__ dst2,
+                                           float x, float y, float z, float aux, float& gx, float& gy, float& gz) {
+  Axis<DP> ax, ay, az;
+  make_axis<DP, kW | kDW>(x, DP, a.d0x, 1, a.bx, ax);
+  make_axis<DP, kW | kDW>(y, DP, a.d0y, a.d0x, a.by, ay);
+  make_axis<DP, kW | kDW>(z, DP, a.d0z, a.d0x * a.d0y, a.bz, az);
+  float val = 0.f;
+  if (KIND == 2) sample_val_grad<DP, true>(src, ax, ay, az, val, gx, gy, gz);
+  else sample_grad<DP, true>(src, ax, ay, az, gx, gy, gz);
+  const float w = (KIND == 1) ? aux : val - aux;
+  if (KIND == 2 && isfinite(w)) scatter_val<DP>(dst2, ax, ay, az, w);
+  return w;
+}
+
+// LDS box of a gather kind, in floats. The objective's box is twice as large: it first
 // holds the staged image (float) and then, reused, the int64 scatter accumulator.
-template <int TX, int TY, int TZ, int CAP, int KIND>
-struct GatherLds {
-  static constexpr int BOXF = KIND == 2 ? 2 * CAP : CAP;         // box
-  static constexpr int NACC = KIND == 0 ? 1 : 3 * Shape<TX, TY, TZ>::N;  // per-sample pullback
-  static constexpr int NR = KIND == 2 ? Shape<TX, TY, TZ>::N : 1;         // per-sample residual
-};
+template <int CAP, int KIND>
+constexpr int box_floats() {
+  return KIND == 2 ? 2 * CAP : CAP;
+}
 
 // One tile of pull (KIND 0), the pull pullback (1) or the fused objective (2).
-// box: GatherLds::BOXF floats (8-byte aligned); acc: 3*N floats for KIND 1/2;
-// rbuf: N floats for KIND 2; wsum: kTB/64 floats.
+// box: box_floats<CAP, KIND>() floats (8-byte aligned); wsum: kTB/64 floats.
+// The pullback of every sample is summed over channels in registers (gsum), and the
+// objective keeps each sample's residual in registers between its two passes.
 template <int DP, int TX, int TY, int TZ, int CAP, int KIND>
-__device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, float* box, float* acc, float* rbuf,
-                                            float* wsum, TileState& S) {
-  constexpr int N = Shape<TX, TY, TZ>::N;
+__device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, float* box, float* wsum,
+                                            TileState& S) {
   constexpr int VPT = Shape<TX, TY, TZ>::VPT;
   const float* phi = a.phi + bb * a.phi_bs;
-  unsigned inside, valid, inl;
-  {
-    Vox<VPT> V;
-    tile_setup<DP, TX, TY, TZ, CAP>(a, phi, tile, V, S);
-    inside = V.inside;
-    valid = V.valid;
-    inl = V.inl;
-  }
+  Vox<VPT> V;
+  tile_setup<DP, TX, TY, TZ, CAP>(a, phi, tile, V, S);
   const bool lds = S.lds != 0;
   const int BX = S.b[0], BXY = S.b[0] * S.b[1];
   const long long n1 = a.n1;
-  if (KIND != 0) {
+  float gsum[3][VPT];
 #pragma unroll
-    for (int q = 0; q < VPT; ++q) {
-      const int l = threadIdx.x + kTB * q;
-      acc[l] = acc[N + l] = acc[2 * N + l] = 0.f;
-    }
-  }
+  for (int q = 0; q < VPT; ++q) gsum[0][q] = gsum[1][q] = gsum[2][q] = 0.f;
   float lsum = 0.f;
   for (int c = 0; c < a.nchan; ++c) {
     const float* src = a.f + bb * a.f_bs + c * a.f_cs;
     unsigned rfin = 0u;  // KIND 2: in-window samples with a finite residual (scattered via LDS)
+    float auxv[VPT];     // upstream gradient / target, loaded before staging so both overlap
+    float res[VPT];      // KIND 2: residual of each sample
+#pragma unroll
+    for (int q = 0; q < VPT; ++q) {
+      auxv[q] = (KIND != 0 && ((V.valid >> q) & 1u)) ? a.aux[bb * a.aux_bs + c * a.aux_cs + V.v[q]] : 0.f;
+      res[q] = 0.f;
+    }
     if (lds) {
       stage_box(box, S, src, a);
       __syncthreads();
     }
 #pragma unroll 1
-    for (int q = 0; q < VPT; ++q) {
-      if (!((inside >> q) & 1u)) continue;
-      bool in;
-      const int v = tile_voxel<TX, TY, TZ>(a, S, q, in);
-      if (!((valid >> q) & 1u)) {
+    for (int q = 0; q < VPT; ++q, V.rotate(), rot(auxv), rot(res), rot_rows(gsum)) {
+      if (!((V.inside >> q) & 1u)) continue;
+      const int v = V.v[0];
+      if (!((V.valid >> q) & 1u)) {
         if (KIND == 0) a.out[bb * a.out_bs + c * a.out_cs + v] = __builtin_nanf("");
         continue;
       }
-      float x, y, z;
-      load_x(a, phi, v, x, y, z);
-      const bool use_lds = lds && ((inl >> q) & 1u);
+      const float x = V.x[0], y = V.y[0], z = V.z[0];
+      const bool use_lds = lds && ((V.inl >> q) & 1u);
       if (KIND == 0) {
         float val;
         if (use_lds) {
@@ -473,12 +526,12 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
           make_axis<DP, kW>(x, DP, a.d0x, 1, a.bx, ax);
           make_axis<DP, kW>(y, DP, a.d0y, a.d0x, a.by, ay);
           make_axis<DP, kW>(z, DP, a.d0z, a.d0x * a.d0y, a.bz, az);
-          val = sample_val<DP>(src, ax, ay, az);
+          val = sample_val<DP, true>(src, ax, ay, az);
         }
         a.out[bb * a.out_bs + c * a.out_cs + v] = val;
         continue;
       }
-      const float aux = a.aux[bb * a.aux_bs + c * a.aux_cs + v];
+      const float aux = auxv[0];
       float val = 0.f, gx, gy, gz, w;
       if (use_lds) {
         LSample<DP> L;
@@ -486,46 +539,34 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
         lds_grad<DP, KIND == 2>(box, L, BX, BXY, val, gx, gy, gz);
         w = (KIND == 1) ? aux : val - aux;
         if (KIND == 2 && isfinite(w)) {
-          rbuf[threadIdx.x + kTB * q] = w;
+          res[0] = w;
           rfin |= 1u << q;
         }
       } else {
-        Axis<DP> ax, ay, az;
-        make_axis<DP, kW | kDW>(x, DP, a.d0x, 1, a.bx, ax);
-        make_axis<DP, kW | kDW>(y, DP, a.d0y, a.d0x, a.by, ay);
-        make_axis<DP, kW | kDW>(z, DP, a.d0z, a.d0x * a.d0y, a.bz, az);
-        if (KIND == 2) sample_val_grad<DP>(src, ax, ay, az, val, gx, gy, gz);
-        else sample_grad<DP>(src, ax, ay, az, gx, gy, gz);
-        w = (KIND == 1) ? aux : val - aux;
-        if (KIND == 2 && isfinite(w)) scatter_val<DP>(a.out2 + bb * a.out2_bs + c * a.out2_cs, ax, ay, az, w);
+        w = outlier_grad<DP, KIND>(a, src, a.out2 + bb * a.out2_bs + c * a.out2_cs, x, y, z, aux, gx, gy, gz);
       }
       if (KIND == 2 && isfinite(w)) lsum += 0.5f * w * w;
-      const int l = threadIdx.x + kTB * q;
-      acc[l] += w * gx;
-      acc[N + l] += w * gy;
-      acc[2 * N + l] += w * gz;
+      gsum[0][0] += w * gx;
+      gsum[1][0] += w * gy;
+      gsum[2][0] += w * gz;
     }
     if (lds) {
       __syncthreads();
       if (KIND == 2) {  // scatter the residuals of the in-window samples through the int64 box
         float vmax = 0.f;
-#pragma unroll 1
+#pragma unroll
         for (int q = 0; q < VPT; ++q)
-          if ((rfin >> q) & 1u) vmax = fmaxf(vmax, fabsf(rbuf[threadIdx.x + kTB * q]));
+          if ((rfin >> q) & 1u) vmax = fmaxf(vmax, fabsf(res[q]));
         const int scale = fixed_scale(block_max(vmax, wsum));  // also orders the staging reads
         acc_t* box64 = reinterpret_cast<acc_t*>(box);
         zero_acc(box64, S.cells);
         __syncthreads();
 #pragma unroll 1
-        for (int q = 0; q < VPT; ++q) {
+        for (int q = 0; q < VPT; ++q, V.rotate(), rot(res)) {
           if (!((rfin >> q) & 1u)) continue;
-          bool in;
-          const int v = tile_voxel<TX, TY, TZ>(a, S, q, in);
-          float x, y, z;
-          load_x(a, phi, v, x, y, z);
           LSample<DP> L;
-          lsample<DP, false>(x, y, z, S, L);
-          const float w = rbuf[threadIdx.x + kTB * q];
+          lsample<DP, false>(V.x[0], V.y[0], V.z[0], S, L);
+          const float w = res[0];
 #pragma unroll
           for (int k = 0; k < DP; ++k)
 #pragma unroll
@@ -545,12 +586,10 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
   if (KIND != 0) {
     const float nan = __builtin_nanf("");
 #pragma unroll 1
-    for (int q = 0; q < VPT; ++q) {
-      if (!((inside >> q) & 1u)) continue;
-      bool in;
-      const int v = tile_voxel<TX, TY, TZ>(a, S, q, in);
-      const int l = threadIdx.x + kTB * q;
-      if ((valid >> q) & 1u) put3(a.out + bb * a.out_bs + v, n1, acc[l], acc[N + l], acc[2 * N + l], a.accum);
+    for (int q = 0; q < VPT; ++q, V.rotate(), rot_rows(gsum)) {
+      if (!((V.inside >> q) & 1u)) continue;
+      const int v = V.v[0];
+      if ((V.valid >> q) & 1u) put3(a.out + bb * a.out_bs + v, n1, gsum[0][0], gsum[1][0], gsum[2][0], a.accum);
       else put3(a.out + bb * a.out_bs + v, n1, nan, nan, nan, 0);
     }
   }
@@ -569,24 +608,21 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
 
 template <int DP, int TX, int TY, int TZ, int CAP, int KIND>
 __global__ __launch_bounds__(kTB) void k_gather_tile(KArgs a) {
-  using G = GatherLds<TX, TY, TZ, CAP, KIND>;
-  __shared__ __align__(8) float box[G::BOXF];
-  __shared__ float acc[G::NACC];  // per-sample phi pullback, summed over channels
-  __shared__ float rbuf[G::NR];
+  __shared__ __align__(8) float box[box_floats<CAP, KIND>()];
   __shared__ TileState S;
   __shared__ float wsum[kTB / 64];
-  gather_tile<DP, TX, TY, TZ, CAP, KIND>(a, xcd_block(blockIdx.x, gridDim.x), blockIdx.y, box, acc, rbuf, wsum, S);
+  gather_tile<DP, TX, TY, TZ, CAP, KIND>(a, xcd_block(blockIdx.x, gridDim.x), blockIdx.y, box, wsum, S);
 }
 
 // ---- launcher ------------------------------------------------------------------
-// Box budgets in cells: gathers stage 8192 floats (32 KiB); scatters accumulate 6144
-// int64 cells (48 KiB); the objective stages and accumulates in one 6144-cell region.
-constexpr int kCap = 8192;
-constexpr int kCapAcc = 6144;
-
-template <int KIND>
-constexpr int cap_for() {
-  return KIND == 0 || KIND == 1 ? kCap : kCapAcc;
+// Box budgets in cells, per degree: small boxes let more workgroups share a CU, whose
+// latency chains (load phi, size the box, stage, compute) then overlap. A tile whose box
+// exceeds the budget runs per sample through global memory. Gathers stage floats;
+// scatters and the objective accumulate int64 cells.
+template <int DP, int KIND>
+constexpr int cap_cells() {
+  return (KIND == 0 || KIND == 1) ? (DP <= 2 ? 4096 : DP == 3 ? 6144 : 8192)
+                                  : (DP <= 2 ? 3072 : DP == 3 ? 4096 : 6144);
 }
 
 template <int TX, int TY, int TZ>
@@ -598,11 +634,21 @@ template <int DP, int TX, int TY, int TZ>
 static hipError_t launch_shape(Op op, const KArgs& a, int nbatch, hipStream_t s) {
   const dim3 grid((unsigned)tiles_of<TX, TY, TZ>(a), (unsigned)nbatch, 1), block(kTB);
   switch (op) {
-    case OP_PULL: hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, kCap, 0>), grid, block, 0, s, a); break;
-    case OP_PULL_BWD: hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, kCap, 1>), grid, block, 0, s, a); break;
-    case OP_OBJGRAD: hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, kCapAcc, 2>), grid, block, 0, s, a); break;
-    case OP_PUSH: hipLaunchKernelGGL((k_push_tile<DP, TX, TY, TZ, kCapAcc, false>), grid, block, 0, s, a); break;
-    case OP_PUSHG: hipLaunchKernelGGL((k_push_tile<DP, TX, TY, TZ, kCapAcc, true>), grid, block, 0, s, a); break;
+    case OP_PULL:
+      hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, cap_cells<DP, 0>(), 0>), grid, block, 0, s, a);
+      break;
+    case OP_PULL_BWD:
+      hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, cap_cells<DP, 1>(), 1>), grid, block, 0, s, a);
+      break;
+    case OP_OBJGRAD:
+      hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, cap_cells<DP, 2>(), 2>), grid, block, 0, s, a);
+      break;
+    case OP_PUSH:
+      hipLaunchKernelGGL((k_push_tile<DP, TX, TY, TZ, cap_cells<DP, 3>(), false>), grid, block, 0, s, a);
+      break;
+    case OP_PUSHG:
+      hipLaunchKernelGGL((k_push_tile<DP, TX, TY, TZ, cap_cells<DP, 4>(), true>), grid, block, 0, s, a);
+      break;
     default: return hipErrorNotSupported;
   }
   return hipGetLastError();
@@ -631,12 +677,9 @@ constexpr int kBTX = 16, kBTY = 8, kBTZ = 8;
 template <int DP, int KIND>
 __global__ __launch_bounds__(kTB) void k_batch(const BatchVol* __restrict__ vols, int nvol, int total) {
   constexpr int KG = KIND <= 2 ? KIND : 0;
-  constexpr int CAPE = (DP >= 2 && DP <= 4) ? cap_for<KIND>() : 0;
-  using G = GatherLds<kBTX, kBTY, kBTZ, CAPE, KG>;
-  constexpr int BOXF = KIND <= 2 ? G::BOXF : 2 * CAPE;  // push: CAPE int64 cells
+  constexpr int CAPE = (DP >= 2 && DP <= 4) ? cap_cells<DP, KIND>() : 0;
+  constexpr int BOXF = KIND <= 2 ? box_floats<CAPE, KG>() : 2 * CAPE;  // push: CAPE int64 cells
   __shared__ __align__(8) float box[BOXF > 0 ? BOXF : 2];
-  __shared__ float acc[G::NACC];
-  __shared__ float rbuf[G::NR];
   __shared__ TileState S;
   __shared__ float wsum[kTB / 64];
   const int nb = gridDim.x / kXcds;  // the host launches a multiple of 8 workgroups
@@ -652,7 +695,7 @@ __global__ __launch_bounds__(kTB) void k_batch(const BatchVol* __restrict__ vols
     }
     const KArgs a = vols[lo].a;
     const int tile = t - vols[lo].tile0;
-    if (KIND <= 2) gather_tile<DP, kBTX, kBTY, kBTZ, CAPE, KG>(a, tile, 0, box, acc, rbuf, wsum, S);
+    if (KIND <= 2) gather_tile<DP, kBTX, kBTY, kBTZ, CAPE, KG>(a, tile, 0, box, wsum, S);
     else push_tile<DP, kBTX, kBTY, kBTZ, CAPE, KIND == 4>(a, tile, 0, reinterpret_cast<acc_t*>(box), wsum, S);
   }
 }
