@@ -245,20 +245,29 @@ __device__ __forceinline__ void stage_box(float* box, const TileState& S, const 
 // gfx950 executes float atomics (LDS ds_add_f32 and global alike) at a small fraction
 // of the rate of integer LDS atomics, so scatters accumulate into an int64 LDS box
 // with ds_add_u64: every contribution c is stored as round(c * 2^scale). Per channel
-// and tile, scale = 52 - e where 2^e bounds the largest contribution: no cell can
-// overflow (at most 2^10 samples per tile touch a cell), integer sums are exact and
-// order-independent, and each cell is rounded to float once, when it is flushed.
+// and tile, scale = 50 - e where 2^e bounds the largest contribution: every scaled
+// contribution is below 2^50 and no cell can overflow (at most 2^10 samples per tile
+// touch a cell), integer sums are exact and order-independent, and each cell is
+// rounded to float once, when it is flushed.
 typedef unsigned long long acc_t;
 
 __device__ __forceinline__ int fixed_scale(float vmax) {
   if (!(vmax > 0.0f)) return 0;
   int e;
   (void)frexpf(vmax, &e);  // vmax < 2^e
-  return 52 - e;
+  return 50 - e;
 }
 
-__device__ __forceinline__ void acc_add(acc_t* cell, float c, int scale) {
-  atomicAdd(cell, (acc_t)__float2ll_rn(ldexpf(c, scale)));
+// round(a * b) as int64 for |a * b| < 2^51: the FMA with 2^52 + 2^51 rounds the product
+// to an integer held in the low mantissa bits (one FMA instead of a float -> int64
+// conversion sequence); fix2 adds two such products.
+constexpr double kMagic = 6755399441055744.0;  // 2^52 + 2^51
+
+__device__ __forceinline__ acc_t fix1(double a, double b) {
+  return (acc_t)(__double_as_longlong(fma(a, b, kMagic)) - __double_as_longlong(kMagic));
+}
+__device__ __forceinline__ acc_t fix2(double a, double b, double c, double d) {
+  return (acc_t)(__double_as_longlong(fma(a, b, fma(c, d, kMagic))) - __double_as_longlong(kMagic));
 }
 
 __device__ __forceinline__ void zero_acc(acc_t* box, int cells) {
@@ -363,20 +372,23 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, acc_
       if ((use >> q) & 1u) {
         LSample<DP> L;
         lsample<DP, GRAD>(x, y, z, S, L);
+        const double s0 = ldexp((double)g0, scale);  // exact
+        const double s1 = GRAD ? ldexp((double)g1, scale) : 0.0;
+        const double s2 = GRAD ? ldexp((double)g2, scale) : 0.0;
 #pragma unroll
         for (int k = 0; k < DP; ++k)
 #pragma unroll
           for (int j = 0; j < DP; ++j) {
             acc_t* row = box + L.base + BX * j + BXY * k;
             if (GRAD) {
-              const float cx = (L.wz[k] * L.wy[j]) * g0;
-              const float cyz = (L.wz[k] * L.dy[j]) * g1 + (L.dz[k] * L.wy[j]) * g2;
+              const double cx = (double)(L.wz[k] * L.wy[j]) * s0;
+              const double cyz = (double)(L.wz[k] * L.dy[j]) * s1 + (double)(L.dz[k] * L.wy[j]) * s2;
 #pragma unroll
-              for (int i = 0; i < DP; ++i) acc_add(row + i, L.dx[i] * cx + L.wx[i] * cyz, scale);
+              for (int i = 0; i < DP; ++i) atomicAdd(row + i, fix2(L.dx[i], cx, L.wx[i], cyz));
             } else {
-              const float wv = (L.wz[k] * L.wy[j]) * g0;
+              const double wv = (double)(L.wz[k] * L.wy[j]) * s0;
 #pragma unroll
-              for (int i = 0; i < DP; ++i) acc_add(row + i, L.wx[i] * wv, scale);
+              for (int i = 0; i < DP; ++i) atomicAdd(row + i, fix1(L.wx[i], wv));
             }
           }
       } else {
@@ -566,15 +578,15 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
           if (!((rfin >> q) & 1u)) continue;
           LSample<DP> L;
           lsample<DP, false>(V.x[0], V.y[0], V.z[0], S, L);
-          const float w = res[0];
+          const double ws = ldexp((double)res[0], scale);  // exact
 #pragma unroll
           for (int k = 0; k < DP; ++k)
 #pragma unroll
             for (int j = 0; j < DP; ++j) {
               acc_t* row = box64 + L.base + BX * j + BXY * k;
-              const float wv = (L.wz[k] * L.wy[j]) * w;
+              const double wv = (double)(L.wz[k] * L.wy[j]) * ws;
 #pragma unroll
-              for (int i = 0; i < DP; ++i) acc_add(row + i, L.wx[i] * wv, scale);
+              for (int i = 0; i < DP; ++i) atomicAdd(row + i, fix1(L.wx[i], wv));
             }
         }
         __syncthreads();
