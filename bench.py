@@ -31,7 +31,14 @@ N = 256
 DEGS = (1, 2, 3)
 HBM_PEAK_GBS = 8000.0
 BYTES = {"pull": 20, "push": 20, "backward": 32}  # per voxel at C = 1
-KERNEL_NAMES = {"pull": "k_gather_tile<pull>", "push": "k_push_tile", "backward": "k_gather_tile<pullback>"}
+TILED_NAMES = {"pull": "k_gather_tile<pull>", "push": "k_push_tile", "backward": "k_gather_tile<pullback>"}
+SAMPLE_NAMES = {"pull": "k_pull", "push": "k_push", "backward": "k_pull_bwd"}
+
+
+def kernel_name(op, deg, tiling):
+    """Kernel that runs `op` at `deg` under the library's PP_OPT_TILING choice."""
+    per_sample = tiling == 0 or (tiling == 1 and deg == 1 and op in ("pull", "backward"))
+    return (SAMPLE_NAMES if per_sample else TILED_NAMES)[op]
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_256cubed.txt")
 
 
@@ -308,7 +315,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--tiling", type=int, default=1, help="1: LDS-tiled kernels, 0: one work-item per sample")
+    ap.add_argument("--tiling", type=int, default=1,
+                    help="1: automatic kernel choice, 2: LDS-tiled kernels, 0: one work-item per sample")
     ap.add_argument("--shape", type=int, default=0, help="tile shape index (0: 16x8x8, 1: 32x8x4, 2: 16x16x4)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched-uneven and matching-loop legs")
@@ -447,10 +455,10 @@ def main():
         "data": "synthetic (smoothed Gaussian noise image and upstream gradient; identity + smooth displacement, max 6 voxels)",
         "config": {"workload": "pull+push+backward, 256^3 x 1 channel, degrees 1,2,3 (one triple each per step), "
                                "reflect boundary, ext=1", "volume": [N, N, N], "channels": 1, "degrees": list(DEGS),
-                   "kernels": "tiled" if args.tiling else "per-sample", "tile_shape": args.shape},
-        "roofline": {"bound": "hbm", "kernel": f"{KERNEL_NAMES[op]} deg{deg}", "achieved": round(gbs, 1),
+                   "kernels": {0: "per-sample", 1: "automatic", 2: "tiled"}[args.tiling], "tile_shape": args.shape},
+        "roofline": {"bound": "hbm", "kernel": f"{kernel_name(op, deg, args.tiling)} deg{deg}", "achieved": round(gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
-                     "traffic": _traffic(KERNEL_NAMES[op], deg), "bytes_per_launch": BYTES[op] * n1,
+                     "traffic": _traffic(kernel_name(op, deg, args.tiling), deg), "bytes_per_launch": BYTES[op] * n1,
                      "avg_ms": round(ms, 4), "op_ms_with_fill": round(ms_op, 4)},
         "per_degree": detail,
         "wall_s": round(wall, 3),

@@ -49,7 +49,7 @@ constexpr int kBlock = 256;
 // ---- kernel selection --------------------------------------------------------------
 // Variants: 0 = one work-item per sample; 1 + k = tiled with tile shape k.
 constexpr int kVariants = 4;
-std::atomic<int> g_tiling{1};  // PP_OPT_TILING
+std::atomic<int> g_tiling{1};  // PP_OPT_TILING: 0 per sample, 1 automatic, 2 tiled
 std::atomic<int> g_shape{0};   // PP_OPT_TILE_SHAPE (default when nothing is tuned)
 std::atomic<int> g_tuned{1};   // PP_OPT_USE_TUNED
 thread_local int t_force = -1; // variant forced while pp_tune times candidates
@@ -74,14 +74,21 @@ hipError_t launch_variant(pp::Op op, const pp::KArgs& a, int nbatch, hipStream_t
 
 hipError_t launch(pp::Op op, const pp::KArgs& a, int nbatch, hipStream_t s) {
   if (t_force >= 0) return launch_variant(op, a, nbatch, s, t_force);
-  if (!g_tiling.load(std::memory_order_relaxed)) return launch_variant(op, a, nbatch, s, 0);
+  const int tiling = g_tiling.load(std::memory_order_relaxed);
+  const int tiled = 1 + g_shape.load(std::memory_order_relaxed);
+  if (tiling == 0) return launch_variant(op, a, nbatch, s, 0);
+  if (tiling == 2) return launch_variant(op, a, nbatch, s, tiled);
   if (g_tuned.load(std::memory_order_relaxed)) {
     const bool iso = a.dpx == a.dpy && a.dpx == a.dpz;
     std::lock_guard<std::mutex> lock(g_tune_mu);
     auto it = g_tune.find({(int)op, iso ? a.dpx : 0, size_class(a.n1)});
     if (it != g_tune.end()) return launch_variant(op, a, nbatch, s, it->second);
   }
-  return launch_variant(op, a, nbatch, s, 1 + g_shape.load(std::memory_order_relaxed));
+  // Built-in choice: trilinear gathers (8 taps, served by L1/L2) run per sample, which
+  // beats staging a box at that degree on gfx950; everything else runs tiled.
+  const bool trilinear = a.dpx == 2 && a.dpy == 2 && a.dpz == 2;
+  if (trilinear && (op == pp::OP_PULL || op == pp::OP_PULL_BWD)) return launch_variant(op, a, nbatch, s, 0);
+  return launch_variant(op, a, nbatch, s, tiled);
 }
 
 struct Geo {
@@ -685,7 +692,8 @@ int pp_device_sync(void) {
 int pp_set_option(int key, int value) {
   switch (key) {
     case PP_OPT_TILING:
-      g_tiling.store(value != 0);
+      if (value < 0 || value > 2) return fail(PP_EINVAL, "PP_OPT_TILING takes 0, 1 or 2, not %d", value);
+      g_tiling.store(value);
       return ok();
     case PP_OPT_TILE_SHAPE:
       if (value < 0 || value > 2) return fail(PP_EINVAL, "tile shape %d not in 0..2", value);

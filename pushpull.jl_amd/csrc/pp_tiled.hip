@@ -22,6 +22,9 @@ namespace pp {
 
 constexpr int kTB = 256;
 
+// Largest box extent along y or z: 2 * (window half-width) + dp for every tile shape.
+constexpr int kMaxYZ = 64;
+
 struct TileState {
   int lo[3];      // unfolded image index of box cell (0,0,0), per axis
   int b[3];       // box extent per axis
@@ -30,6 +33,9 @@ struct TileState {
   int ctr[3];     // window centre (knot of the tile's centre sample)
   int org[3];     // tile origin in the phi grid
   int red[6][4];  // per-wave partial min/max
+  // folded image offset of box plane y = i (d0x * fold(lo[1] + i)) and z = i
+  // (d0x * d0y * fold(lo[2] + i)); bit 31 set where the boundary flips the sign
+  int yoff[kMaxYZ], zoff[kMaxYZ];
 };
 
 template <int TX, int TY, int TZ>
@@ -41,6 +47,7 @@ struct Shape {
   static constexpr int WX = TX + 8, WY = TY + 8, WZ = TZ + 8;
   // a box row (<= 2 * WX + dp cells) fits in one workgroup pass (box_walk relies on it)
   static_assert(2 * WX + 5 < kTB, "box rows must be shorter than the workgroup");
+  static_assert(2 * WY + 5 <= kMaxYZ && 2 * WZ + 5 <= kMaxYZ, "box planes fit the fold tables");
   // local index of the centre sample
   static constexpr int LC = TX / 2 + TX * (TY / 2 + TY * (TZ / 2));
 };
@@ -161,82 +168,115 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
     S.red[3][wid] = mx0; S.red[4][wid] = mx1; S.red[5][wid] = mx2;
   }
   __syncthreads();
+  // every thread sizes the box (a broadcast read of the partials)
+  int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+#pragma unroll
+  for (int w = 0; w < kTB / 64; ++w)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      lo[q] = min(lo[q], S.red[q][w]);
+      hi[q] = max(hi[q], S.red[3 + q][w]);
+    }
+  const bool any = lo[0] <= hi[0];  // some in-window sample
+  const int by = hi[1] - lo[1] + DP, bz = hi[2] - lo[2] + DP;
+  const int cells = any ? (hi[0] - lo[0] + DP) * by * bz : 0;
+  const bool use = any && cells <= CAP;
   if (tid == 0) {
-    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
-    for (int w = 0; w < kTB / 64; ++w)
-      for (int q = 0; q < 3; ++q) {
-        lo[q] = min(lo[q], S.red[q][w]);
-        hi[q] = max(hi[q], S.red[3 + q][w]);
-      }
-    if (lo[0] > hi[0]) {  // no in-window sample
-      S.cells = 0;
-      S.lds = 0;
-    } else {
-      int cells = 1;
-      for (int q = 0; q < 3; ++q) {
-        S.lo[q] = lo[q];
-        S.b[q] = hi[q] - lo[q] + DP;
-        cells *= S.b[q];
-      }
-      S.cells = cells;
-      S.lds = cells <= CAP;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      S.lo[q] = lo[q];
+      S.b[q] = hi[q] - lo[q] + DP;
+    }
+    S.cells = cells;
+    S.lds = use;
+  }
+  if (use) {  // fold tables of the box planes
+    bool neg;
+    if (tid < by) {
+      const int f = fold32(a.by, lo[1] + tid, a.d0y, neg);
+      S.yoff[tid] = (a.d0x * f) | (neg ? INT_MIN : 0);
+    } else if (tid >= 128 && tid - 128 < bz) {
+      const int f = fold32(a.bz, lo[2] + tid - 128, a.d0z, neg);
+      S.zoff[tid - 128] = (a.d0x * a.d0y * f) | (neg ? INT_MIN : 0);
     }
   }
   __syncthreads();
 }
 
 // The box is walked by the whole workgroup: thread t owns column t % BX of rows
-// t / BX, t / BX + RP, ... (RP = kTB / BX rows per pass; BX <= 2 * WX + DP < kTB), and
-// issues kBatch rows' accesses before it uses any of them, so a wave has that many
-// global loads in flight instead of waiting out each one in turn.
+// t / BX, t / BX + RP, ... (RP = kTB / BX rows per pass; BX <= 2 * WX + DP < kTB; rows
+// run over y, then z). Row coordinates advance incrementally and the folds of y and z
+// come from the tile's tables, so a cell costs a few instructions. kBatch rows' loads
+// are issued before any is used, so a wave has that many in flight.
 constexpr int kBatch = 16;
 
 struct BoxWalk {
-  int c, r0, rp;  // column, first row, rows per pass; r0 >= rp: this thread idles
+  int c, rp;       // column, rows per pass
+  int ry, rz;      // plane indices of the current row
+  int dy, dz;      // plane step per pass: RP = dz * BY + dy
+  int r, rows;     // current row, rows in the box
+  int fx;          // folded x offset of the column
+  bool nx;         // sign flip of the column
+  bool active;     // r0 < RP: this thread owns a column
 };
 
-__device__ __forceinline__ BoxWalk box_walk(const TileState& S) {
-  const int BX = S.b[0];
+__device__ __forceinline__ BoxWalk box_walk(const TileState& S, const KArgs& a) {
+  const int BX = S.b[0], BY = S.b[1];
   BoxWalk w;
   w.c = threadIdx.x % BX;
-  w.r0 = threadIdx.x / BX;
+  const int r0 = threadIdx.x / BX;
   w.rp = kTB / BX;
+  w.active = r0 < w.rp;
+  w.r = r0;
+  w.rows = BY * S.b[2];
+  w.ry = r0 % BY;
+  w.rz = r0 / BY;
+  w.dy = w.rp % BY;
+  w.dz = w.rp / BY;
+  w.fx = fold32(a.bx, S.lo[0] + w.c, a.d0x, w.nx);
   return w;
 }
 
-// Folded image offset of box row r (rows run over y, then z) and its boundary sign.
-__device__ __forceinline__ int row_offset(const TileState& S, const KArgs& a, int r, bool& neg) {
-  bool ny, nz;
-  const int fy = fold32(a.by, S.lo[1] + r % S.b[1], a.d0y, ny);
-  const int fz = fold32(a.bz, S.lo[2] + r / S.b[1], a.d0z, nz);
-  neg = ny ^ nz;
-  return a.d0x * (fy + a.d0y * fz);
+__device__ __forceinline__ void walk_next(BoxWalk& w, int BY) {
+  w.r += w.rp;
+  w.ry += w.dy;
+  w.rz += w.dz;
+  if (w.ry >= BY) {
+    w.ry -= BY;
+    ++w.rz;
+  }
+}
+
+// Folded image offset of the walk's current cell and whether its sign flips.
+__device__ __forceinline__ int walk_offset(const TileState& S, const BoxWalk& w, bool& neg) {
+  const int yo = S.yoff[w.ry], zo = S.zoff[w.rz];
+  neg = ((yo ^ zo) < 0) != w.nx;
+  return w.fx + (yo & INT_MAX) + (zo & INT_MAX);
 }
 
 // Stage the box from image channel `src` (fold + boundary sign applied here).
 __device__ __forceinline__ void stage_box(float* box, const TileState& S, const float* __restrict__ src,
                                           const KArgs& a) {
-  const BoxWalk w = box_walk(S);
-  if (w.r0 >= w.rp) return;
-  const int BX = S.b[0], rows = S.b[1] * S.b[2];
-  bool nx;
-  const int fx = fold32(a.bx, S.lo[0] + w.c, a.d0x, nx);
-  for (int rb = w.r0; rb < rows; rb += kBatch * w.rp) {
+  BoxWalk w = box_walk(S, a);
+  if (!w.active) return;
+  const int BX = S.b[0], BY = S.b[1];
+  while (w.r < w.rows) {
+    const int r0 = w.r;
     float v[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
-      const int r = rb + u * w.rp;
       v[u] = 0.f;
-      if (r < rows) {
+      if (w.r < w.rows) {
         bool neg;
-        const float t = src[fx + row_offset(S, a, r, neg)];
-        v[u] = (neg ^ nx) ? -t : t;
+        const float t = src[walk_offset(S, w, neg)];
+        v[u] = neg ? -t : t;
       }
+      walk_next(w, BY);
     }
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
-      const int r = rb + u * w.rp;
-      if (r < rows) box[r * BX + w.c] = v[u];
+      const int r = r0 + u * w.rp;
+      if (r < w.rows) box[r * BX + w.c] = v[u];
     }
   }
 }
@@ -289,18 +329,16 @@ __device__ __forceinline__ float block_max(float v, float* wsum) {
 // non-zero cell.
 __device__ __forceinline__ void flush_acc(const acc_t* box, const TileState& S, int scale, float* __restrict__ dst,
                                           const KArgs& a) {
-  const BoxWalk w = box_walk(S);
-  if (w.r0 >= w.rp) return;
-  const int BX = S.b[0], rows = S.b[1] * S.b[2];
-  bool nx;
-  const int fx = fold32(a.bx, S.lo[0] + w.c, a.d0x, nx);
-  for (int r = w.r0; r < rows; r += w.rp) {
-    const long long q = (long long)box[r * BX + w.c];
+  BoxWalk w = box_walk(S, a);
+  if (!w.active) return;
+  const int BX = S.b[0], BY = S.b[1];
+  for (; w.r < w.rows; walk_next(w, BY)) {
+    const long long q = (long long)box[w.r * BX + w.c];
     if (q != 0) {
       bool neg;
-      const int off = row_offset(S, a, r, neg);
+      const int off = walk_offset(S, w, neg);
       const float v = ldexpf((float)q, -scale);
-      unsafeAtomicAdd(dst + fx + off, (neg ^ nx) ? -v : v);
+      unsafeAtomicAdd(dst + off, neg ? -v : v);
     }
   }
 }

@@ -33,7 +33,7 @@ def path(request):
     push_grad, the pull pullback and the fused objective."""
     old = pp._lib.get_option(pp._lib.PP_OPT_TILING)
     old_t = pp._lib.get_option(pp._lib.PP_OPT_USE_TUNED)
-    pp._lib.set_option(pp._lib.PP_OPT_TILING, 1 if request.param == "tiled" else 0)
+    pp._lib.set_option(pp._lib.PP_OPT_TILING, 2 if request.param == "tiled" else 0)
     pp._lib.set_option(pp._lib.PP_OPT_USE_TUNED, 0)
     yield request.param
     pp._lib.set_option(pp._lib.PP_OPT_TILING, old)
