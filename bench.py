@@ -227,6 +227,39 @@ def bench_batched(torch, pp, reps=5, seed=4):
             "max_rel_diff_batched_vs_loop": worst}
 
 
+def bench_exp(torch, pp, reps=5, seed=6, nsq=8):
+    """Exp (scaling and squaring, src/compositions.jl:41-49) of a 256^3 velocity (smoothed
+    noise, sigma 4, max 6 voxels): nsq compositions, each a 3-channel trilinear pull with
+    sliding boundaries whose deformation Id + u is formed in the kernel. Algorithmic bytes
+    per squaring: read u (12 B/voxel) and write the new u (12 B/voxel)."""
+    import ctypes as ct
+
+    dev = torch.device("cuda:0")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    d = (N, N, N)
+    v = _smooth_noise(torch, (3,) + d, 4.0, gen, dev)
+    v *= 6.0 / v.abs().max()
+    v = v.contiguous()
+    out, work = torch.empty_like(v), torch.empty_like(v)
+    L = pp._lib.lib()
+    dims = (ct.c_longlong * 3)(N, N, N)
+    P = lambda t: ct.c_void_p(t.data_ptr())  # noqa: E731
+    st = ct.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def run():
+        pp._lib.check(L.pp_exp(P(out), P(v), P(work), dims, 1, nsq, st), "pp_exp")
+
+    run()
+    ms = _time_ms(torch, run, reps, None)
+    n = N ** 3
+    per = ms / nsq
+    gbs = 24 * n / (per * 1e-3) / 1e9
+    return {"volume": [N, N, N], "squarings": nsq, "ms": round(ms, 4), "ms_per_squaring": round(per, 4),
+            "Gvoxel_s": round(n * nsq / (ms * 1e-3) / 1e9, 3), "GB_s_per_squaring": round(gbs, 1),
+            "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
 def bench_matching(torch, pp, iters=50, alpha=0.02, seed=5, check=True):
     """8 subjects at 256^3, 4 channels, degree 3: `iters` iterations of objective and
     gradient (pull, residual, push, pullback) with phi <- phi - alpha * dL/dphi, all
@@ -470,6 +503,9 @@ def main():
         torch.cuda.empty_cache()
         _progress("matching loop")
         res["matching_loop"] = bench_matching(torch, pp)
+        torch.cuda.empty_cache()
+        _progress("Exp (scaling and squaring)")
+        res["exp_256"] = bench_exp(torch, pp)
         torch.cuda.empty_cache()
     if not args.no_cpu:
         _progress("cpu baseline")

@@ -135,6 +135,22 @@ int pp_pull_grad_backward(const pp_desc *d, float *dphi, const float *phi, const
 int pp_objective_grad(const pp_desc *d, float *dphi, float *df0, double *loss, const float *phi,
                       const float *f0, const float *g, int accumulate, void *stream);
 
+/* ---- compositions (src/compositions.jl) -------------------------------------
+ * pp_compose replaces compose!/compose (src/compositions.jl:9-39): psi = phi o theta for
+ * a displacement phi (3 channels on grid d0) sampled at the 1-based deformation theta
+ * (grid d1): channel c is pulled with degree 1, extrapolation, and boundary 1 on every axis
+ * except 2 on axis c (the "sliding" matrix of compositions.jl:33), all three channels in
+ * one launch. psi has theta's size; nbatch volumes lie back to back. psi must not overlap
+ * phi or theta.
+ * pp_exp replaces Exp (src/compositions.jl:41-49): scaling and squaring of a velocity v
+ * (3 channels, grid dim): u = v / 2^nsq, then nsq times u = compose(u, Id + u), then
+ * out = u + Id (1-based). The reference squares 8 times. Id + u is formed inside the kernel
+ * with the reference's float rounding. work is a caller buffer of v's size; out, v and work
+ * must not overlap. Both are stream-ordered and graph-capturable. */
+int pp_compose(float *psi, const float *phi, const float *theta, const long long d0[3], const long long d1[3],
+               int nbatch, void *stream);
+int pp_exp(float *out, const float *v, float *work, const long long dim[3], int nbatch, int nsq, void *stream);
+
 /* ---- batches of uneven volumes --------------------------------------------
  * No counterpart in the reference, whose wrappers loop over same-size (channel, batch)
  * slices with one native call each (src/pushpull_cpu.jl:42-51, ext/pushpull_gpu.jl:37-48).

@@ -285,3 +285,45 @@ def affine_push(f1, A0, d0, sett, prec="f32"):
             fn(_ptr(out[b, c]), _ptr(f1[b, c]), _ptr(d0u), n1, _ptr(d1u), _ptr(a12), _ptr(bnd), _ptr(dp),
                int(sett.ext))
     return out
+
+
+# ---- compositions (src/compositions.jl) ----------------------------------------------
+SLIDING = [(2, 1, 1), (1, 2, 1), (1, 1, 2)]  # compositions.jl:33; column c = channel c's triple
+
+
+class _Sett:
+    """Settings as the Julia constructor leaves them (deg tuple, list of bnd triples, ext)."""
+
+    def __init__(self, deg, bnd, ext):
+        self.deg, self.bnd, self.ext = tuple(deg), list(bnd), int(ext)
+
+
+def identity(dzyx):
+    """id (src/pushpull_general.jl:81-91): 1-based grid, (3, Z, Y, X) float32."""
+    Z, Y, X = dzyx
+    g = np.empty((3, Z, Y, X), dtype=np.float32)
+    g[0] = np.arange(1, X + 1, dtype=np.float32)[None, None, :]
+    g[1] = np.arange(1, Y + 1, dtype=np.float32)[None, :, None]
+    g[2] = np.arange(1, Z + 1, dtype=np.float32)[:, None, None]
+    return g
+
+
+def compose(phi, theta, prec="f32"):
+    """compose! (src/compositions.jl:27-39): for dim = 1:3, psi[.., dim] = pull(phi[.., dim],
+    theta, Settings((1,1,1), bnd[:, dim], true)) -- one pull per channel with its own
+    boundary triple. phi (B,3,Z0,Y0,X0) displacement, theta (B,3,Z1,Y1,X1) 1-based."""
+    return pull(_as5(phi, 5), _as5(theta, 5), _Sett((1, 1, 1), SLIDING, 1), prec)
+
+
+def exp(v, nsq=8, prec="f32"):
+    """Exp (src/compositions.jl:41-49) as written: u = v / 2^nsq (exact); nsq times
+    u = compose(u, Id .+ u) with Id .+ u formed in Float32 (the deformation is Float32 in
+    the reference); finally u .+ Id. v (B,3,Z,Y,X)."""
+    dt = _dtype(prec)
+    v = np.asarray(_as5(v, 5), dtype=np.float32)
+    Id = identity(v.shape[2:])[None]
+    u = (v / np.float32(2.0 ** nsq)).astype(dt)
+    for _ in range(nsq):
+        theta = (Id + u.astype(np.float32)).astype(np.float32) if prec != "f64" else (Id + u).astype(np.float32)
+        u = compose(u, theta, prec)
+    return (u + Id).astype(dt)

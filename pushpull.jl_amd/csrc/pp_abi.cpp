@@ -135,6 +135,7 @@ void base_args(const pp_desc* d, pp::KArgs& a, const Geo& g) {
   a.ly = (float)d->d0[1] - 0.99f;
   a.lz = (float)d->d0[2] - 0.99f;
   a.phi_bs = 3 * g.n1;
+  a.vscale = 1.0f;
 }
 
 bool uniform_bnd(const pp_desc* d) {
@@ -144,9 +145,31 @@ bool uniform_bnd(const pp_desc* d) {
   return true;
 }
 
-// Launch `op` over all channels: one launch when the boundary codes are shared, else
-// one per channel (pointers offset by the channel strides; channel-reducing outputs
-// accumulate after the first launch).
+// Channels whose boundary codes differ only between reflect (1) and antisymmetric (2)
+// fold to the same indices and differ only in sign. Returns true when d's per-channel
+// codes are like that on every axis (or equal), with the shared fold codes in `base` and
+// the per-channel sign bits (3c + axis) in `mask`.
+bool sign_split(const pp_desc* d, int base[3], unsigned& mask) {
+  if (d->nchan > 10 || d->nbnd != d->nchan) return false;
+  mask = 0u;
+  for (int q = 0; q < 3; ++q) {
+    bool any2 = false, same = true, refl = true;
+    for (int c = 0; c < d->nchan; ++c) {
+      const int code = d->bnd[3 * c + q];
+      any2 |= code == 2;
+      same &= code == d->bnd[q];
+      refl &= code == 1 || code == 2;
+      if (code == 2) mask |= 1u << (3 * c + q);
+    }
+    if (!same && !refl) return false;
+    base[q] = any2 ? 2 : d->bnd[q];
+  }
+  return true;
+}
+
+// Launch `op` over all channels: one launch when the boundary codes are shared (or, for
+// the tiled pull and push, differ only in sign), else one per channel (pointers offset by
+// the channel strides; channel-reducing outputs accumulate after the first launch).
 int run(pp::Op op, const pp_desc* d, const Geo& g, pp::KArgs a, hipStream_t s) {
   const bool reduce = (op == pp::OP_PULL_BWD || op == pp::OP_PULLG_BWD || op == pp::OP_OBJGRAD);
   if (uniform_bnd(d)) {
@@ -157,12 +180,27 @@ int run(pp::Op op, const pp_desc* d, const Geo& g, pp::KArgs a, hipStream_t s) {
     hipError_t e = launch(op, a, d->nbatch, s);
     return e == hipSuccess ? ok() : hip_fail(e, "kernel launch");
   }
+  int base[3];
+  unsigned mask;
+  const bool iso_tiled = a.dpx == a.dpy && a.dpx == a.dpz && a.dpx >= 2 && a.dpx <= 4;
+  if ((op == pp::OP_PULL || op == pp::OP_PUSH) && iso_tiled && g_tiling.load(std::memory_order_relaxed) != 0 &&
+      sign_split(d, base, mask)) {
+    a.bx = base[0];
+    a.by = base[1];
+    a.bz = base[2];
+    a.csign_on = 1;
+    a.csign = mask;
+    a.nchan = d->nchan;
+    hipError_t e = launch_variant(op, a, d->nbatch, s, 1 + g_shape.load(std::memory_order_relaxed));
+    return e == hipSuccess ? ok() : hip_fail(e, "kernel launch");
+  }
   for (int c = 0; c < d->nchan; ++c) {
     pp::KArgs b = a;
     b.bx = d->bnd[3 * c];
     b.by = d->bnd[3 * c + 1];
     b.bz = d->bnd[3 * c + 2];
     b.nchan = 1;
+    b.chan0 = c;
     b.f = a.f + c * a.f_cs;
     if (a.aux) b.aux = a.aux + c * a.aux_cs;
     if (reduce) {
@@ -276,6 +314,7 @@ int tierA_args(const pp_usize* d0, pp_usize n1, const int* bnd, const pp_usize* 
   a.lz = (float)d0[2] - 0.99f;
   a.nchan = 1;
   a.phi_bs = 3 * (long long)n1;
+  a.vscale = 1.0f;
   return PP_OK;
 }
 
@@ -495,6 +534,75 @@ int pp_objective_grad(const pp_desc* d, float* dphi, float* df0, double* loss, c
   return run(pp::OP_OBJGRAD, d, g, a, S(s));
 }
 
+// ---- compositions (src/compositions.jl) ---------------------------------------------
+namespace {
+
+// Descriptor of compose: 3 channels, degree 1, extrapolation, boundary (1,1,1) with 2 on
+// the channel's own axis (compositions.jl:32-35).
+void compose_desc(pp_desc& d, const long long d0[3], const long long d1[3], int nbatch) {
+  std::memset(&d, 0, sizeof d);
+  for (int q = 0; q < 3; ++q) {
+    d.d0[q] = d0[q];
+    d.d1[q] = d1[q];
+    d.deg[q] = 1;
+  }
+  d.nchan = 3;
+  d.nbatch = nbatch;
+  d.ext = 1;
+  d.nbnd = 3;
+  for (int c = 0; c < 3; ++c)
+    for (int q = 0; q < 3; ++q) d.bnd[3 * c + q] = (q == c) ? 2 : 1;
+}
+
+bool overlap(const float* p, const float* q, long long n) { return p < q + n && q < p + n; }
+
+}  // namespace
+
+int pp_compose(float* psi, const float* phi, const float* theta, const long long d0[3], const long long d1[3],
+               int nbatch, void* s) {
+  if (!psi || !phi || !theta || !d0 || !d1) return fail(PP_EINVAL, "null argument");
+  pp_desc d;
+  compose_desc(d, d0, d1, nbatch);
+  Geo g;
+  if (int r = check_desc(&d, g)) return r;
+  if (overlap(psi, phi, 3 * g.n0 * nbatch) || overlap(psi, theta, 3 * g.n1 * nbatch))
+    return fail(PP_EINVAL, "psi must not overlap phi or theta");
+  return pull_family(pp::OP_PULL, 1, &d, psi, theta, phi, s);
+}
+
+int pp_exp(float* out, const float* v, float* work, const long long dim[3], int nbatch, int nsq, void* s) {
+  if (!out || !v || !work || !dim) return fail(PP_EINVAL, "null argument");
+  if (nsq < 1 || nsq > 30) return fail(PP_EINVAL, "nsq must be in 1..30, not %d", nsq);
+  pp_desc d;
+  compose_desc(d, dim, dim, nbatch);
+  Geo g;
+  if (int r = check_desc(&d, g)) return r;
+  const long long n = 3 * g.n1 * nbatch;
+  if (overlap(out, v, n) || overlap(work, v, n) || overlap(out, work, n))
+    return fail(PP_EINVAL, "out, v and work must not overlap");
+  pp::KArgs a;
+  base_args(&d, a, g);
+  a.disp = 1;
+  a.f_cs = g.n1;
+  a.f_bs = 3 * g.n1;
+  a.out_cs = g.n1;
+  a.out_bs = 3 * g.n1;
+  const float* src = v;
+  for (int k = 1; k <= nsq; ++k) {
+    // the last squaring lands in `out`: alternate so that (nsq - k) even writes out
+    float* dst = ((nsq - k) % 2 == 0) ? out : work;
+    a.f = src;
+    a.phi = src;
+    a.out = dst;
+    // first squaring: u = v / 2^nsq, applied to the displacement and the gathered values
+    a.dscale = a.vscale = (k == 1) ? std::ldexp(1.0f, -nsq) : 1.0f;
+    a.add_id = k == nsq;  // u .+= Id
+    if (int r = run(pp::OP_PULL, &d, g, a, S(s))) return r;
+    src = dst;
+  }
+  return ok();
+}
+
 // ---- batches of uneven volumes -----------------------------------------------------
 struct pp_batch_group {
   int dp;      // degree + 1 of every volume in the group
@@ -554,6 +662,7 @@ int pp_batch_plan_create(int op, const pp_volume* vols, int nvol, pp_batch_plan*
     a.ly = (float)v.d0[1] - 0.99f;
     a.lz = (float)v.d0[2] - 0.99f;
     a.nchan = v.nchan;
+    a.vscale = 1.0f;
     a.phi = v.phi;
     a.phi_bs = 3 * n1;
     a.f = v.f;

@@ -26,6 +26,12 @@ __global__ __launch_bounds__(256) void k_pull(KArgs a) {
     return;
   }
   const float* f = a.f + bb * a.f_bs;
+  if (a.vscale != 1.0f || a.add_id) {
+    const int jk = v / a.d1x, i = v - jk * a.d1x;
+    for (int c = 0; c < a.nchan; ++c)
+      out[c * a.out_cs] = pull_out(a, sample_val<DP>(f + c * a.f_cs, ax, ay, az), c, i, jk % a.d1y, jk / a.d1y);
+    return;
+  }
   for (int c = 0; c < a.nchan; ++c) out[c * a.out_cs] = sample_val<DP>(f + c * a.f_cs, ax, ay, az);
 }
 

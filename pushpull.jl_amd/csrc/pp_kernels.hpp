@@ -43,7 +43,30 @@ struct KArgs {
   float lx, ly, lz;          // d0 - 0.99f per axis (VOXOK upper bounds)
   int affine;                // 1: sample coordinates come from aff, not from phi
   float aff[12];             // 0-based 3x4 affine, column-major (artifacts/C/pushpull.c:226-242)
+  // Per-channel boundary signs (tiled pull and push): all channels fold with bx/by/bz;
+  // channel c takes the antisymmetric sign along axis a only where bit 3c+a of csign is
+  // set (codes 1 and 2 fold to the same index and differ only in sign).
+  int csign_on;
+  unsigned csign;
+  // Displacement mode (scaling and squaring): phi holds a displacement u and the sample's
+  // 1-based coordinate is (index + 1) + dscale * u, rounded as the sum of two floats.
+  int disp;
+  float dscale;
+  // Pull outputs: value * vscale, plus the 1-based index along axis c for channel c < 3
+  // when add_id is set.
+  float vscale;
+  int add_id;
+  int chan0;  // channel index of the launch's first channel (per-channel launches)
 };
+
+// Boundary code of channel c along `axis` for a launch folding with `code`.
+__host__ __device__ inline int chan_code(const KArgs& a, int c, int axis, int code) {
+  return (a.csign_on && code == 2 && !((a.csign >> (3 * c + axis)) & 1u)) ? 1 : code;
+}
+// Axes (bit a) along which channel c takes the boundary sign.
+__host__ __device__ inline unsigned chan_signs(const KArgs& a, int c) {
+  return a.csign_on ? (a.csign >> (3 * c)) & 7u : 7u;
+}
 
 // Launch op on `stream`; dp is (DP,DP,DP) when isotropic, else the runtime path is used.
 // grid.y = nbatch. Returns hipSuccess or the launch error.

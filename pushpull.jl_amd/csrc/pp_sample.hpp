@@ -27,16 +27,43 @@ __device__ __forceinline__ void affine_x(const float* A, int i, int j, int k, fl
 }
 #pragma clang fp contract(fast)
 
-// Coordinates (0-based) of sample v: phi - 1 (pushpull.c:65-67) or the affine map.
-__device__ __forceinline__ void sample_coords(const KArgs& a, const float* phi, int v, float& x, float& y, float& z) {
+// Coordinates (0-based) of sample v at grid index (i, j, k): phi - 1 (pushpull.c:65-67),
+// the affine map, or, in displacement mode, (id + dscale * u) - 1 with id the 1-based
+// identity (src/compositions.jl:45: Id .+ u, then pull's phi - 1).
+__device__ __forceinline__ void sample_coords_at(const KArgs& a, const float* phi, int v, int i, int j, int k,
+                                                 float& x, float& y, float& z) {
   if (a.affine) {
-    const int i = v % a.d1x, jk = v / a.d1x;
-    affine_x(a.aff, i, jk % a.d1y, jk / a.d1y, x, y, z);
+    affine_x(a.aff, i, j, k, x, y, z);
+    return;
+  }
+  if (a.disp) {
+    x = ((float)(i + 1) + a.dscale * phi[v]) - 1.0f;
+    y = ((float)(j + 1) + a.dscale * phi[v + a.n1]) - 1.0f;
+    z = ((float)(k + 1) + a.dscale * phi[v + 2 * (long long)a.n1]) - 1.0f;
     return;
   }
   x = phi[v] - 1.0f;
   y = phi[v + a.n1] - 1.0f;
   z = phi[v + 2 * (long long)a.n1] - 1.0f;
+}
+
+__device__ __forceinline__ void sample_coords(const KArgs& a, const float* phi, int v, float& x, float& y, float& z) {
+  int i = 0, j = 0, k = 0;
+  if (a.affine || a.disp) {
+    const int jk = v / a.d1x;
+    i = v - jk * a.d1x;
+    j = jk % a.d1y;
+    k = jk / a.d1y;
+  }
+  sample_coords_at(a, phi, v, i, j, k, x, y, z);
+}
+
+// Pull output transform (vscale, identity added in displacement mode).
+__device__ __forceinline__ float pull_out(const KArgs& a, float val, int c, int i, int j, int k) {
+  val *= a.vscale;
+  c += a.chan0;
+  if (a.add_id && c < 3) val += (float)((c == 0 ? i : c == 1 ? j : k) + 1);
+  return val;
 }
 
 template <int DP, int NEED>
@@ -58,8 +85,9 @@ template <int DP, bool LEAN = false>
 __device__ __forceinline__ float gather_val(const float* __restrict__ f, const Axis<DP>& ax, const Axis<DP>& ay,
                                             const Axis<DP>& az) {
   constexpr int NP = Axis<DP>::NP;
+  constexpr int KU = LEAN ? 1 : NP;  // z-planes unrolled
   float acc = 0.f;
-#pragma unroll(LEAN ? 1 : NP)
+#pragma unroll KU
   for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
 #pragma unroll
       for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
@@ -78,8 +106,9 @@ template <int DP, bool LEAN = false>
 __device__ __forceinline__ void gather_grad(const float* __restrict__ f, const Axis<DP>& ax, const Axis<DP>& ay,
                                             const Axis<DP>& az, float& gx, float& gy, float& gz) {
   constexpr int NP = Axis<DP>::NP;
+  constexpr int KU = LEAN ? 1 : NP;  // z-planes unrolled
   gx = gy = gz = 0.f;
-#pragma unroll(LEAN ? 1 : NP)
+#pragma unroll KU
   for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
 #pragma unroll
       for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
@@ -103,8 +132,9 @@ template <int DP, bool LEAN = false>
 __device__ __forceinline__ void gather_val_grad(const float* __restrict__ f, const Axis<DP>& ax, const Axis<DP>& ay,
                                                 const Axis<DP>& az, float& val, float& gx, float& gy, float& gz) {
   constexpr int NP = Axis<DP>::NP;
+  constexpr int KU = LEAN ? 1 : NP;  // z-planes unrolled
   val = gx = gy = gz = 0.f;
-#pragma unroll(LEAN ? 1 : NP)
+#pragma unroll KU
   for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
 #pragma unroll
       for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {

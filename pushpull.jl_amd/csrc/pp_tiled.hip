@@ -134,7 +134,7 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
     bool ok = false;
     if (in) {
       V.inside |= 1u << q;
-      sample_coords(a, phi, v, x, y, z);
+      sample_coords_at(a, phi, v, gx, gy, gz, x, y, z);
       ok = voxok(x, y, z, a.ext, a.lx, a.ly, a.lz);
     }
     V.x[q] = x;
@@ -247,16 +247,18 @@ __device__ __forceinline__ void walk_next(BoxWalk& w, int BY) {
   }
 }
 
-// Folded image offset of the walk's current cell and whether its sign flips.
-__device__ __forceinline__ int walk_offset(const TileState& S, const BoxWalk& w, bool& neg) {
+// Folded image offset of the walk's current cell and whether its sign flips; `signs`
+// (bit a) selects the axes whose boundary sign applies to the channel.
+__device__ __forceinline__ int walk_offset(const TileState& S, const BoxWalk& w, unsigned signs, bool& neg) {
   const int yo = S.yoff[w.ry], zo = S.zoff[w.rz];
-  neg = ((yo ^ zo) < 0) != w.nx;
+  neg = ((yo < 0) && (signs & 2u)) != ((zo < 0) && (signs & 4u));
+  neg = neg != (w.nx && (signs & 1u));
   return w.fx + (yo & INT_MAX) + (zo & INT_MAX);
 }
 
 // Stage the box from image channel `src` (fold + boundary sign applied here).
 __device__ __forceinline__ void stage_box(float* box, const TileState& S, const float* __restrict__ src,
-                                          const KArgs& a) {
+                                          const KArgs& a, unsigned signs) {
   BoxWalk w = box_walk(S, a);
   if (!w.active) return;
   const int BX = S.b[0], BY = S.b[1];
@@ -268,7 +270,7 @@ __device__ __forceinline__ void stage_box(float* box, const TileState& S, const 
       v[u] = 0.f;
       if (w.r < w.rows) {
         bool neg;
-        const float t = src[walk_offset(S, w, neg)];
+        const float t = src[walk_offset(S, w, signs, neg)];
         v[u] = neg ? -t : t;
       }
       walk_next(w, BY);
@@ -328,7 +330,7 @@ __device__ __forceinline__ float block_max(float v, float* wsum) {
 // Flush the int64 box into image channel `dst`: one folded global float atomic per
 // non-zero cell.
 __device__ __forceinline__ void flush_acc(const acc_t* box, const TileState& S, int scale, float* __restrict__ dst,
-                                          const KArgs& a) {
+                                          const KArgs& a, unsigned signs) {
   BoxWalk w = box_walk(S, a);
   if (!w.active) return;
   const int BX = S.b[0], BY = S.b[1];
@@ -336,7 +338,7 @@ __device__ __forceinline__ void flush_acc(const acc_t* box, const TileState& S, 
     const long long q = (long long)box[w.r * BX + w.c];
     if (q != 0) {
       bool neg;
-      const int off = walk_offset(S, w, neg);
+      const int off = walk_offset(S, w, signs, neg);
       const float v = ldexpf((float)q, -scale);
       unsafeAtomicAdd(dst + off, neg ? -v : v);
     }
@@ -432,16 +434,16 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, acc_
       } else {
         Axis<DP> ax, ay, az;
         constexpr int NEED = GRAD ? (kW | kDW) : kW;
-        make_axis<DP, NEED>(x, DP, a.d0x, 1, a.bx, ax);
-        make_axis<DP, NEED>(y, DP, a.d0y, a.d0x, a.by, ay);
-        make_axis<DP, NEED>(z, DP, a.d0z, a.d0x * a.d0y, a.bz, az);
+        make_axis<DP, NEED>(x, DP, a.d0x, 1, chan_code(a, c, 0, a.bx), ax);
+        make_axis<DP, NEED>(y, DP, a.d0y, a.d0x, chan_code(a, c, 1, a.by), ay);
+        make_axis<DP, NEED>(z, DP, a.d0z, a.d0x * a.d0y, chan_code(a, c, 2, a.bz), az);
         if (GRAD) scatter_grad<DP>(dst, ax, ay, az, g0, g1, g2);
         else scatter_val<DP>(dst, ax, ay, az, g0);
       }
     }
     if (lds) {
       __syncthreads();
-      flush_acc(box, S, scale, dst, a);
+      flush_acc(box, S, scale, dst, a, chan_signs(a, c));
       __syncthreads();
     }
   }
@@ -552,7 +554,7 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
       res[q] = 0.f;
     }
     if (lds) {
-      stage_box(box, S, src, a);
+      stage_box(box, S, src, a, chan_signs(a, c));
       __syncthreads();
     }
 #pragma unroll 1
@@ -573,10 +575,14 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
           val = lds_val<DP>(box, L, BX, BXY);
         } else {
           Axis<DP> ax, ay, az;
-          make_axis<DP, kW>(x, DP, a.d0x, 1, a.bx, ax);
-          make_axis<DP, kW>(y, DP, a.d0y, a.d0x, a.by, ay);
-          make_axis<DP, kW>(z, DP, a.d0z, a.d0x * a.d0y, a.bz, az);
+          make_axis<DP, kW>(x, DP, a.d0x, 1, chan_code(a, c, 0, a.bx), ax);
+          make_axis<DP, kW>(y, DP, a.d0y, a.d0x, chan_code(a, c, 1, a.by), ay);
+          make_axis<DP, kW>(z, DP, a.d0z, a.d0x * a.d0y, chan_code(a, c, 2, a.bz), az);
           val = sample_val<DP, true>(src, ax, ay, az);
+        }
+        if (a.vscale != 1.0f || a.add_id) {
+          const int l = threadIdx.x + kTB * q;
+          val = pull_out(a, val, c, S.org[0] + l % TX, S.org[1] + (l / TX) % TY, S.org[2] + l / (TX * TY));
         }
         a.out[bb * a.out_bs + c * a.out_cs + v] = val;
         continue;
@@ -628,7 +634,7 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
             }
         }
         __syncthreads();
-        flush_acc(box64, S, scale, a.out2 + bb * a.out2_bs + c * a.out2_cs, a);
+        flush_acc(box64, S, scale, a.out2 + bb * a.out2_bs + c * a.out2_cs, a, chan_signs(a, c));
         __syncthreads();
       }
     }

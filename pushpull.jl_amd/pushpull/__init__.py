@@ -9,11 +9,13 @@ from .settings import Settings, id  # noqa: A001
 from .autograd import pull, pull_grad, push
 from .ops import push_grad, pull_hess, pull_backward, pull_grad_backward, objective_grad
 from .affine import affine_pull, affine_push, adjust_affine
+from .compositions import compose, compose_, Exp
 from . import _lib
 
 __all__ = [
     "Settings", "id", "pull", "push", "pull_grad", "push_grad", "pull_hess",
     "pull_backward", "pull_grad_backward", "objective_grad", "affine_pull", "affine_push", "adjust_affine",
+    "compose", "compose_", "Exp",
 ]
 
 

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(LIB_DIR, "libpushpull_hip.so")
 HEADER = os.path.join(os.path.dirname(os.path.dirname(_PKG)), "include", "pushpull_hip.h")
 
 PP_MAX_CHAN_BND = 64
+PP_OK, PP_EINVAL, PP_EDEVICE, PP_ENOMEM, PP_ERANGE = 0, -1, -2, -3, -4
 PP_OPT_TILING = 1
 PP_OPT_TILE_SHAPE = 2
 PP_OPT_USE_TUNED = 3
@@ -57,6 +58,8 @@ _SIGS = {
     "pp_pull_backward": ([_P, _P, _P, _P, _P, _P], ct.c_int),
     "pp_pull_grad_backward": ([_P, _P, _P, _P, _P, _P], ct.c_int),
     "pp_objective_grad": ([_P, _P, _P, _P, _P, _P, _P, ct.c_int, _P], ct.c_int),
+    "pp_compose": ([_P, _P, _P, ct.POINTER(ct.c_longlong), ct.POINTER(ct.c_longlong), ct.c_int, _P], ct.c_int),
+    "pp_exp": ([_P, _P, _P, ct.POINTER(ct.c_longlong), ct.c_int, ct.c_int, _P], ct.c_int),
     "pp_malloc": ([ct.POINTER(ct.c_void_p), ct.c_size_t], ct.c_int),
     "pp_free": ([_P], ct.c_int),
     "pp_memcpy_h2d": ([_P, _P, ct.c_size_t, _P], ct.c_int),

@@ -62,3 +62,25 @@ def smooth_field(rng, d, sigma, maxabs):
     u = gauss_smooth(rng.standard_normal((3,) + tuple(d)), sigma)
     u *= maxabs / np.abs(u).max()
     return u.astype(np.float32)
+
+
+def max_abs(a):
+    """Largest finite |a| (0 when none)."""
+    a = np.asarray(a)
+    f = a[np.isfinite(a)]
+    return float(np.abs(f).max()) if f.size else 0.0
+
+
+def check_close(got, ref, what, tol=1e-5, scale=None):
+    """NaN masks equal; ||got - ref|| <= tol*||ref|| over finite entries, plus (when `scale`,
+    the largest input magnitude, is given) an absolute floor 1e-6*sqrt(N)*scale at FP32
+    rounding level for outputs that are mathematically (near) zero."""
+    got = np.asarray(got)
+    ref = np.asarray(ref)
+    assert got.shape == ref.shape, (what, got.shape, ref.shape)
+    assert same_nan_mask(got, ref), f"{what}: NaN mask differs"
+    m = np.isfinite(ref)
+    num = float(np.linalg.norm(got[m].astype(np.float64) - ref[m]))
+    den = float(np.linalg.norm(ref[m].astype(np.float64)))
+    floor = 1e-6 * np.sqrt(max(1, int(m.sum()))) * scale if scale is not None else 0.0
+    assert num <= tol * den + floor, f"{what}: rel err {num / den if den else num:.3g}"
