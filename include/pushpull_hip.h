@@ -135,6 +135,25 @@ int pp_pull_grad_backward(const pp_desc *d, float *dphi, const float *phi, const
 int pp_objective_grad(const pp_desc *d, float *dphi, float *df0, double *loss, const float *phi,
                       const float *f0, const float *g, int accumulate, void *stream);
 
+/* ---- push with a count image ------------------------------------------------
+ * pp_push_count replaces pushc (pushpull.cu:128-142: push1 with its count at
+ * pushpull_dev.cu:140-154) and the push pairs of its callers: invert_def pushes the
+ * identity and ones through one deformation (src/multigrid.jl:270-273); restrict pushes an
+ * image and ones through one affine (src/multigrid.jl:26-30, via pp_affine_push_count).
+ * f0 (d0, nchan, nbatch) receives push(f1) channel by channel with each channel's
+ * boundary triple; c0 (d0, 1, nbatch) receives the push of ones with the count triple:
+ * triple number nchan of d->bnd when d->nbnd == nchan + 1, else the first triple (as in
+ * pushc, whose count shares the image's boundary, signs included). f1 == NULL pushes the
+ * identity instead: channel c holds the 1-based index along axis c of the phi grid (nchan
+ * must be 3). Invalid samples and non-finite values are skipped as in push; accumulate = 0
+ * zeroes f0 and c0 first. One launch when every triple folds alike (equal, or differing
+ * only between 1 and 2) and the degree is isotropic 1-3; otherwise one push per channel
+ * and one for the count. */
+int pp_push_count(const pp_desc *d, float *f0, float *c0, const float *phi, const float *f1, int accumulate,
+                  void *stream);
+int pp_affine_push_count(const pp_desc *d, float *f0, float *c0, const float *f1, const float *aff, int accumulate,
+                         void *stream);
+
 /* ---- compositions (src/compositions.jl) -------------------------------------
  * pp_compose replaces compose!/compose (src/compositions.jl:9-39): psi = phi o theta for
  * a displacement phi (3 channels on grid d0) sampled at the 1-based deformation theta

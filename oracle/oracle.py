@@ -327,3 +327,58 @@ def exp(v, nsq=8, prec="f32"):
         theta = (Id + u.astype(np.float32)).astype(np.float32) if prec != "f64" else (Id + u).astype(np.float32)
         u = compose(u, theta, prec)
     return (u + Id).astype(dt)
+
+
+# ---- push with a count image, multigrid resampling (src/multigrid.jl) ----------------
+def push_count(f1, phi, dim, sett, count_bnd=None, prec="f32"):
+    """pushc (artifacts/C/pushpull.cu:128-142) as its callers use it: (push(f1), push(ones))
+    through the same deformation; the count uses `count_bnd` (default: sett's first triple).
+    f1 None pushes the identity of phi's grid (src/multigrid.jl:269-271)."""
+    phi = np.asarray(_as5(phi, 5), dtype=np.float32)
+    B = phi.shape[0]
+    if f1 is None:
+        f1 = np.broadcast_to(identity(phi.shape[2:])[None], (B, 3) + phi.shape[2:]).copy()
+    cb = tuple(count_bnd) if count_bnd is not None else tuple(sett.bnd[0])
+    f0 = push(f1, phi, dim, sett, prec)
+    ones = np.ones((B, 1) + phi.shape[2:], np.float32)
+    c0 = push(ones, phi, dim, _Sett(sett.deg, [cb], sett.ext), prec)
+    return f0, c0
+
+
+def invert_def_rhs(phi, prec="f32"):
+    """invert_def's pushes (src/multigrid.jl:266-275): g = push(Id, phi, Settings(1, sliding,
+    0)), h = push(ones, phi, Settings(1, 1, 0)), g .-= h .* Id (elementwise in Float32)."""
+    phi = np.asarray(_as5(phi, 5), dtype=np.float32)
+    d = phi.shape[2:]
+    g, h = push_count(None, phi, d, _Sett((1, 1, 1), SLIDING, 0), (1, 1, 1), prec)
+    Id = identity(d)[None]
+    g = g - h * Id
+    return g, h
+
+
+def zoom_affine(d_from, d_to):
+    """Mat of restrict / prolong (src/multigrid.jl:23-25, 45-47): zm = d_to ./ d_from and
+    os = (1 - zm) / 2 per Julia axis, in Float64, converted to Float32 (sizes (Z, Y, X))."""
+    M = np.zeros((4, 4))
+    for q in range(3):
+        zm = d_to[2 - q] / d_from[2 - q]
+        M[q, q] = zm
+        M[q, 3] = (1 - zm) / 2
+    M[3, 3] = 1
+    return M.astype(np.float32)
+
+
+def restrict(f1, d0, deg=2, prec="f32"):
+    """restrict (src/multigrid.jl:21-35): affine_push(f1) ./ affine_push(ones)."""
+    f1 = np.asarray(_as5(f1, 5), dtype=np.float32)
+    M = adjust_affine(zoom_affine(f1.shape[2:], d0))
+    s = _Sett((deg,) * 3, [(1, 1, 1)], 1)
+    f0 = affine_push(f1, M, d0, s, prec)
+    c0 = affine_push(np.ones((f1.shape[0], 1) + f1.shape[2:], np.float32), M, d0, s, prec)
+    return f0 / c0
+
+
+def prolong(f0, d1, deg=2, prec="f32"):
+    """prolong (src/multigrid.jl:43-52): affine_pull with the sliding triples."""
+    f0 = np.asarray(_as5(f0, 5), dtype=np.float32)
+    return affine_pull(f0, adjust_affine(zoom_affine(d1, f0.shape[2:])), d1, _Sett((deg,) * 3, SLIDING, 1), prec)

@@ -534,6 +534,102 @@ int pp_objective_grad(const pp_desc* d, float* dphi, float* df0, double* loss, c
   return run(pp::OP_OBJGRAD, d, g, a, S(s));
 }
 
+// ---- push with a count image ----------------------------------------------------------
+namespace {
+
+int push_count_impl(const pp_desc* d, float* f0, float* c0, const float* phi, const float* f1, const float* aff,
+                    int accumulate, void* s) {
+  if (!d) return fail(PP_EINVAL, "null descriptor");
+  pp_desc dc = *d;  // the channels' triples only
+  int cnt[3];
+  if (d->nchan >= 1 && d->nbnd == d->nchan + 1 && d->nbnd <= PP_MAX_CHAN_BND) {
+    for (int q = 0; q < 3; ++q) cnt[q] = d->bnd[3 * d->nchan + q];
+    dc.nbnd = d->nchan;
+  } else {
+    for (int q = 0; q < 3; ++q) cnt[q] = d->bnd[q];
+  }
+  for (int q = 0; q < 3; ++q)
+    if (cnt[q] < 0 || cnt[q] > 2) return fail(PP_EINVAL, "count boundary code %d not in {0,1,2}", cnt[q]);
+  Geo g;
+  if (int r = check_desc(&dc, g)) return r;
+  if (!f0 || !c0 || (!aff && !phi)) return fail(PP_EINVAL, "null array");
+  if (!f1 && d->nchan != 3) return fail(PP_EINVAL, "pushing the identity needs nchan == 3, not %d", d->nchan);
+  const int nc = d->nchan, nb = d->nbatch;
+  if (!accumulate) {
+    hipError_t e = pp::launch_fill(f0, g.n0 * nc * nb, 0.0f, S(s));
+    if (e == hipSuccess) e = pp::launch_fill(c0, g.n0 * nb, 0.0f, S(s));
+    if (e != hipSuccess) return hip_fail(e, "zero fill");
+  }
+  pp::KArgs a;
+  base_args(&dc, a, g);
+  a.f = f1;
+  a.f_cs = g.n1;
+  a.f_bs = g.n1 * nc;
+  a.vmode = f1 ? 0 : 2;
+  if (aff) {
+    a.affine = 1;
+    for (int q = 0; q < 12; ++q) a.aff[q] = aff[q];
+    a.phi = c0;  // not read: coordinates come from the matrix
+    a.phi_bs = 0;
+  } else {
+    a.phi = phi;
+  }
+  a.out = f0;
+  a.out_cs = g.n0;
+  a.out_bs = g.n0 * nc;
+  // one launch: channels and count fold alike and the tiled push covers the degree
+  pp_desc dd = dc;
+  dd.nchan = nc + 1;
+  dd.nbnd = nc + 1;
+  const bool fits = nc + 1 <= PP_MAX_CHAN_BND;
+  if (fits) {
+    for (int c = 0; c < nc; ++c)
+      for (int q = 0; q < 3; ++q) dd.bnd[3 * c + q] = dc.bnd[3 * (dc.nbnd == 1 ? 0 : c) + q];
+    for (int q = 0; q < 3; ++q) dd.bnd[3 * nc + q] = cnt[q];
+  }
+  int base[3];
+  unsigned mask;
+  const bool iso_tiled = a.dpx == a.dpy && a.dpx == a.dpz && a.dpx >= 2 && a.dpx <= 4;
+  if (fits && iso_tiled && g_tiling.load(std::memory_order_relaxed) != 0 && sign_split(&dd, base, mask)) {
+    a.bx = base[0];
+    a.by = base[1];
+    a.bz = base[2];
+    a.csign_on = 1;
+    a.csign = mask;
+    a.nchan = nc;
+    a.out2 = c0;
+    a.out2_bs = g.n0;
+    hipError_t e = launch_variant(pp::OP_PUSH, a, nb, S(s), 1 + g_shape.load(std::memory_order_relaxed));
+    return e == hipSuccess ? ok() : hip_fail(e, "kernel launch");
+  }
+  if (int r = run(pp::OP_PUSH, &dc, g, a, S(s))) return r;
+  pp::KArgs b = a;  // the count: ones through the count's triple
+  b.vmode = 1;
+  b.nchan = 1;
+  b.chan0 = 0;
+  b.bx = cnt[0];
+  b.by = cnt[1];
+  b.bz = cnt[2];
+  b.out = c0;
+  b.out_cs = g.n0;
+  b.out_bs = g.n0;
+  hipError_t e = launch(pp::OP_PUSH, b, nb, S(s));
+  return e == hipSuccess ? ok() : hip_fail(e, "kernel launch");
+}
+
+}  // namespace
+
+int pp_push_count(const pp_desc* d, float* f0, float* c0, const float* phi, const float* f1, int accumulate,
+                  void* s) {
+  return push_count_impl(d, f0, c0, phi, f1, nullptr, accumulate, s);
+}
+
+int pp_affine_push_count(const pp_desc* d, float* f0, float* c0, const float* f1, const float* aff, int accumulate,
+                         void* s) {
+  if (!aff) return fail(PP_EINVAL, "null affine");
+  return push_count_impl(d, f0, c0, nullptr, f1, aff, accumulate, s);
+}
+
 // ---- compositions (src/compositions.jl) ---------------------------------------------
 namespace {
 

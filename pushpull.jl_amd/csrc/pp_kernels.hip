@@ -90,8 +90,16 @@ __global__ __launch_bounds__(256) void k_push(KArgs a) {
   const float* f1 = a.f + bb * a.f_bs + v;
   if (!setup<DP, kW>(a, phi, v, ax, ay, az)) return;
   float* dst = a.out + bb * a.out_bs;
+  int idx[3] = {0, 0, 0};
+  if (a.vmode == 2) {
+    const int jk = v / a.d1x;
+    idx[0] = v - jk * a.d1x;
+    idx[1] = jk % a.d1y;
+    idx[2] = jk / a.d1y;
+  }
   for (int c = 0; c < a.nchan; ++c) {
-    const float val = f1[c * a.f_cs];
+    const int ch = c + a.chan0;
+    const float val = a.vmode == 0 ? f1[c * a.f_cs] : a.vmode == 1 ? 1.0f : (float)(idx[ch < 3 ? ch : 0] + 1);
     if (isfinite(val)) scatter_val<DP>(dst + c * a.out_cs, ax, ay, az, val);
   }
 }

@@ -57,6 +57,10 @@ struct KArgs {
   float vscale;
   int add_id;
   int chan0;  // channel index of the launch's first channel (per-channel launches)
+  // Push values: 0 from f; 1 ones (a count image); 2 the 1-based index of the sample along
+  // axis (channel + chan0) (pushing the identity, src/multigrid.jl:270). With out2 set, the
+  // tiled push also scatters ones into out2 as one more channel (index nchan): the count.
+  int vmode;
 };
 
 // Boundary code of channel c along `axis` for a launch folding with `code`.

@@ -375,15 +375,27 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, acc_
   const bool lds = S.lds != 0;
   const int BX = S.b[0], BXY = S.b[0] * S.b[1];
   const long long n1 = a.n1;
-  for (int c = 0; c < a.nchan; ++c) {
+  // push with a count image (out2): one more channel of ones after the nchan channels
+  const int nsrc = a.nchan + ((!GRAD && a.out2) ? 1 : 0);
+  for (int c = 0; c < nsrc; ++c) {
+    const bool count = c == a.nchan;
+    const int vmode = GRAD ? 0 : count ? 1 : a.vmode;  // push_grad reads its gradients
     const float* src = a.f + bb * a.f_bs + c * a.f_cs;
-    float* dst = a.out + bb * a.out_bs + c * a.out_cs;
+    float* dst = count ? a.out2 + bb * a.out2_bs : a.out + bb * a.out_bs + c * a.out_cs;
     // this channel's values of all the thread's valid samples, loaded together
     float g[NG][VPT];
 #pragma unroll
-    for (int q = 0; q < VPT; ++q)
+    for (int q = 0; q < VPT; ++q) {
+      const int l = threadIdx.x + kTB * q;
+      const int ax = c + a.chan0;
+      const int id = ax == 0 ? S.org[0] + l % TX : ax == 1 ? S.org[1] + (l / TX) % TY : S.org[2] + l / (TX * TY);
 #pragma unroll
-      for (int e = 0; e < NG; ++e) g[e][q] = ((V.valid >> q) & 1u) ? src[V.v[q] + e * n1] : 0.f;
+      for (int e = 0; e < NG; ++e)
+        g[e][q] = !((V.valid >> q) & 1u) ? 0.f
+                  : vmode == 0           ? src[V.v[q] + e * n1]
+                  : vmode == 1           ? 1.0f
+                                         : (float)(id + 1);
+    }
     unsigned use = lds ? V.inl : 0u;  // samples accumulated in LDS for this channel
     int scale = 0;
     if (lds) {

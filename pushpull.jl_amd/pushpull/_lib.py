@@ -58,6 +58,8 @@ _SIGS = {
     "pp_pull_backward": ([_P, _P, _P, _P, _P, _P], ct.c_int),
     "pp_pull_grad_backward": ([_P, _P, _P, _P, _P, _P], ct.c_int),
     "pp_objective_grad": ([_P, _P, _P, _P, _P, _P, _P, ct.c_int, _P], ct.c_int),
+    "pp_push_count": ([_P, _P, _P, _P, _P, ct.c_int, _P], ct.c_int),
+    "pp_affine_push_count": ([_P, _P, _P, _P, _P, ct.c_int, _P], ct.c_int),
     "pp_compose": ([_P, _P, _P, ct.POINTER(ct.c_longlong), ct.POINTER(ct.c_longlong), ct.c_int, _P], ct.c_int),
     "pp_exp": ([_P, _P, _P, ct.POINTER(ct.c_longlong), ct.c_int, ct.c_int, _P], ct.c_int),
     "pp_malloc": ([ct.POINTER(ct.c_void_p), ct.c_size_t], ct.c_int),

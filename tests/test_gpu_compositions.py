@@ -114,7 +114,9 @@ def test_exp_graph_replays():
     run()
     torch.cuda.synchronize()
     eager = H(out).copy()
-    g = pp.batch.Graph(run)
+    from pushpull import batch as ppb
+
+    g = ppb.Graph(run)
     for _ in range(3):
         out.fill_(0)
         work.fill_(7)
