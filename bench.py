@@ -260,6 +260,50 @@ def bench_exp(torch, pp, reps=5, seed=6, nsq=8):
             "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
+def bench_invert_def(torch, pp, reps=5, seed=7):
+    """invert_def's pushes (src/multigrid.jl:266-275) at 256^3: the identity (3 channels,
+    sliding boundaries) and the count of ones through one deformation, ext=0, degree 1 --
+    as one pp_push_count launch against the reference's two separate pushes (identity
+    materialised, then ones). Algorithmic bytes: phi 12 + g 12 + h 4 = 28 B/voxel (the
+    identity and the ones are not read)."""
+    import ctypes as ct
+
+    dev = torch.device("cuda:0")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    d = (N, N, N)
+    u = _smooth_noise(torch, (3,) + d, 4.0, gen, dev)
+    u *= 4.0 / u.abs().max()
+    phi = (torch.from_numpy(pp.id(dim=d)).to(dev) + u).contiguous()
+    Id = torch.from_numpy(pp.id(dim=d)).to(dev).contiguous()
+    ones = torch.ones(d, device=dev)
+    g, h = torch.empty_like(phi), torch.empty(d, device=dev)
+    L = pp._lib.lib()
+    P = lambda t: ct.c_void_p(t.data_ptr())  # noqa: E731
+    st = ct.c_void_p(torch.cuda.current_stream().cuda_stream)
+    sl = pp.Settings(1, pp.multigrid.SLIDING, 0)
+    dc = pp.multigrid._count_desc(sl, d, d, 3, 1, (1, 1, 1))
+    d3 = pp.ops._desc(sl, d, d, 3, 1)
+    d1 = pp.ops._desc(pp.Settings(1, 1, 0), d, d, 1, 1)
+
+    def fused():
+        pp._lib.check(L.pp_push_count(ct.byref(dc), P(g), P(h), P(phi), None, 0, st), "pp_push_count")
+
+    def separate():
+        pp._lib.check(L.pp_push(ct.byref(d3), P(g), P(phi), P(Id), 0, st), "pp_push")
+        pp._lib.check(L.pp_push(ct.byref(d1), P(h), P(phi), P(ones), 0, st), "pp_push")
+
+    fused()
+    separate()
+    ms_f = _time_ms(torch, fused, reps, None)
+    ms_s = _time_ms(torch, separate, reps, None)
+    n = N ** 3
+    gbs = 28 * n / (ms_f * 1e-3) / 1e9
+    return {"volume": [N, N, N], "fused_ms": round(ms_f, 4), "separate_pushes_ms": round(ms_s, 4),
+            "speedup": round(ms_s / ms_f, 2), "Gvoxel_s": round(n / (ms_f * 1e-3) / 1e9, 3),
+            "GB_s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
 def bench_matching(torch, pp, iters=50, alpha=0.02, seed=5, check=True):
     """8 subjects at 256^3, 4 channels, degree 3: `iters` iterations of objective and
     gradient (pull, residual, push, pullback) with phi <- phi - alpha * dL/dphi, all
@@ -506,6 +550,9 @@ def main():
         torch.cuda.empty_cache()
         _progress("Exp (scaling and squaring)")
         res["exp_256"] = bench_exp(torch, pp)
+        torch.cuda.empty_cache()
+        _progress("invert_def pushes (push with count)")
+        res["invert_def_push_count_256"] = bench_invert_def(torch, pp)
         torch.cuda.empty_cache()
     if not args.no_cpu:
         _progress("cpu baseline")
