@@ -155,8 +155,9 @@ def bench_batched(torch, pp, reps=5, seed=4):
     """24 volumes of uneven size (X 96-224, Y 96-256, Z 96-192), 1-4 channels, degree 1
     or 3, each with its own smooth deformation (max 5 voxels): the fused objective (pull,
     residual, push, pullback) for all volumes as one persistent launch replayed from a HIP
-    graph, against a per-volume loop of the same fused call and against the per-volume
-    loop of separate pull / residual / push / pullback calls."""
+    graph, against a per-volume loop of the same fused call, against the per-volume loop of
+    separate pull / residual / push / pullback calls, and against the reference's structure:
+    one call per (volume, channel) slice with the phi gradient from pull_grad."""
     from pushpull import _lib, ops
     from pushpull import batch as ppb
 
@@ -200,6 +201,23 @@ def bench_batched(torch, pp, reps=5, seed=4):
             _lib.check(L.pp_push(ct.byref(dd), P(o["df"]), P(v["phi"]), P(o["r"]), 0, sp), "push")
             _lib.check(L.pp_pull_backward(ct.byref(dd), P(o["dphi"]), P(v["phi"]), P(v["f"]), P(o["r"]), sp), "bwd")
 
+    # the reference's structure (src/pushpull_cpu.jl:42-51, src/pushpull_pb.jl:11-17): one
+    # call per (volume, channel) slice, the phi gradient from a materialised pull_grad
+    # contracted with the residual
+    d1s = [ops._desc(pp.Settings(v["deg"], 1, 1), v["d"], v["d"], 1, 1) for v in vols]
+    gbuf = [torch.empty((3,) + v["d"], device=dev) for v in vols]
+
+    def loop_slices():
+        for v, dd, o, gb in zip(vols, d1s, lo, gbuf):
+            o["dphi"].zero_()
+            for c in range(v["C"]):
+                f_c, r_c, f1_c = v["f"][c], o["r"][c], o["f1"][c]
+                _lib.check(L.pp_pull(ct.byref(dd), P(f1_c), P(v["phi"]), P(f_c), sp), "pull")
+                torch.sub(f1_c, v["aux"][c], out=r_c)
+                _lib.check(L.pp_push(ct.byref(dd), P(o["df"][c]), P(v["phi"]), P(r_c), 0, sp), "push")
+                _lib.check(L.pp_pull_grad(ct.byref(dd), P(gb), P(v["phi"]), P(f_c), sp), "pull_grad")
+                o["dphi"].addcmul_(gb, r_c.unsqueeze(0))
+
     with torch.cuda.stream(st):
         graph.launch()
         loop_fused()
@@ -207,6 +225,7 @@ def bench_batched(torch, pp, reps=5, seed=4):
         t_graph = _time_ms(torch, graph.launch, reps, st)
         t_fused = _time_ms(torch, loop_fused, reps, st)
         t_unfused = _time_ms(torch, loop_unfused, reps, st)
+        t_slices = _time_ms(torch, loop_slices, reps, st)
         graph.launch()
         loop_fused()
         st.synchronize()
@@ -220,8 +239,10 @@ def bench_batched(torch, pp, reps=5, seed=4):
     gv = lambda ms: round(chan_vox / (ms * 1e-3) / 1e9, 3)  # noqa: E731
     return {"volumes": 24, "channel_voxels": chan_vox,
             "batched_graph_ms": round(t_graph, 3), "loop_fused_ms": round(t_fused, 3),
-            "loop_unfused_ms": round(t_unfused, 3),
-            "Gvoxel_s": {"batched_graph": gv(t_graph), "loop_fused": gv(t_fused), "loop_unfused": gv(t_unfused)},
+            "loop_unfused_ms": round(t_unfused, 3), "loop_reference_slices_ms": round(t_slices, 3),
+            "Gvoxel_s": {"batched_graph": gv(t_graph), "loop_fused": gv(t_fused), "loop_unfused": gv(t_unfused),
+                         "loop_reference_slices": gv(t_slices)},
+            "speedup_vs_loop_reference_slices": round(t_slices / t_graph, 2),
             "speedup_vs_loop_unfused": round(t_unfused / t_graph, 2),
             "speedup_vs_loop_fused": round(t_fused / t_graph, 2),
             "max_rel_diff_batched_vs_loop": worst}

@@ -705,7 +705,7 @@ struct pp_batch_group {
   int first;   // index of the group's first volume in the device array
   int nvol;
   int total;   // tiles over the group's volumes
-  int grid;    // persistent workgroups for this degree's kernel
+  pp::XcdSplit split;  // per-XCD tile ranges of equal work
 };
 
 struct pp_batch_plan {
@@ -787,14 +787,31 @@ int pp_batch_plan_create(int op, const pp_volume* vols, int nvol, pp_batch_plan*
   std::stable_sort(hv.begin(), hv.end(), [](const pp::BatchVol& x, const pp::BatchVol& y) { return x.dp < y.dp; });
   std::vector<pp_batch_group> groups;
   for (int i = 0; i < nvol; ++i) {
-    if (groups.empty() || groups.back().dp != hv[i].dp) {
-      const pp::Op o = static_cast<pp::Op>(op);
-      groups.push_back(pp_batch_group{hv[i].dp, i, 0, 0, pp::batch_grid(o, hv[i].dp)});
-    }
+    if (groups.empty() || groups.back().dp != hv[i].dp) groups.push_back(pp_batch_group{hv[i].dp, i, 0, 0, {}});
     pp_batch_group& g = groups.back();
     hv[i].tile0 = g.total;
     g.total += pp::batch_tiles(hv[i].a);
     g.nvol += 1;
+  }
+  // per-XCD ranges of equal work: a tile costs its volume's channel count plus one (the
+  // per-tile setup); the cuts fall on tile boundaries
+  for (pp_batch_group& g : groups) {
+    double work = 0.0;
+    for (int i = g.first; i < g.first + g.nvol; ++i) work += (double)pp::batch_tiles(hv[i].a) * (hv[i].a.nchan + 1);
+    int x = 1;
+    double acc = 0.0;
+    g.split.s[0] = 0;
+    for (int i = g.first; i < g.first + g.nvol && x < pp::kBatchXcds; ++i) {
+      const int nt = pp::batch_tiles(hv[i].a);
+      const double c = hv[i].a.nchan + 1;
+      while (x < pp::kBatchXcds && acc + nt * c >= work * x / pp::kBatchXcds) {
+        const long long k = (long long)std::ceil((work * x / pp::kBatchXcds - acc) / c);
+        g.split.s[x++] = hv[i].tile0 + (int)std::min<long long>(std::max<long long>(k, 0), nt);
+      }
+      acc += nt * c;
+    }
+    while (x <= pp::kBatchXcds) g.split.s[x++] = g.total;
+    for (int q = 1; q <= pp::kBatchXcds; ++q) g.split.s[q] = std::max(g.split.s[q], g.split.s[q - 1]);
   }
   pp::BatchVol* d = nullptr;
   hipError_t e = hipMalloc((void**)&d, sizeof(pp::BatchVol) * (size_t)nvol);
@@ -813,7 +830,7 @@ int pp_batch_run(const pp_batch_plan* plan, void* s) {
   hipError_t e = pp::launch_batch_zero(plan->op, plan->dvols, plan->nvol, S(s));
   for (const pp_batch_group& g : plan->groups) {
     if (e != hipSuccess) break;
-    e = pp::launch_batch_group(plan->op, g.dp, plan->dvols + g.first, g.nvol, g.total, g.grid, S(s));
+    e = pp::launch_batch_group(plan->op, g.dp, plan->dvols + g.first, g.nvol, g.split, S(s));
   }
   return e == hipSuccess ? ok() : hip_fail(e, "batch launch");
 }

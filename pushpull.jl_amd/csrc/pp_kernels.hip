@@ -6,6 +6,7 @@
 // with the x weights first, then weighted by w_y*w_z. Scatters use the hardware float
 // atomic (global_atomic_add_f32). Block ids are remapped so that each XCD's L2 sees a
 // contiguous range of voxels (neighbouring rows share most of their source footprint).
+#define PP_KERNEL_SOURCE 1  // kernel arguments carry the global address space (pp_kernels.hpp)
 #include "pp_sample.hpp"
 
 namespace pp {

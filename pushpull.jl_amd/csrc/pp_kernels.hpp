@@ -18,15 +18,25 @@ enum Op : int {
   OP_COUNT = 8
 };
 
+// Every array in the kernel arguments lives in device global memory. In the kernel
+// sources' device pass the pointers carry that address space, so that pointers read from
+// a table of arguments (the batched launches) still compile to global rather than flat
+// memory instructions; the layout is the same in every pass.
+#if defined(__HIP_DEVICE_COMPILE__) && defined(PP_KERNEL_SOURCE)
+#define PP_GLOBAL __attribute__((address_space(1)))
+#else
+#define PP_GLOBAL
+#endif
+
 // Kernel arguments, passed by value (kernarg segment; no module globals, so launches
 // on different streams never share state).
 struct KArgs {
-  const float* f;    // gathered image: f0 (pull family) / pushed values (push family)
-  const float* phi;  // deformation, 1-based coordinates, 3 comps of n1 voxels
-  const float* aux;  // upstream gradient (backward ops) / target image (objgrad)
-  float* out;        // output image / dphi
-  float* out2;       // objgrad: dL/df0 (scatter target)
-  double* red;       // objgrad: loss accumulator (one double)
+  const PP_GLOBAL float* f;    // gathered image: f0 (pull family) / pushed values (push family)
+  const PP_GLOBAL float* phi;  // deformation, 1-based coordinates, 3 comps of n1 voxels
+  const PP_GLOBAL float* aux;  // upstream gradient (backward ops) / target image (objgrad)
+  PP_GLOBAL float* out;        // output image / dphi
+  PP_GLOBAL float* out2;       // objgrad: dL/df0 (scatter target)
+  PP_GLOBAL double* red;       // objgrad: loss accumulator (one double)
   long long f_cs, f_bs;      // channel / batch strides of f (elements)
   long long phi_bs;          // batch stride of phi
   long long aux_cs, aux_bs;  // strides of aux
@@ -92,10 +102,16 @@ struct BatchVol {
   long long zero_n; // floats zeroed before a push / objective launch
 };
 int batch_tiles(const KArgs& a);   // tiles of one volume (16x8x8 tiles of the phi grid)
-int batch_grid(Op op, int dp);     // workgroups of one degree's kernel that fit at once (multiple of 8)
+// Contiguous tile ranges of a degree group, one per XCD: XCD x takes tiles
+// [s[x], s[x+1]), chosen so that the ranges carry equal work (a tile costs its volume's
+// channel count plus one).
+constexpr int kBatchXcds = 8;
+struct XcdSplit {
+  int s[kBatchXcds + 1];
+};
 hipError_t launch_batch_zero(Op op, const BatchVol* dvols, int nvol, hipStream_t stream);
 // One degree group: dvols[0..nvol) all have this dp; tile0 counts from the group start.
-hipError_t launch_batch_group(Op op, int dp, const BatchVol* dvols, int nvol, int total_tiles, int grid,
+hipError_t launch_batch_group(Op op, int dp, const BatchVol* dvols, int nvol, const XcdSplit& split,
                               hipStream_t stream);
 
 // Fill n floats with v (graph-capturable; no hipMemsetAsync nodes).

@@ -14,8 +14,10 @@
 // far away, sent outside the field of view, or on the large-coordinate path) are
 // handled individually through global memory, so a few outliers never inflate the box.
 // Tiles whose box would exceed CAP fall back to the global path entirely.
+#define PP_KERNEL_SOURCE 1  // kernel arguments carry the global address space (pp_kernels.hpp)
 #include "pp_sample.hpp"
 
+#include <algorithm>
 #include <climits>
 
 namespace pp {
@@ -731,54 +733,52 @@ static hipError_t launch_deg(Op op, const KArgs& a, int nbatch, hipStream_t s, T
   }
 }
 
-// ---- batched persistent launch over volumes of uneven size --------------------------
-// A plan orders its volumes by degree; each degree present gets one persistent launch of
-// a kernel compiled for that degree only (a single kernel switching over five degrees
-// would carry the register footprint of all of them). Within a launch the tiles of the
-// group's volumes are numbered consecutively (BatchVol::tile0, relative to the group);
-// as many workgroups as fit on the device walk them, each XCD a contiguous range of
-// tile numbers so neighbouring tiles share its L2. Degrees 1-3 use the LDS tile bodies;
-// degrees 0 and 4 run the same bodies with an empty box budget, i.e. per sample through
-// global memory.
+// ---- batched launch over volumes of uneven size ----------------------------------------
+// A plan orders its volumes by degree; each degree present gets one launch of a kernel
+// compiled for that degree only (a single kernel switching over five degrees would carry
+// the register footprint of all of them). Within a launch the tiles of the group's volumes
+// are numbered consecutively (BatchVol::tile0, relative to the group) and cut into eight
+// contiguous ranges of equal work, one per XCD (workgroup b runs on XCD b % 8), so that
+// neighbouring tiles share an L2 and no XCD idles while another still holds the heavy
+// (many-channel) volumes. Degrees 1-3 use the LDS tile bodies; degrees 0 and 4 run the same
+// bodies with an empty box budget, i.e. per sample through global memory.
 constexpr int kBTX = 16, kBTY = 8, kBTZ = 8;
 
 template <int DP, int KIND>
-__global__ __launch_bounds__(kTB) void k_batch(const BatchVol* __restrict__ vols, int nvol, int total) {
+__global__ __launch_bounds__(kTB) void k_batch(const BatchVol* __restrict__ vols, int nvol, XcdSplit split) {
   constexpr int KG = KIND <= 2 ? KIND : 0;
   constexpr int CAPE = (DP >= 2 && DP <= 4) ? cap_cells<DP, KIND>() : 0;
   constexpr int BOXF = KIND <= 2 ? box_floats<CAPE, KG>() : 2 * CAPE;  // push: CAPE int64 cells
   __shared__ __align__(8) float box[BOXF > 0 ? BOXF : 2];
   __shared__ TileState S;
   __shared__ float wsum[kTB / 64];
-  const int nb = gridDim.x / kXcds;  // the host launches a multiple of 8 workgroups
-  const int x = blockIdx.x % kXcds, i = blockIdx.x / kXcds;
-  const int per = (total + kXcds - 1) / kXcds;
-  const int t_end = min(total, (x + 1) * per);
-  for (int t = x * per + i; t < t_end; t += nb) {
-    int lo = 0, hi = nvol - 1;  // last volume whose first tile is <= t
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (vols[mid].tile0 <= t) lo = mid;
-      else hi = mid - 1;
-    }
-    const KArgs a = vols[lo].a;
-    const int tile = t - vols[lo].tile0;
-    if (KIND <= 2) gather_tile<DP, kBTX, kBTY, kBTZ, CAPE, KG>(a, tile, 0, box, wsum, S);
-    else push_tile<DP, kBTX, kBTY, kBTZ, CAPE, KIND == 4>(a, tile, 0, reinterpret_cast<acc_t*>(box), wsum, S);
+  // workgroup b runs on XCD b % 8 and takes the next tile of that XCD's range
+  const int x = blockIdx.x % kBatchXcds;
+  const int t = split.s[x] + (int)(blockIdx.x / kBatchXcds);
+  if (t >= split.s[x + 1]) return;
+  int lo = 0, hi = nvol - 1;  // last volume whose first tile is <= t
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (vols[mid].tile0 <= t) lo = mid;
+    else hi = mid - 1;
   }
+  const KArgs a = vols[lo].a;
+  const int tile = t - vols[lo].tile0;
+  if (KIND <= 2) gather_tile<DP, kBTX, kBTY, kBTZ, CAPE, KG>(a, tile, 0, box, wsum, S);
+  else push_tile<DP, kBTX, kBTY, kBTZ, CAPE, KIND == 4>(a, tile, 0, reinterpret_cast<acc_t*>(box), wsum, S);
 }
+
+int batch_tiles(const KArgs& a) { return tiles_of<kBTX, kBTY, kBTZ>(a); }
 
 // Zero what a batched push / objective accumulates into (graph-capturable).
 __global__ void k_batch_zero(const BatchVol* __restrict__ vols, int kind) {
   const BatchVol& b = vols[blockIdx.y];
-  float* p = (kind == OP_OBJGRAD) ? b.a.out2 : b.a.out;
+  PP_GLOBAL float* p = (kind == OP_OBJGRAD) ? b.a.out2 : b.a.out;
   const long long n = b.zero_n;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     p[i] = 0.0f;
   if (kind == OP_OBJGRAD && blockIdx.x == 0 && threadIdx.x == 0) b.a.red[0] = 0.0;
 }
-
-int batch_tiles(const KArgs& a) { return tiles_of<kBTX, kBTY, kBTZ>(a); }
 
 template <int KIND>
 static const void* batch_kernel(int dp) {
@@ -808,23 +808,17 @@ hipError_t launch_batch_zero(Op op, const BatchVol* dvols, int nvol, hipStream_t
   return hipGetLastError();
 }
 
-hipError_t launch_batch_group(Op op, int dp, const BatchVol* dvols, int nvol, int total, int grid, hipStream_t s) {
-  if (nvol <= 0 || total <= 0) return hipSuccess;
+hipError_t launch_batch_group(Op op, int dp, const BatchVol* dvols, int nvol, const XcdSplit& split, hipStream_t s) {
+  int longest = 0;
+  for (int x = 0; x < kBatchXcds; ++x) longest = std::max(longest, split.s[x + 1] - split.s[x]);
+  if (nvol <= 0 || longest <= 0) return hipSuccess;
   const void* k = batch_kernel_for(op, dp);
   if (!k) return hipErrorNotSupported;
-  void* args[] = {(void*)&dvols, (void*)&nvol, (void*)&total};
+  XcdSplit sp = split;
+  void* args[] = {(void*)&dvols, (void*)&nvol, (void*)&sp};
+  const long long grid = (long long)longest * kBatchXcds;
+  if (grid > INT_MAX) return hipErrorInvalidValue;
   return hipLaunchKernel(k, dim3((unsigned)grid), dim3(kTB), args, 0, s);
-}
-
-int batch_grid(Op op, int dp) {
-  const void* k = batch_kernel_for(op, dp);
-  int per_cu = 0;
-  hipError_t e = k ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kTB, 0) : hipErrorNotSupported;
-  int dev = 0, cus = 256;
-  if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  if (e != hipSuccess || per_cu < 1) per_cu = 1;
-  const int g = per_cu * cus;
-  return ((g + kXcds - 1) / kXcds) * kXcds;
 }
 
 hipError_t launch_tiled(Op op, const KArgs& a, int nbatch, hipStream_t s, TileCfg cfg) {
