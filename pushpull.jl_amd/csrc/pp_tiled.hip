@@ -35,6 +35,7 @@ struct TileState {
   int ctr[3];     // window centre (knot of the tile's centre sample)
   int org[3];     // tile origin in the phi grid
   int red[6][4];  // per-wave partial min/max
+  float redf[4];  // per-wave partial bound (tile_setup's `bound` hook)
   // folded image offset of box plane y = i (d0x * fold(lo[1] + i)) and z = i
   // (d0x * d0y * fold(lo[2] + i)); bit 31 set where the boundary flips the sign
   int yoff[kMaxYZ], zoff[kMaxYZ];
@@ -102,51 +103,68 @@ __device__ __forceinline__ bool is_fast3(float x, float y, float z) {
   return fabsf(x) < kFastLimit && fabsf(y) < kFastLimit && fabsf(z) < kFastLimit;
 }
 
+typedef unsigned long long acc_t;  // fixed-point LDS accumulator cell (scatters)
+
+// No-op hooks of tile_setup.
+struct NoPrefetch {
+  template <class VoxT>
+  __device__ __forceinline__ void operator()(const VoxT&) const {}
+};
+struct NoBound {
+  __device__ __forceinline__ float operator()() const { return 0.f; }
+};
+
 // Load the tile's samples, classify them and size the LDS box. Ends with a barrier.
-template <int DP, int TX, int TY, int TZ, int CAP>
+// Hooks, so that callers share its two barriers instead of adding their own:
+//   pre(V)    runs once the samples' grid indices are known, before the deformation is
+//             read: loads issued there overlap the deformation's;
+//   bound()   a per-thread value >= 0 whose workgroup maximum lands in *block_bound;
+//   zero_box  when given, the box's int64 cells are zeroed before the final barrier.
+template <int DP, int TX, int TY, int TZ, int CAP, class Pre = NoPrefetch, class Bound = NoBound>
 __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restrict__ phi, int tile,
-                                           Vox<Shape<TX, TY, TZ>::VPT>& V, TileState& S) {
+                                           Vox<Shape<TX, TY, TZ>::VPT>& V, TileState& S, Pre pre = Pre(),
+                                           Bound bound = Bound(), acc_t* zero_box = nullptr,
+                                           float* block_bound = nullptr) {
   using SH = Shape<TX, TY, TZ>;
   constexpr int VPT = SH::VPT;
   const int ntx = (a.d1x + TX - 1) / TX, nty = (a.d1y + TY - 1) / TY;
   const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
   const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
   const int tid = threadIdx.x;
-  if (tid == 0) {  // fallback centre: the tile centre mapped by the grid-size ratio
-    S.org[0] = x0;
-    S.org[1] = y0;
-    S.org[2] = z0;
-    S.ctr[0] = (int)((long long)(x0 + TX / 2) * a.d0x / a.d1x) - (DP - 1) / 2;
-    S.ctr[1] = (int)((long long)(y0 + TY / 2) * a.d0y / a.d1y) - (DP - 1) / 2;
-    S.ctr[2] = (int)((long long)(z0 + TZ / 2) * a.d0z / a.d1z) - (DP - 1) / 2;
-  }
-  __syncthreads();
   V.inside = 0;
   V.valid = 0;
   V.inl = 0;
 #pragma unroll
   for (int q = 0; q < VPT; ++q) {
     const int l = tid + kTB * q;
-    const int lx = l % TX, ly = (l / TX) % TY, lz = l / (TX * TY);
-    const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+    const int gx = x0 + l % TX, gy = y0 + (l / TX) % TY, gz = z0 + l / (TX * TY);
     const bool in = gx < a.d1x && gy < a.d1y && gz < a.d1z;
-    const int v = in ? gx + a.d1x * (gy + a.d1y * gz) : 0;
-    V.v[q] = v;
+    V.v[q] = in ? gx + a.d1x * (gy + a.d1y * gz) : 0;
+    if (in) V.inside |= 1u << q;
+  }
+  pre(V);
+#pragma unroll
+  for (int q = 0; q < VPT; ++q) {
+    const int l = tid + kTB * q;
+    const int gx = x0 + l % TX, gy = y0 + (l / TX) % TY, gz = z0 + l / (TX * TY);
     float x = 0.f, y = 0.f, z = 0.f;
     bool ok = false;
-    if (in) {
-      V.inside |= 1u << q;
-      sample_coords_at(a, phi, v, gx, gy, gz, x, y, z);
+    if ((V.inside >> q) & 1u) {
+      sample_coords_at(a, phi, V.v[q], gx, gy, gz, x, y, z);
       ok = voxok(x, y, z, a.ext, a.lx, a.ly, a.lz);
     }
     V.x[q] = x;
     V.y[q] = y;
     V.z[q] = z;
     if (ok) V.valid |= 1u << q;
-    if (l == SH::LC && ok && is_fast3(x, y, z)) {
-      S.ctr[0] = knot_fast<DP>(x);
-      S.ctr[1] = knot_fast<DP>(y);
-      S.ctr[2] = knot_fast<DP>(z);
+    if (l == SH::LC) {  // window centre: the centre sample's knot, else the tile centre
+      S.org[0] = x0;    // mapped by the grid-size ratio
+      S.org[1] = y0;
+      S.org[2] = z0;
+      const bool fast = ok && is_fast3(x, y, z);
+      S.ctr[0] = fast ? knot_fast<DP>(x) : (int)((long long)(x0 + TX / 2) * a.d0x / a.d1x) - (DP - 1) / 2;
+      S.ctr[1] = fast ? knot_fast<DP>(y) : (int)((long long)(y0 + TY / 2) * a.d0y / a.d1y) - (DP - 1) / 2;
+      S.ctr[2] = fast ? knot_fast<DP>(z) : (int)((long long)(z0 + TZ / 2) * a.d0z / a.d1z) - (DP - 1) / 2;
     }
   }
   __syncthreads();
@@ -164,12 +182,22 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
   }
   mn0 = wave_min(mn0); mn1 = wave_min(mn1); mn2 = wave_min(mn2);
   mx0 = wave_max(mx0); mx1 = wave_max(mx1); mx2 = wave_max(mx2);
+  float bnd = bound();
+  if (block_bound)
+    for (int o = 32; o > 0; o >>= 1) bnd = fmaxf(bnd, __shfl_xor(bnd, o));
   const int wid = tid >> 6;
   if ((tid & 63) == 0) {
     S.red[0][wid] = mn0; S.red[1][wid] = mn1; S.red[2][wid] = mn2;
     S.red[3][wid] = mx0; S.red[4][wid] = mx1; S.red[5][wid] = mx2;
+    S.redf[wid] = bnd;
   }
   __syncthreads();
+  if (block_bound) {
+    float m = S.redf[0];
+#pragma unroll
+    for (int w = 1; w < kTB / 64; ++w) m = fmaxf(m, S.redf[w]);
+    *block_bound = m;
+  }
   // every thread sizes the box (a broadcast read of the partials)
   int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
 #pragma unroll
@@ -192,6 +220,8 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
     S.cells = cells;
     S.lds = use;
   }
+  if (use && zero_box)
+    for (int i = tid; i < cells; i += kTB) zero_box[i] = 0ull;
   if (use) {  // fold tables of the box planes
     bool neg;
     if (tid < by) {
@@ -293,8 +323,6 @@ __device__ __forceinline__ void stage_box(float* box, const TileState& S, const 
 // contribution is below 2^50 and no cell can overflow (at most 2^10 samples per tile
 // touch a cell), integer sums are exact and order-independent, and each cell is
 // rounded to float once, when it is flushed.
-typedef unsigned long long acc_t;
-
 __device__ __forceinline__ int fixed_scale(float vmax) {
   if (!(vmax > 0.0f)) return 0;
   int e;
@@ -312,6 +340,16 @@ __device__ __forceinline__ acc_t fix1(double a, double b) {
 }
 __device__ __forceinline__ acc_t fix2(double a, double b, double c, double d) {
   return (acc_t)(__double_as_longlong(fma(a, b, fma(c, d, kMagic))) - __double_as_longlong(kMagic));
+}
+
+// q * 2^-scale rounded to float: below 2^51 in magnitude q converts exactly through the
+// same magic constant (as fix1, in reverse); larger sums take the general conversion.
+__device__ __forceinline__ float fixed_to_float(long long q, int scale) {
+  if (q > -(1LL << 51) && q < (1LL << 51)) {
+    const double d = __longlong_as_double(q + __double_as_longlong(kMagic)) - kMagic;
+    return (float)ldexp(d, -scale);
+  }
+  return ldexpf((float)q, -scale);
 }
 
 __device__ __forceinline__ void zero_acc(acc_t* box, int cells) {
@@ -341,7 +379,7 @@ __device__ __forceinline__ void flush_acc(const acc_t* box, const TileState& S, 
     if (q != 0) {
       bool neg;
       const int off = walk_offset(S, w, signs, neg);
-      const float v = ldexpf((float)q, -scale);
+      const float v = fixed_to_float(q, scale);
       unsafeAtomicAdd(dst + off, neg ? -v : v);
     }
   }
@@ -372,11 +410,34 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, acc_
   constexpr int VPT = Shape<TX, TY, TZ>::VPT;
   constexpr int NG = GRAD ? 3 : 1;
   const float* phi = a.phi + bb * a.phi_bs;
+  const long long n1 = a.n1;
+  // the first channel's values are loaded before the deformation is read
+  float g[NG][VPT];
+  const float* src0 = a.f + bb * a.f_bs;
+  const bool read0 = GRAD || (a.vmode == 0 && a.nchan > 0);
+  auto prefetch = [&](const Vox<VPT>& W) {
+#pragma unroll
+    for (int q = 0; q < VPT; ++q)
+#pragma unroll
+      for (int e = 0; e < NG; ++e) g[e][q] = (read0 && ((W.inside >> q) & 1u)) ? src0[W.v[q] + e * n1] : 0.f;
+  };
+  // channel 0's scale bound: the largest finite per-sample bound over all samples in the
+  // grid (a superset of the in-window ones, so a valid bound), reduced with the box
+  auto bound0 = [&]() {
+    float m = 0.f;
+#pragma unroll
+    for (int q = 0; q < VPT; ++q) {
+      float b = fabsf(g[0][q]);
+      if (GRAD) b += fabsf(g[NG > 1 ? 1 : 0][q]) + fabsf(g[NG > 2 ? 2 : 0][q]);
+      if (isfinite(b)) m = fmaxf(m, b);
+    }
+    return m;
+  };
   Vox<VPT> V;
-  tile_setup<DP, TX, TY, TZ, CAP>(a, phi, tile, V, S);
+  float bound_c0 = 0.f;
+  tile_setup<DP, TX, TY, TZ, CAP>(a, phi, tile, V, S, prefetch, bound0, box, &bound_c0);
   const bool lds = S.lds != 0;
   const int BX = S.b[0], BXY = S.b[0] * S.b[1];
-  const long long n1 = a.n1;
   // push with a count image (out2): one more channel of ones after the nchan channels
   const int nsrc = a.nchan + ((!GRAD && a.out2) ? 1 : 0);
   for (int c = 0; c < nsrc; ++c) {
@@ -385,18 +446,19 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, acc_
     const float* src = a.f + bb * a.f_bs + c * a.f_cs;
     float* dst = count ? a.out2 + bb * a.out2_bs : a.out + bb * a.out_bs + c * a.out_cs;
     // this channel's values of all the thread's valid samples, loaded together
-    float g[NG][VPT];
+    if (!(c == 0 && read0)) {
 #pragma unroll
-    for (int q = 0; q < VPT; ++q) {
-      const int l = threadIdx.x + kTB * q;
-      const int ax = c + a.chan0;
-      const int id = ax == 0 ? S.org[0] + l % TX : ax == 1 ? S.org[1] + (l / TX) % TY : S.org[2] + l / (TX * TY);
+      for (int q = 0; q < VPT; ++q) {
+        const int l = threadIdx.x + kTB * q;
+        const int ax = c + a.chan0;
+        const int id = ax == 0 ? S.org[0] + l % TX : ax == 1 ? S.org[1] + (l / TX) % TY : S.org[2] + l / (TX * TY);
 #pragma unroll
-      for (int e = 0; e < NG; ++e)
-        g[e][q] = !((V.valid >> q) & 1u) ? 0.f
-                  : vmode == 0           ? src[V.v[q] + e * n1]
-                  : vmode == 1           ? 1.0f
-                                         : (float)(id + 1);
+        for (int e = 0; e < NG; ++e)
+          g[e][q] = !((V.valid >> q) & 1u) ? 0.f
+                    : vmode == 0           ? src[V.v[q] + e * n1]
+                    : vmode == 1           ? 1.0f
+                                           : (float)(id + 1);
+      }
     }
     unsigned use = lds ? V.inl : 0u;  // samples accumulated in LDS for this channel
     int scale = 0;
@@ -411,9 +473,13 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, acc_
         if (isfinite(b)) vmax = fmaxf(vmax, b);
         else use &= ~(1u << q);
       }
-      scale = fixed_scale(block_max(vmax, wsum));
-      zero_acc(box, S.cells);
-      __syncthreads();
+      if (c == 0 && read0) {  // bound reduced and box zeroed within tile_setup's barriers
+        scale = fixed_scale(bound_c0);
+      } else {
+        scale = fixed_scale(block_max(vmax, wsum));
+        zero_acc(box, S.cells);
+        __syncthreads();
+      }
     }
 #pragma unroll 1
     for (int q = 0; q < VPT; ++q, V.rotate(), rot_rows(g)) {
@@ -548,8 +614,14 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
                                             TileState& S) {
   constexpr int VPT = Shape<TX, TY, TZ>::VPT;
   const float* phi = a.phi + bb * a.phi_bs;
+  float auxv[VPT];  // upstream gradient / target; channel 0's is loaded before phi is read
+  auto prefetch = [&](const Vox<VPT>& W) {
+#pragma unroll
+    for (int q = 0; q < VPT; ++q)
+      auxv[q] = (KIND != 0 && ((W.inside >> q) & 1u)) ? a.aux[bb * a.aux_bs + W.v[q]] : 0.f;
+  };
   Vox<VPT> V;
-  tile_setup<DP, TX, TY, TZ, CAP>(a, phi, tile, V, S);
+  tile_setup<DP, TX, TY, TZ, CAP>(a, phi, tile, V, S, prefetch);
   const bool lds = S.lds != 0;
   const int BX = S.b[0], BXY = S.b[0] * S.b[1];
   const long long n1 = a.n1;
@@ -560,11 +632,11 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
   for (int c = 0; c < a.nchan; ++c) {
     const float* src = a.f + bb * a.f_bs + c * a.f_cs;
     unsigned rfin = 0u;  // KIND 2: in-window samples with a finite residual (scattered via LDS)
-    float auxv[VPT];     // upstream gradient / target, loaded before staging so both overlap
     float res[VPT];      // KIND 2: residual of each sample
 #pragma unroll
     for (int q = 0; q < VPT; ++q) {
-      auxv[q] = (KIND != 0 && ((V.valid >> q) & 1u)) ? a.aux[bb * a.aux_bs + c * a.aux_cs + V.v[q]] : 0.f;
+      if (c > 0)  // loaded before staging so both overlap
+        auxv[q] = (KIND != 0 && ((V.valid >> q) & 1u)) ? a.aux[bb * a.aux_bs + c * a.aux_cs + V.v[q]] : 0.f;
       res[q] = 0.f;
     }
     if (lds) {
