@@ -205,7 +205,7 @@ __global__ __launch_bounds__(256) void k_objgrad(KArgs a) {
     }
   }
   // block reduction of the loss, one double atomic per block
-  for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o);
+  lsum = __ockl_wfred_add_f32(lsum);
   __shared__ float wsum[4];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (lane == 0) wsum[wid] = lsum;

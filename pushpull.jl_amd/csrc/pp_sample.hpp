@@ -66,6 +66,13 @@ __device__ __forceinline__ float pull_out(const KArgs& a, float val, int c, int 
   return val;
 }
 
+// Wave-wide reductions of the device library (DPP lane moves; the result reaches every
+// active lane).
+extern "C" __device__ __attribute__((const)) int __ockl_wfred_min_i32(int);
+extern "C" __device__ __attribute__((const)) int __ockl_wfred_max_i32(int);
+extern "C" __device__ __attribute__((const)) float __ockl_wfred_max_f32(float);
+extern "C" __device__ __attribute__((const)) float __ockl_wfred_add_f32(float);
+
 template <int DP, int NEED>
 __device__ __forceinline__ bool setup(const KArgs& a, const float* phi, int v, Axis<DP>& ax, Axis<DP>& ay,
                                       Axis<DP>& az) {

@@ -85,14 +85,9 @@ struct Vox {
   }
 };
 
-__device__ __forceinline__ int wave_min(int v) {
-  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
-  return v;
-}
-__device__ __forceinline__ int wave_max(int v) {
-  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
-  return v;
-}
+// Wave-wide reductions (DPP lane moves, not LDS-routed permutes).
+__device__ __forceinline__ int wave_min(int v) { return __ockl_wfred_min_i32(v); }
+__device__ __forceinline__ int wave_max(int v) { return __ockl_wfred_max_i32(v); }
 
 template <int DP>
 __device__ __forceinline__ int knot_fast(float x) {
@@ -161,10 +156,15 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
       S.org[0] = x0;    // mapped by the grid-size ratio
       S.org[1] = y0;
       S.org[2] = z0;
-      const bool fast = ok && is_fast3(x, y, z);
-      S.ctr[0] = fast ? knot_fast<DP>(x) : (int)((long long)(x0 + TX / 2) * a.d0x / a.d1x) - (DP - 1) / 2;
-      S.ctr[1] = fast ? knot_fast<DP>(y) : (int)((long long)(y0 + TY / 2) * a.d0y / a.d1y) - (DP - 1) / 2;
-      S.ctr[2] = fast ? knot_fast<DP>(z) : (int)((long long)(z0 + TZ / 2) * a.d0z / a.d1z) - (DP - 1) / 2;
+      if (ok && is_fast3(x, y, z)) {
+        S.ctr[0] = knot_fast<DP>(x);
+        S.ctr[1] = knot_fast<DP>(y);
+        S.ctr[2] = knot_fast<DP>(z);
+      } else {  // (double: exact for grids below 2^26 per axis, and no 64-bit division)
+        S.ctr[0] = (int)((double)(x0 + TX / 2) * a.d0x / a.d1x) - (DP - 1) / 2;
+        S.ctr[1] = (int)((double)(y0 + TY / 2) * a.d0y / a.d1y) - (DP - 1) / 2;
+        S.ctr[2] = (int)((double)(z0 + TZ / 2) * a.d0z / a.d1z) - (DP - 1) / 2;
+      }
     }
   }
   __syncthreads();
@@ -184,7 +184,7 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
   mx0 = wave_max(mx0); mx1 = wave_max(mx1); mx2 = wave_max(mx2);
   float bnd = bound();
   if (block_bound)
-    for (int o = 32; o > 0; o >>= 1) bnd = fmaxf(bnd, __shfl_xor(bnd, o));
+    bnd = __ockl_wfred_max_f32(bnd);
   const int wid = tid >> 6;
   if ((tid & 63) == 0) {
     S.red[0][wid] = mn0; S.red[1][wid] = mn1; S.red[2][wid] = mn2;
@@ -358,7 +358,7 @@ __device__ __forceinline__ void zero_acc(acc_t* box, int cells) {
 
 // Largest value over the workgroup (v >= 0); uses and releases wsum.
 __device__ __forceinline__ float block_max(float v, float* wsum) {
-  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+  v = __ockl_wfred_max_f32(v);
   if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
   __syncthreads();
   float m = wsum[0];
@@ -736,7 +736,7 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
     }
   }
   if (KIND == 2) {
-    for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o);
+    lsum = __ockl_wfred_add_f32(lsum);
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = lsum;
     __syncthreads();
     if (threadIdx.x == 0) {
