@@ -74,6 +74,26 @@ void pull_affine(float *f1, const float *f0, const pp_usize *d0, pp_usize n1, co
 void push_affine(float *f0, const float *f1, const pp_usize *d0, pp_usize n1, const pp_usize *d1,
                  const float *Aff, const int *bnd, const pp_usize *dp, int ext);
 
+/* Sparse regulariser (artifacts/C/sparse_operator.c, built by the reference as its own
+ * library, `oplib` of src/operator_sparse.jl). d = (X, Y, Z, nchan, hcode); the stencil
+ * is the `sparsify` tuple of src/operator.jl:206-295 (offset, length: nchan x nchan,
+ * column-major; values, patch_indices: the lists). `indices` is accepted and not read:
+ * interior offsets are derived from patch_indices and d (sparsify writes both from the
+ * same positions). Results are bit-identical to the reference's CPU code. */
+
+/* u = L v. Like the reference, voxels within (dp-1)/2 of a face ADD to the caller's u and
+ * the others overwrite it. Replaces sparse_operator.c:45 `vel2mom`. */
+void vel2mom(float *u, const float *v, const pp_usize *d, const int *offset, const int *length,
+             const float *values, const int *indices, const int *patch_indices, const pp_usize *dp,
+             const int *bnd);
+
+/* One Gauss-Seidel sweep of (L + H) v = g, updating v. h has nchan (d[4] == 1) or
+ * nchan*(nchan+1)/2 (d[4] == 2) channels, or is unused (d[4] == 0). Replaces
+ * sparse_operator.c:116 `relax`. */
+void relax(float *v, const pp_usize *d, const float *g, const float *h, const int *offset,
+           const int *length, const float *values, const int *indices, const int *patch_indices,
+           const pp_usize *dp, const int *bnd);
+
 /* ---------------------------------------------------------------- Tier B ---- */
 
 enum {
@@ -169,6 +189,37 @@ int pp_affine_push_count(const pp_desc *d, float *f0, float *c0, const float *f1
 int pp_compose(float *psi, const float *phi, const float *theta, const long long d0[3], const long long d1[3],
                int nbatch, void *stream);
 int pp_exp(float *out, const float *v, float *work, const long long dim[3], int nbatch, int nsq, void *stream);
+
+/* ---- sparse regulariser (src/operator_sparse.jl, src/operator.jl) ------------
+ * A stencil L in the `sparsify` list format (src/operator.jl:206-295) is uploaded once
+ * for volumes of size d; the calls then take device arrays (X, Y, Z, nchan), column-major,
+ * and a host array bnd of 3*nchan boundary codes (channel c's triple at 3c). nchan <= 8,
+ * stride odd and <= 5 per axis. Results are bit-identical to artifacts/C/sparse_operator.c.
+ * The kernel object is tied to the device current at creation. */
+typedef struct pp_sparse_kernel pp_sparse_kernel;
+
+/* offset/length: nchan*nchan (pair (i, j) at i + nchan*j, i the output channel); the list
+ * of a pair is values/patch_indices[offset .. offset+length), values[offset-1] its sum of
+ * non-centre values. Host arrays, copied. Replaces the kernel tuple passed to every call
+ * of src/operator_sparse.jl:16-62. */
+int pp_sparse_create(pp_sparse_kernel **kernel, const int64_t d[3], int nchan, const int32_t stride[3],
+                     const int32_t *offset, const int32_t *length, const float *values,
+                     const int32_t *patch_indices, int64_t nvalues);
+int pp_sparse_destroy(pp_sparse_kernel *kernel);
+
+/* u = L v (src/operator_sparse.jl:6-40). accumulate_edges = 1 reproduces vel2mom!'s
+ * effect on a non-zero u exactly (voxels within (dp-1)/2 of a face add to u); 0 overwrites
+ * u everywhere (vel2mom). u and v must be different arrays. */
+int pp_vel2mom(const pp_sparse_kernel *kernel, float *u, const float *v, const int32_t *bnd,
+               int accumulate_edges, void *stream);
+
+/* nit Gauss-Seidel sweeps of (L + H) v = g, updating v in place (src/operator_sparse.jl:43-62).
+ * hcode 0: no H (h may be NULL); 1: h holds nchan diagonal channels; 2: nchan*(nchan+1)/2
+ * channels, the diagonal first and then the pairs (j, i > j) in order. Boundary codes must
+ * be 0, 1 or 2 (src/operator_sparse.jl:79-81). Each sweep runs the reference's colour
+ * order as one launch per colour of each of its seven regions. */
+int pp_relax(const pp_sparse_kernel *kernel, float *v, const float *g, const float *h, int hcode,
+             const int32_t *bnd, int nit, void *stream);
 
 /* ---- batches of uneven volumes --------------------------------------------
  * No counterpart in the reference, whose wrappers loop over same-size (channel, batch)

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "pp_kernels.hpp"
+#include "pp_sparse.hpp"
 #include "pushpull_hip.h"
 
 namespace {
@@ -397,6 +398,11 @@ void nan_fill(float* p, long long n) {
 long long prod3(const pp_usize* d) { return d ? (long long)(d[0] * d[1] * d[2]) : 0; }
 
 }  // namespace
+
+namespace pp {
+int abi_fail(int code, const char* msg) { return fail(code, "%s", msg); }
+int abi_ok() { return ok(); }
+}  // namespace pp
 
 extern "C" {
 

@@ -1,0 +1,68 @@
+// pp_sparse.hpp -- host-side view of the sparse-regulariser kernels (internal to
+// libpushpull_hip): u = L v (vel2mom) and the Gauss-Seidel update of (L + H) v = g (relax)
+// for a sparse stencil L in the list format of src/operator.jl:206-295 (`sparsify`).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "pp_kernels.hpp"
+
+namespace pp {
+
+constexpr int kSpMaxChan = 8;  // MAXN of artifacts/C/sparse_operator_dev.cu:8
+constexpr int kSpMaxBox = 5;   // largest stencil box side (src/operator_sparse.jl:64)
+
+// One (output i, input j) list: entries [start, start + len) of the entry table. Entry
+// `start` is the list's first position (its value is read as f0); w0 is its weight and
+// c0w = w0 + (sum of the list's other weights), rounded as one float add.
+struct SpPair {
+  int start, len;
+  float w0, c0w;
+};
+// One stencil position: off is the linear offset from the centre voxel inside the volume
+// (interior voxels); code packs the box offsets (dx, dy, dz) as bytes biased by 16.
+struct SpEntry {
+  int off;
+  int code;
+  float w;
+  int pad;
+};
+
+struct SpArgs {
+  const PP_GLOBAL SpPair* pairs;  // nc * nc, pair (i, j) at i + nc * j
+  const PP_GLOBAL SpEntry* ent;
+  PP_GLOBAL float* v;             // vel2mom: input; relax: the estimate, updated in place
+  PP_GLOBAL float* u;             // vel2mom: output
+  const PP_GLOBAL float* g;       // relax: right-hand side (nc channels)
+  const PP_GLOBAL float* h;       // relax: H (nc or nc (nc + 1) / 2 channels), or null
+  long long nd;                   // voxels per channel
+  int d[3];
+  int nc;
+  int hcode;                      // 0 no H, 1 diagonal, 2 full symmetric
+  int accum;                      // vel2mom: edge voxels add to u rather than overwrite
+  int bnd[3 * kSpMaxChan];        // boundary codes of input channel j at 3 j
+};
+
+// The voxels start + step * (ix, iy, iz), 0 <= i* < cnt*, numbered with x fastest.
+// rx, ry: 1/cnt[0], 1/cnt[1] in double (exact-quotient decode with one correction step).
+// split/key (relax under a circulant fold whose period is not a multiple of the colour
+// stride): bit 0 / bit 1 of split select the x / y test "within r of the upper end"; a
+// voxel runs only when its test bits equal key.
+struct SpLattice {
+  int start[3], step[3], cnt[3];
+  long long n;
+  double rx, ry;
+  int split, key;
+  int r[3];
+};
+
+enum SpOp : int { SP_VEL2MOM = 0, SP_RELAX = 1 };
+
+// edge: fold every stencil read through the boundary codes (voxels within r of a face);
+// otherwise the lattice lies in the interior and reads use the entries' linear offsets.
+hipError_t launch_sparse(SpOp op, const SpArgs& a, const SpLattice& l, bool edge, hipStream_t stream);
+
+// Error state of the C ABI (pp_last_error / pp_last_status), shared by its source files.
+int abi_fail(int code, const char* msg);
+int abi_ok();
+
+}  // namespace pp

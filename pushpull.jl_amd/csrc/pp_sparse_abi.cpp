@@ -1,0 +1,398 @@
+// pp_sparse_abi.cpp -- extern "C" entry points of the sparse regulariser (declared in
+// include/pushpull_hip.h): kernel upload, the sweep schedule of relax, Tier A staging.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "pp_sparse.hpp"
+#include "pushpull_hip.h"
+
+// A sparse stencil on one device, for volumes of size d.
+struct pp_sparse_kernel {
+  int dev;
+  int d[3];
+  int nc;
+  int dp[3];
+  void* mem;  // one allocation: pairs, then entries
+  pp::SpPair* pairs;
+  pp::SpEntry* ent;
+};
+
+namespace {
+
+constexpr long long kMaxVox = (1LL << 31) - 1;
+constexpr long long kMaxDim = 1LL << 29;
+
+int fail(int code, const char* fmt, long long a = 0, long long b = 0, long long c = 0) {
+  char buf[512];
+  std::snprintf(buf, sizeof buf, fmt, a, b, c);
+  return pp::abi_fail(code, buf);
+}
+
+int hip_fail(hipError_t e, const char* where) {
+  char buf[512];
+  std::snprintf(buf, sizeof buf, "%s: %s", where, hipGetErrorString(e));
+  return pp::abi_fail(PP_EDEVICE, buf);
+}
+
+// Checks the list arrays (the `sparsify` tuple, src/operator.jl:206-295) and builds the
+// device tables: for each pair its entries in list order, the first one being the value
+// read as f0 (sparse_operator_dev.cu:15).
+int create(pp_sparse_kernel** out, const long long d[3], int nc, const int dp[3], const int* offset, const int* length,
+           const float* values, const int* patch, long long nvalues) {
+  if (!out) return fail(PP_EINVAL, "null kernel handle");
+  *out = nullptr;
+  if (!d || !dp || !offset || !length || !values || !patch) return fail(PP_EINVAL, "null argument");
+  if (nc < 1 || nc > pp::kSpMaxChan) return fail(PP_EINVAL, "nchan=%lld outside 1..8", nc);
+  long long nvox = 1;
+  for (int q = 0; q < 3; ++q) {
+    if (d[q] < 1 || d[q] > kMaxDim) return fail(PP_EINVAL, "d[%lld]=%lld out of range", q, d[q]);
+    if (dp[q] < 1 || dp[q] > pp::kSpMaxBox) return fail(PP_EINVAL, "regulariser dimensions too big (stride[%lld]=%lld)", q, dp[q]);
+    if (dp[q] % 2 == 0) return fail(PP_EINVAL, "stride[%lld]=%lld is even (stencil boxes are odd)", q, dp[q]);
+    nvox *= d[q];
+  }
+  if (nvox > kMaxVox) return fail(PP_ERANGE, "volume larger than 2^31-1 voxels");
+  const int nbox = dp[0] * dp[1] * dp[2];
+  std::vector<pp::SpPair> pairs(nc * nc);
+  std::vector<pp::SpEntry> ent;
+  for (int p = 0; p < nc * nc; ++p) {
+    const long long o = offset[p], n = length[p];
+    if (n < 1 || o < 1 || o + n > nvalues)
+      return fail(PP_EINVAL, "list %lld: offset %lld / length %lld outside the value array", p, o, n);
+    pairs[p].start = (int)ent.size();
+    pairs[p].len = (int)n;
+    pairs[p].w0 = values[o];
+    pairs[p].c0w = values[o] + values[o - 1];
+    for (long long e = o; e < o + n; ++e) {
+      const int t = patch[e];
+      if (t < 0 || t >= nbox) return fail(PP_EINVAL, "list %lld: patch index %lld outside the stencil box", p, t);
+      const int dx = t % dp[0] - dp[0] / 2, dy = (t / dp[0]) % dp[1] - dp[1] / 2, dz = t / (dp[0] * dp[1]) - dp[2] / 2;
+      pp::SpEntry s;
+      s.off = (int)(dx + d[0] * (dy + d[1] * (long long)dz));
+      s.code = (dx + 16) | ((dy + 16) << 8) | ((dz + 16) << 16);
+      s.w = values[e];
+      s.pad = 0;
+      ent.push_back(s);
+    }
+  }
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  const size_t pb = pairs.size() * sizeof(pp::SpPair), eb = ent.size() * sizeof(pp::SpEntry);
+  const size_t pa = (pb + 255) & ~(size_t)255;
+  void* mem = nullptr;
+  if ((e = hipMalloc(&mem, pa + eb)) != hipSuccess) return fail(PP_ENOMEM, "hipMalloc of the sparse kernel failed");
+  char* base = static_cast<char*>(mem);
+  if ((e = hipMemcpy(base, pairs.data(), pb, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(base + pa, ent.data(), eb, hipMemcpyHostToDevice)) != hipSuccess) {
+    (void)hipFree(mem);
+    return hip_fail(e, "upload of the sparse kernel");
+  }
+  pp_sparse_kernel* k = new pp_sparse_kernel;
+  k->dev = dev;
+  k->nc = nc;
+  for (int q = 0; q < 3; ++q) {
+    k->d[q] = (int)d[q];
+    k->dp[q] = dp[q];
+  }
+  k->mem = mem;
+  k->pairs = reinterpret_cast<pp::SpPair*>(base);
+  k->ent = reinterpret_cast<pp::SpEntry*>(base + pa);
+  *out = k;
+  return pp::abi_ok();
+}
+
+int check_kernel(const pp_sparse_kernel* k) {
+  if (!k) return fail(PP_EINVAL, "null sparse kernel");
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev != k->dev) return fail(PP_EINVAL, "sparse kernel lives on device %lld, current device is %lld", k->dev, dev);
+  return PP_OK;
+}
+
+pp::SpArgs base_args(const pp_sparse_kernel* k, const int* bnd) {
+  pp::SpArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.pairs = k->pairs;
+  a.ent = k->ent;
+  a.nd = (long long)k->d[0] * k->d[1] * k->d[2];
+  for (int q = 0; q < 3; ++q) a.d[q] = k->d[q];
+  a.nc = k->nc;
+  for (int q = 0; q < 3 * k->nc; ++q) a.bnd[q] = bnd[q];
+  return a;
+}
+
+struct Box {
+  int lo[3], hi[3];
+};
+
+// The seven regions of artifacts/C/sparse_operator.c:48-61 / :121-135, in that order:
+// six slabs within r = (dp-1)/2 of a face, then the interior block.
+void regions(const pp_sparse_kernel* k, Box b[7]) {
+  int s[3], e[3], d[3];
+  for (int q = 0; q < 3; ++q) {
+    d[q] = k->d[q];
+    s[q] = std::min((k->dp[q] - 1) / 2, d[q]);
+    e[q] = std::max(s[q], d[q] - (k->dp[q] - 1) / 2);
+  }
+  const int r[7][6] = {{0, d[0], 0, d[1], 0, s[2]},       {0, d[0], 0, s[1], s[2], e[2]},
+                       {0, s[0], s[1], e[1], s[2], e[2]}, {e[0], d[0], s[1], e[1], s[2], e[2]},
+                       {0, d[0], e[1], d[1], s[2], e[2]}, {0, d[0], 0, d[1], e[2], d[2]},
+                       {s[0], e[0], s[1], e[1], s[2], e[2]}};
+  for (int i = 0; i < 7; ++i)
+    for (int q = 0; q < 3; ++q) {
+      b[i].lo[q] = r[i][2 * q];
+      b[i].hi[q] = r[i][2 * q + 1];
+    }
+}
+
+pp::SpLattice lattice(const pp_sparse_kernel* k, const int start[3], const int step[3], const int hi[3]) {
+  pp::SpLattice l;
+  std::memset(&l, 0, sizeof l);
+  l.n = 1;
+  for (int q = 0; q < 3; ++q) {
+    l.start[q] = start[q];
+    l.step[q] = step[q];
+    l.cnt[q] = start[q] < hi[q] ? (hi[q] - start[q] + step[q] - 1) / step[q] : 0;
+    l.n *= l.cnt[q];
+    l.r[q] = (k->dp[q] - 1) / 2;
+  }
+  l.rx = l.cnt[0] ? 1.0 / l.cnt[0] : 0.0;
+  l.ry = l.cnt[1] ? 1.0 / l.cnt[1] : 0.0;
+  return l;
+}
+
+int vel2mom_impl(const pp_sparse_kernel* k, float* u, const float* v, const int* bnd, int accum, hipStream_t s) {
+  if (int r = check_kernel(k)) return r;
+  if (!u || !v || !bnd) return fail(PP_EINVAL, "null argument");
+  if (u == v) return fail(PP_EINVAL, "u and v must be different arrays");
+  pp::SpArgs a = base_args(k, bnd);
+  a.v = const_cast<float*>(v);
+  a.u = u;
+  a.accum = accum != 0;
+  Box b[7];
+  regions(k, b);
+  const int one[3] = {1, 1, 1};
+  for (int i = 0; i < 7; ++i) {
+    const pp::SpLattice l = lattice(k, b[i].lo, one, b[i].hi);
+    hipError_t e = pp::launch_sparse(pp::SP_VEL2MOM, a, l, i < 6, s);
+    if (e != hipSuccess) return hip_fail(e, "vel2mom launch");
+  }
+  return pp::abi_ok();
+}
+
+// One sweep: every region, colour by colour (sparse_operator.c:66-114). Under a circulant
+// fold whose period is not a multiple of dp, the first and last voxels of a colour along
+// x (or y) read each other through the wrap; the sequential sweep updates the lower one
+// first, so such a colour runs in up to four launches: voxels away from the upper ends,
+// then those within r of the upper x end, of the upper y end, of both.
+int relax_sweep(const pp_sparse_kernel* k, const pp::SpArgs& a, hipStream_t s) {
+  Box b[7];
+  regions(k, b);
+  int wrap = 0;  // bit q: some channel folds axis q circulantly with d % dp != 0
+  for (int q = 0; q < 2; ++q)
+    for (int c = 0; c < k->nc; ++c)
+      if (a.bnd[3 * c + q] == 0 && k->dp[q] > 1 && k->d[q] % k->dp[q] != 0) wrap |= 1 << q;
+  for (int i = 0; i < 7; ++i) {
+    const Box& r = b[i];
+    int split = 0;
+    for (int q = 0; q < 2; ++q)
+      if ((wrap >> q & 1) && r.lo[q] == 0 && r.hi[q] == k->d[q]) split |= 1 << q;
+    int c[3];
+    for (c[2] = r.lo[2]; c[2] < std::min(r.hi[2], r.lo[2] + k->dp[2]); ++c[2])
+      for (c[1] = r.lo[1]; c[1] < std::min(r.hi[1], r.lo[1] + k->dp[1]); ++c[1])
+        for (c[0] = r.lo[0]; c[0] < std::min(r.hi[0], r.lo[0] + k->dp[0]); ++c[0]) {
+          pp::SpLattice l = lattice(k, c, k->dp, r.hi);
+          if (l.n == 0) continue;
+          l.split = split;
+          for (int key = 0; key < 4; ++key) {
+            if (key & ~split) continue;
+            l.key = key;
+            hipError_t e = pp::launch_sparse(pp::SP_RELAX, a, l, i < 6, s);
+            if (e != hipSuccess) return hip_fail(e, "relax launch");
+          }
+        }
+  }
+  return PP_OK;
+}
+
+int relax_impl(const pp_sparse_kernel* k, float* v, const float* g, const float* h, int hcode, const int* bnd, int nit,
+               hipStream_t s) {
+  if (int r = check_kernel(k)) return r;
+  if (!v || !g || !bnd) return fail(PP_EINVAL, "null argument");
+  if (hcode < 0 || hcode > 2) return fail(PP_EINVAL, "hcode=%lld outside 0..2", hcode);
+  if (hcode && !h) return fail(PP_EINVAL, "null h with hcode %lld", hcode);
+  if (v == g || (hcode && v == h)) return fail(PP_EINVAL, "v must not alias g or h");
+  if (nit < 0) return fail(PP_EINVAL, "nit=%lld is negative", nit);
+  pp::SpArgs a = base_args(k, bnd);
+  a.v = v;
+  a.g = g;
+  a.h = hcode ? h : nullptr;
+  a.hcode = hcode;
+  for (int it = 0; it < nit; ++it)
+    if (int r = relax_sweep(k, a, s)) return r;
+  return pp::abi_ok();
+}
+
+// Tier A: host arrays; the list-array length is implied by the offsets and lengths.
+long long values_len(int nc, const int* offset, const int* length) {
+  long long n = 0;
+  for (int p = 0; p < nc * nc; ++p) n = std::max(n, (long long)offset[p] + length[p]);
+  return n;
+}
+
+void nan_fill(float* p, long long n) {
+  if (!p) return;
+  for (long long i = 0; i < n; ++i) p[i] = NAN;
+}
+
+void destroy_quiet(pp_sparse_kernel* k) {
+  if (!k) return;
+  (void)hipFree(k->mem);
+  delete k;
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+// d: (d0, d1, d2, nchan[, hcode]); returns voxels per channel or -1 when d is unusable.
+long long tierA_dims(const pp_usize* d, long long dd[3], int& nc) {
+  if (!d) return -1;
+  long long n = 1;
+  for (int q = 0; q < 3; ++q) {
+    if (d[q] < 1 || d[q] > (pp_usize)kMaxDim) return -1;
+    dd[q] = (long long)d[q];
+    n *= dd[q];
+  }
+  if (d[3] < 1 || d[3] > (pp_usize)pp::kSpMaxChan || n > kMaxVox) return -1;
+  nc = (int)d[3];
+  return n;
+}
+
+int tierA_kernel(pp_sparse_kernel** k, const pp_usize* d, const int* offset, const int* length, const float* values,
+                 const int* patch, const pp_usize* dp) {
+  long long dd[3];
+  int nc = 0;
+  if (tierA_dims(d, dd, nc) < 0) return fail(PP_EINVAL, "d out of range");
+  if (!offset || !length || !dp) return fail(PP_EINVAL, "null argument");
+  int sdp[3];
+  for (int q = 0; q < 3; ++q) {
+    if (dp[q] < 1 || dp[q] > (pp_usize)pp::kSpMaxBox)
+      return fail(PP_EINVAL, "regulariser dimensions too big (stride[%lld]=%lld)", q, (long long)dp[q]);
+    sdp[q] = (int)dp[q];
+  }
+  return create(k, dd, nc, sdp, offset, length, values, patch, values_len(nc, offset, length));
+}
+
+}  // namespace
+
+extern "C" {
+
+int pp_sparse_create(pp_sparse_kernel** k, const int64_t d[3], int nchan, const int32_t stride[3],
+                     const int32_t* offset, const int32_t* length, const float* values, const int32_t* patch_indices,
+                     int64_t nvalues) {
+  if (!d || !stride) return fail(PP_EINVAL, "null argument");
+  const long long dd[3] = {d[0], d[1], d[2]};
+  const int dp[3] = {stride[0], stride[1], stride[2]};
+  return create(k, dd, nchan, dp, offset, length, values, patch_indices, nvalues);
+}
+
+int pp_sparse_destroy(pp_sparse_kernel* k) {
+  if (!k) return pp::abi_ok();
+  hipError_t e = hipFree(k->mem);
+  delete k;
+  if (e != hipSuccess) return hip_fail(e, "hipFree");
+  return pp::abi_ok();
+}
+
+int pp_vel2mom(const pp_sparse_kernel* k, float* u, const float* v, const int32_t* bnd, int accumulate_edges,
+               void* stream) {
+  return vel2mom_impl(k, u, v, bnd, accumulate_edges, static_cast<hipStream_t>(stream));
+}
+
+int pp_relax(const pp_sparse_kernel* k, float* v, const float* g, const float* h, int hcode, const int32_t* bnd,
+             int nit, void* stream) {
+  if (bnd && k)
+    for (int q = 0; q < 3 * k->nc; ++q)
+      if (bnd[q] < 0 || bnd[q] > 2) return fail(PP_EINVAL, "inappropriate boundary conditions (code %lld)", bnd[q]);
+  return relax_impl(k, v, g, h, hcode, bnd, nit, static_cast<hipStream_t>(stream));
+}
+
+// ---- Tier A --------------------------------------------------------------------
+void vel2mom(float* u, const float* v, const pp_usize* d, const int* offset, const int* length, const float* values,
+             const int* indices, const int* patch_indices, const pp_usize* dp, const int* bnd) {
+  (void)indices;  // interior offsets are derived from patch_indices and d
+  long long dd[3];
+  int nc = 0;
+  const long long nd = tierA_dims(d, dd, nc);
+  pp_sparse_kernel* k = nullptr;
+  int rc = tierA_kernel(&k, d, offset, length, values, patch_indices, dp);
+  if (rc == PP_OK && (!u || !v || !bnd)) rc = fail(PP_EINVAL, "null argument");
+  if (rc != PP_OK) {
+    if (nd > 0) nan_fill(u, nd * nc);
+    destroy_quiet(k);
+    return;
+  }
+  const size_t bytes = (size_t)nd * nc * sizeof(float);
+  DevBuf du, dv;
+  hipError_t e = hipMalloc(&du.p, bytes);
+  if (e == hipSuccess) e = hipMalloc(&dv.p, bytes);
+  if (e == hipSuccess) e = hipMemcpy(du.p, u, bytes, hipMemcpyHostToDevice);  // edges accumulate
+  if (e == hipSuccess) e = hipMemcpy(dv.p, v, bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    hip_fail(e, "vel2mom staging");
+  } else if ((rc = vel2mom_impl(k, static_cast<float*>(du.p), static_cast<const float*>(dv.p), bnd, 1, nullptr)) ==
+             PP_OK) {
+    if ((e = hipMemcpy(u, du.p, bytes, hipMemcpyDeviceToHost)) != hipSuccess) hip_fail(e, "vel2mom download");
+    else pp::abi_ok();
+  }
+  if (pp_last_status() != PP_OK) nan_fill(u, nd * nc);
+  destroy_quiet(k);
+}
+
+void relax(float* v, const pp_usize* d, const float* g, const float* h, const int* offset, const int* length,
+           const float* values, const int* indices, const int* patch_indices, const pp_usize* dp, const int* bnd) {
+  (void)indices;
+  long long dd[3];
+  int nc = 0;
+  const long long nd = tierA_dims(d, dd, nc);
+  // d[4] (sparse_operator_dev.cu:34-54): 0 no H, 2 full symmetric, any other value diagonal
+  const int hcode = (nd > 0) ? (d[4] == 0 ? 0 : d[4] == 2 ? 2 : 1) : 0;
+  const int nh = hcode == 0 ? 0 : hcode == 1 ? nc : nc * (nc + 1) / 2;
+  pp_sparse_kernel* k = nullptr;
+  int rc = tierA_kernel(&k, d, offset, length, values, patch_indices, dp);
+  if (rc == PP_OK && (!v || !g || !bnd || (hcode && !h))) rc = fail(PP_EINVAL, "null argument");
+  if (rc != PP_OK) {
+    if (nd > 0) nan_fill(v, nd * nc);
+    destroy_quiet(k);
+    return;
+  }
+  const size_t vb = (size_t)nd * nc * sizeof(float), hb = (size_t)nd * nh * sizeof(float);
+  DevBuf dv, dg, dh;
+  hipError_t e = hipMalloc(&dv.p, vb);
+  if (e == hipSuccess) e = hipMalloc(&dg.p, vb);
+  if (e == hipSuccess && hb) e = hipMalloc(&dh.p, hb);
+  if (e == hipSuccess) e = hipMemcpy(dv.p, v, vb, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dg.p, g, vb, hipMemcpyHostToDevice);
+  if (e == hipSuccess && hb) e = hipMemcpy(dh.p, h, hb, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    hip_fail(e, "relax staging");
+  } else if ((rc = relax_impl(k, static_cast<float*>(dv.p), static_cast<const float*>(dg.p),
+                              static_cast<const float*>(dh.p), hcode, bnd, 1, nullptr)) == PP_OK) {
+    if ((e = hipMemcpy(v, dv.p, vb, hipMemcpyDeviceToHost)) != hipSuccess) hip_fail(e, "relax download");
+    else pp::abi_ok();
+  }
+  if (pp_last_status() != PP_OK) nan_fill(v, nd * nc);
+  destroy_quiet(k);
+}
+
+}  // extern "C"

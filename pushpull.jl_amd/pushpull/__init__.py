@@ -11,12 +11,14 @@ from .ops import push_grad, pull_hess, pull_backward, pull_grad_backward, object
 from .affine import affine_pull, affine_push, adjust_affine
 from .compositions import compose, compose_, Exp
 from .multigrid import push_count, affine_push_count, invert_def_rhs, restrict, prolong
+from .regulariser import registration_operator, reduce2fit, sparsify, vel2mom, vel2mom_, relax_, SparseKernel
 from . import _lib
 
 __all__ = [
     "Settings", "id", "pull", "push", "pull_grad", "push_grad", "pull_hess",
     "pull_backward", "pull_grad_backward", "objective_grad", "affine_pull", "affine_push", "adjust_affine",
     "compose", "compose_", "Exp", "push_count", "affine_push_count", "invert_def_rhs", "restrict", "prolong",
+    "registration_operator", "reduce2fit", "sparsify", "vel2mom", "vel2mom_", "relax_", "SparseKernel",
 ]
 
 

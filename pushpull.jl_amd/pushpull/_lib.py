@@ -62,6 +62,13 @@ _SIGS = {
     "pp_affine_push_count": ([_P, _P, _P, _P, _P, ct.c_int, _P], ct.c_int),
     "pp_compose": ([_P, _P, _P, ct.POINTER(ct.c_longlong), ct.POINTER(ct.c_longlong), ct.c_int, _P], ct.c_int),
     "pp_exp": ([_P, _P, _P, ct.POINTER(ct.c_longlong), ct.c_int, ct.c_int, _P], ct.c_int),
+    "vel2mom": ([_P] * 10, None),
+    "relax": ([_P] * 11, None),
+    "pp_sparse_create": ([ct.POINTER(ct.c_void_p), ct.POINTER(ct.c_int64), ct.c_int, ct.POINTER(ct.c_int32), _P, _P, _P,
+                          _P, ct.c_int64], ct.c_int),
+    "pp_sparse_destroy": ([_P], ct.c_int),
+    "pp_vel2mom": ([_P, _P, _P, _P, ct.c_int, _P], ct.c_int),
+    "pp_relax": ([_P, _P, _P, _P, ct.c_int, _P, ct.c_int, _P], ct.c_int),
     "pp_malloc": ([ct.POINTER(ct.c_void_p), ct.c_size_t], ct.c_int),
     "pp_free": ([_P], ct.c_int),
     "pp_memcpy_h2d": ([_P, _P, ct.c_size_t, _P], ct.c_int),
