@@ -325,6 +325,78 @@ def bench_invert_def(torch, pp, reps=5, seed=7):
             "GB_s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
+REG_OPS = {"membrane": [1e-3, 1.0, 0.0, 0.0], "bending+divergence": [1e-3, 0.1, 0.01, 0.05]}
+
+
+def bench_regulariser(torch, pp, reps=5, seed=8, ops=tuple(REG_OPS)):
+    """The sparse regulariser at 256^3 x 3 channels: vel2mom (u = L v) and one Gauss-Seidel
+    sweep of (L + H) v = g with a full symmetric H (6 channels), for the multigrid default
+    (membrane, 3^3 stencil, src/multigrid.jl:171-176) and bending + divergence (5^3).
+    Algorithmic bytes: vel2mom reads v and writes u, 24 B/voxel; a sweep reads v, g, H and
+    writes v, 60 B/voxel. A sweep is one launch per colour of each region, as the
+    reference's sequential order requires (dp^3 colours)."""
+    from pushpull.regulariser import DEFAULT_BND
+
+    dev = torch.device("cuda:0")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    d = (N, N, N)
+    n = N ** 3
+    v = torch.randn((3,) + d, device=dev, generator=gen)
+    g = torch.randn((3,) + d, device=dev, generator=gen)
+    h = torch.rand((6,) + d, device=dev, generator=gen) + 1.0
+    h[3:] *= 0.1
+    u = torch.empty_like(v)
+    out = {"volume": [N, N, N], "channels": 3, "hessian_channels": 6}
+    for name in ops:
+        k = pp.sparsify(pp.registration_operator(np.float32([1, 1, 1]), np.float32(REG_OPS[name])), d)
+        pp.vel2mom_(u, v, k)  # uploads the kernel and warms up
+        ms_v = _time_ms(torch, lambda: pp.vel2mom_(u, v, k), reps, None)
+        w = v.clone()
+        pp.relax_(g, h, k, DEFAULT_BND, 1, w)
+        ms_r = _time_ms(torch, lambda: pp.relax_(g, h, k, DEFAULT_BND, 1, w), max(2, reps // 2), None)
+        gv, gr = 24 * n / (ms_v * 1e-3) / 1e9, 60 * n / (ms_r * 1e-3) / 1e9
+        out[name] = {"stencil": list(k.stride), "entries": int(k.length.sum()),
+                     "vel2mom_ms": round(ms_v, 4), "vel2mom_GB_s": round(gv, 1), "vel2mom_hbm_frac": round(gv / HBM_PEAK_GBS, 4),
+                     "relax_sweep_ms": round(ms_r, 4), "relax_GB_s": round(gr, 1),
+                     "relax_hbm_frac": round(gr / HBM_PEAK_GBS, 4), "relax_colours": int(np.prod(k.stride))}
+    return out
+
+
+def cpu_regulariser_baseline(res, budget_s=8.0):
+    """The reference's artifacts/C/sparse_operator.c (oracle/_ref, single thread) on a
+    cubic sample: vel2mom and one relax sweep per operator, scaled to voxels/s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_sparse as osp
+    import pushpull as pp
+
+    if not osp.ref_available():
+        return None
+    from pushpull.regulariser import DEFAULT_BND
+
+    rng = np.random.default_rng(3)
+    out = {"kind": "reference", "cores": 1}
+    for name, lam in REG_OPS.items():
+        m = 48
+        d = (m, m, m)
+        k = pp.sparsify(pp.registration_operator(np.float32([1, 1, 1]), np.float32(lam)), d)
+        v = rng.standard_normal((3,) + d).astype(np.float32)
+        h = (1.0 + rng.random((6,) + d)).astype(np.float32)
+        t0 = time.perf_counter()
+        osp.ref_vel2mom(v, k, DEFAULT_BND)
+        t1 = time.perf_counter()
+        osp.ref_relax(v, v, h, k, DEFAULT_BND, 1)
+        t2 = time.perf_counter()
+        vps_v, vps_r = m ** 3 / (t1 - t0), m ** 3 / (t2 - t1)
+        out[name] = {"sample": f"{m}^3 x 3 channels", "vel2mom_Mvoxel_s": round(vps_v / 1e6, 3),
+                     "relax_Mvoxel_s": round(vps_r / 1e6, 3)}
+        gpu = res.get("regulariser_256", {}).get(name)
+        if gpu:
+            out[name]["gpu_vel2mom_speedup"] = round(N ** 3 / (gpu["vel2mom_ms"] * 1e-3) / vps_v, 1)
+            out[name]["gpu_relax_speedup"] = round(N ** 3 / (gpu["relax_sweep_ms"] * 1e-3) / vps_r, 1)
+    return out
+
+
 def bench_matching(torch, pp, iters=50, alpha=0.02, seed=5, check=True):
     """8 subjects at 256^3, 4 channels, degree 3: `iters` iterations of objective and
     gradient (pull, residual, push, pullback) with phi <- phi - alpha * dL/dphi, all
@@ -420,6 +492,10 @@ def main():
                     help="skip the batched-uneven and matching-loop legs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run k_fill and k_axpy over 64 Mi floats (reference kernels for HBM counter scaling)")
+    ap.add_argument("--leg", choices=["regulariser"], default=None,
+                    help="run only this secondary leg and print its JSON (for profiling)")
+    ap.add_argument("--reg-op", choices=["all", *REG_OPS], default="all",
+                    help="operators of the regulariser leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -430,6 +506,11 @@ def main():
     import pushpull as pp
 
     from pushpull import _lib, ops
+
+    if args.leg == "regulariser":
+        ops_sel = list(REG_OPS) if args.reg_op == "all" else [args.reg_op]
+        print(json.dumps({"regulariser_256": bench_regulariser(torch, pp, ops=ops_sel)}))
+        return
 
     dev = torch.device("cuda:0")
     _progress("making inputs")
@@ -575,9 +656,15 @@ def main():
         _progress("invert_def pushes (push with count)")
         res["invert_def_push_count_256"] = bench_invert_def(torch, pp)
         torch.cuda.empty_cache()
+        _progress("sparse regulariser (vel2mom, relax)")
+        res["regulariser_256"] = bench_regulariser(torch, pp)
+        torch.cuda.empty_cache()
     if not args.no_cpu:
         _progress("cpu baseline")
         res["cpu_baseline"] = cpu_baseline(f, phi, delta)
+        if not args.no_extras:
+            _progress("cpu reference of the regulariser")
+            res["regulariser_cpu_reference"] = cpu_regulariser_baseline(res)
     print(json.dumps(res))
 
 

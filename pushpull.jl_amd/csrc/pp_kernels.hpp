@@ -28,6 +28,16 @@ enum Op : int {
 #define PP_GLOBAL
 #endif
 
+constexpr int kXcds = 8;
+
+// Round-robin dispatch puts block b on XCD b % 8; give XCD k a contiguous chunk of the
+// nb blocks, so that neighbouring blocks share that XCD's L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  const int per = nb / kXcds, rem = nb % kXcds;
+  const int x = b % kXcds, i = b / kXcds;
+  return (x < rem) ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
+}
+
 // Kernel arguments, passed by value (kernarg segment; no module globals, so launches
 // on different streams never share state).
 struct KArgs {

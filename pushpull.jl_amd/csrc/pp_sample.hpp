@@ -6,14 +6,6 @@
 
 namespace pp {
 
-constexpr int kXcds = 8;
-
-// Round-robin dispatch puts block b on XCD b % 8; give XCD k a contiguous chunk.
-__device__ __forceinline__ int xcd_block(int b, int nb) {
-  const int per = nb / kXcds, rem = nb % kXcds;
-  const int x = b % kXcds, i = b / kXcds;
-  return (x < rem) ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
-}
 
 // Source coordinate of output voxel (i,j,k) under a 0-based affine, in the reference
 // CPU order xj = (A3 j + A6 k) + A9, x = xj + A0 i (artifacts/C/pushpull.c:231-241),

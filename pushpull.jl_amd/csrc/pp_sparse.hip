@@ -13,6 +13,8 @@
 #define PP_KERNEL_SOURCE 1
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "pp_sparse.hpp"
 
 #pragma clang fp contract(off)
@@ -21,6 +23,17 @@ namespace pp {
 namespace {
 
 constexpr int kSpBlock = 256;
+constexpr size_t kSpCoopLds = 48 * 1024;  // LDS budget of the cooperative kernels
+
+// Launches of at most this many voxels run kSpG lanes per voxel: they are too small to
+// fill the GPU, and their time is one voxel's latency. Larger ones run a thread per voxel.
+long long coop_max_voxels() {
+  static const long long v = [] {
+    const char* e = getenv("PP_SPARSE_COOP_MAX");
+    return e ? atoll(e) : 32768LL;
+  }();
+  return v;
+}
 
 // Fold coordinate c into [0, d) under boundary code `code` (0 circulant, 1 reflect,
 // 2 reflect with sign change on the mirrored copies); flips `neg` for a sign change.
@@ -39,38 +52,66 @@ __device__ inline int fold_axis(int c, int d, int code, bool& neg) {
   return p - 1 - m;
 }
 
-typedef const PP_GLOBAL float* gfloat;
-
-// Value of the stencil entry (code, off) of input channel j (array vj, boundary codes bj)
-// around voxel (x, y, z), linear index m.
-template <bool EDGE>
-__device__ inline float tap(const SpArgs& a, gfloat vj, int code, int off, int x, int y, int z,
-                            long long m, const int* bj) {
-  if (!EDGE) return vj[m + off];
-  const int dx = (code & 255) - 16, dy = ((code >> 8) & 255) - 16, dz = ((code >> 16) & 255) - 16;
+// Value of entry t around voxel (x, y, z) (linear index m; mb = 4 m when the volume has
+// fewer than 2^30 voxels). Interior voxels read v[m + goff] (BIG = false: one scalar base
+// plus a 32-bit byte offset per lane); edge voxels fold each axis by the input channel's
+// boundary codes.
+template <bool EDGE, bool BIG>
+__device__ inline float fetch(const SpArgs& a, const SpEntry& t, int x, int y, int z, long long m, unsigned mb) {
+  if (!EDGE) {
+    if (BIG) return a.v[m + t.goff];
+    return *reinterpret_cast<const PP_GLOBAL float*>(reinterpret_cast<const PP_GLOBAL char*>(a.v + t.goff) + mb);
+  }
+  const int j = (t.code >> 29) & 7;
+  const int dx = (t.code & 255) - 16, dy = ((t.code >> 8) & 255) - 16, dz = ((t.code >> 16) & 255) - 16;
   bool neg = false;
-  const int ix = fold_axis(x + dx, a.d[0], bj[0], neg);
-  const int iy = fold_axis(y + dy, a.d[1], bj[1], neg);
-  const int iz = fold_axis(z + dz, a.d[2], bj[2], neg);
-  const float f = vj[ix + (long long)a.d[0] * (iy + (long long)a.d[1] * iz)];
+  const int ix = fold_axis(x + dx, a.d[0], a.bnd[3 * j], neg);
+  const int iy = fold_axis(y + dy, a.d[1], a.bnd[3 * j + 1], neg);
+  const int iz = fold_axis(z + dz, a.d[2], a.bnd[3 * j + 2], neg);
+  const float f = a.v[a.nd * j + ix + (long long)a.d[0] * (iy + (long long)a.d[1] * iz)];
   return neg ? -f : f;
 }
 
-// sp_conv (sparse_operator_dev.cu:12-20): s = sum_{e>0} w_e (f_e - f0), then
-// s += (w0 + sum of the others) * f0.
-template <bool EDGE>
-__device__ inline float list_conv(const SpArgs& a, int p, gfloat vj, int x, int y, int z,
-                                  long long m, const int* bj) {
-  const SpPair q = a.pairs[p];
-  const SpEntry e0 = a.ent[q.start];
-  const float f0 = tap<EDGE>(a, vj, e0.code, e0.off, x, y, z, m, bj);
-  float s = 0.0f;
-  for (int e = q.start + 1; e < q.start + q.len; ++e) {
-    const SpEntry t = a.ent[e];
-    s += t.w * (tap<EDGE>(a, vj, t.code, t.off, x, y, z, m, bj) - f0);
+// All lists of one voxel, in entry order: sp_conv (sparse_operator_dev.cu:12-20) of list
+// (i, j) is s = sum_{e>0} w_e (f_e - f0), then s += c0w * f0; it is added to acc[i]
+// (vel2mom, sparse_operator_dev.cu:175-181) or subtracted from it (relax, :107-115), for
+// j = 0, 1, ... in turn. The taps are fetched kSpBatch at a time (the entry index is
+// uniform across the wave, so a batch is kSpBatch independent loads in flight) and then
+// accumulated strictly in list order.
+constexpr int kSpBatch = 16;
+
+template <int NC, bool EDGE, bool BIG, bool SUB>
+__device__ inline void apply_lists(const SpArgs& a, float (&acc)[NC], int x, int y, int z, long long m) {
+  const unsigned mb = BIG ? 0u : (unsigned)m * 4u;
+  float f0 = 0.0f, s = 0.0f, c0w = 0.0f;
+  for (int e0 = 0; e0 < a.nent; e0 += kSpBatch) {
+    float f[kSpBatch];
+#pragma unroll
+    for (int b = 0; b < kSpBatch; ++b) {
+      const SpEntry t = a.ent[min(e0 + b, a.nent - 1)];
+      f[b] = fetch<EDGE, BIG>(a, t, x, y, z, m, mb);
+    }
+#pragma unroll
+    for (int b = 0; b < kSpBatch; ++b) {
+      if (e0 + b < a.nent) {
+        const SpEntry t = a.ent[e0 + b];
+        if (t.code & kSpFirst) {
+          f0 = f[b];
+          s = 0.0f;
+          c0w = t.w;
+        } else {
+          s += t.w * (f[b] - f0);
+        }
+        if (t.code & kSpLast) {
+          s += c0w * f0;
+          const int i = (t.code >> 26) & 7;
+#pragma unroll
+          for (int q = 0; q < NC; ++q)
+            if (q == i) acc[q] = SUB ? acc[q] - s : acc[q] + s;
+        }
+      }
+    }
   }
-  s += q.c0w * f0;
-  return s;
 }
 
 template <int NC>
@@ -107,56 +148,12 @@ __device__ inline void chol_solve(float (&A)[NC * NC], float (&b)[NC], float (&x
   }
 }
 
-template <int NC, bool EDGE, int OP>
-__global__ __launch_bounds__(kSpBlock) void k_sparse(SpArgs a, SpLattice l) {
-  const long long t = (long long)blockIdx.x * kSpBlock + threadIdx.x;
-  if (t >= l.n) return;
-  // lattice coordinates: q = t / cnt0 and iz = q / cnt1, exact after one correction
-  long long q = (long long)((double)t * l.rx);
-  if (q * l.cnt[0] > t) --q;
-  else if ((q + 1) * l.cnt[0] <= t) ++q;
-  const int ix = (int)(t - q * l.cnt[0]);
-  long long iz = (long long)((double)q * l.ry);
-  if (iz * l.cnt[1] > q) --iz;
-  else if ((iz + 1) * l.cnt[1] <= q) ++iz;
-  const int iy = (int)(q - iz * l.cnt[1]);
-  const int x = l.start[0] + l.step[0] * ix, y = l.start[1] + l.step[1] * iy, z = l.start[2] + l.step[2] * (int)iz;
-  if (l.split) {
-    const int key = (x >= a.d[0] - l.r[0] ? 1 : 0) | (y >= a.d[1] - l.r[1] ? 2 : 0);
-    if ((key & l.split) != l.key) return;
-  }
-  const long long m = x + (long long)a.d[0] * (y + (long long)a.d[1] * z);
+// relax1 / relax_padded1 after the stencil (sparse_operator_dev.cu:117-131): b -= H v,
+// A = the lists' centres (+ H), sol = A \ b by Cholesky. vc: v at the voxel.
+template <int NC>
+__device__ inline void relax_solve(const SpArgs& a, long long m, float (&b)[NC], const float (&vc)[NC],
+                                   float (&sol)[NC]) {
   const long long nd = a.nd;
-
-  if (OP == SP_VEL2MOM) {
-    float u[NC];
-#pragma unroll
-    for (int i = 0; i < NC; ++i) u[i] = (EDGE && a.accum) ? a.u[m + nd * i] : 0.0f;
-    // sparse_operator_dev.cu:164-204: channel i sums the lists (i, j) over j in order
-#pragma unroll
-    for (int j = 0; j < NC; ++j) {
-      gfloat vj = a.v + nd * j;
-#pragma unroll
-      for (int i = 0; i < NC; ++i) u[i] += list_conv<EDGE>(a, i + NC * j, vj, x, y, z, m, a.bnd + 3 * j);
-    }
-#pragma unroll
-    for (int i = 0; i < NC; ++i) a.u[m + nd * i] = u[i];
-    return;
-  }
-
-  // relax1 / relax_padded1 (sparse_operator_dev.cu:89-161)
-  float b[NC], vc[NC];
-#pragma unroll
-  for (int i = 0; i < NC; ++i) {
-    b[i] = a.g[m + nd * i];
-    vc[i] = a.v[m + nd * i];
-  }
-#pragma unroll
-  for (int j = 0; j < NC; ++j) {
-    gfloat vj = a.v + nd * j;
-#pragma unroll
-    for (int i = 0; i < NC; ++i) b[i] -= list_conv<EDGE>(a, i + NC * j, vj, x, y, z, m, a.bnd + 3 * j);
-  }
   float hv[NC * (NC + 1) / 2];
   if (a.hcode) {  // Hv (sparse_operator_dev.cu:32-54)
 #pragma unroll
@@ -195,26 +192,207 @@ __global__ __launch_bounds__(kSpBlock) void k_sparse(SpArgs a, SpLattice l) {
         for (int i = j + 1; i < NC; ++i, ++ii) A[i + j * NC] = A[j + i * NC] += hv[ii];
     }
   }
-  float sol[NC];
   chol_solve<NC>(A, b, sol);
+}
+
+// Lattice index t -> voxel (x, y, z); false when the voxel is outside this launch's phase.
+__device__ inline bool lattice_voxel(const SpArgs& a, const SpLattice& l, long long t, int& x, int& y, int& z) {
+  // q = t / cnt0 and iz = q / cnt1, exact after one correction
+  long long q = (long long)((double)t * l.rx);
+  if (q * l.cnt[0] > t) --q;
+  else if ((q + 1) * l.cnt[0] <= t) ++q;
+  const int ix = (int)(t - q * l.cnt[0]);
+  long long iz = (long long)((double)q * l.ry);
+  if (iz * l.cnt[1] > q) --iz;
+  else if ((iz + 1) * l.cnt[1] <= q) ++iz;
+  const int iy = (int)(q - iz * l.cnt[1]);
+  x = l.start[0] + l.step[0] * ix;
+  y = l.start[1] + l.step[1] * iy;
+  z = l.start[2] + l.step[2] * (int)iz;
+  if (l.split) {
+    const int key = (x >= a.d[0] - l.r[0] ? 1 : 0) | (y >= a.d[1] - l.r[1] ? 2 : 0);
+    if ((key & l.split) != l.key) return false;
+  }
+  return true;
+}
+
+// One thread per voxel.
+template <int NC, bool EDGE, bool BIG, int OP>
+__global__ __launch_bounds__(kSpBlock) void k_sparse(SpArgs a, SpLattice l) {
+  const long long t = (long long)xcd_block(blockIdx.x, gridDim.x) * kSpBlock + threadIdx.x;
+  if (t >= l.n) return;
+  int x, y, z;
+  if (!lattice_voxel(a, l, t, x, y, z)) return;
+  const long long m = x + (long long)a.d[0] * (y + (long long)a.d[1] * z);
+  const long long nd = a.nd;
+  if (OP == SP_VEL2MOM) {
+    float u[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) u[i] = (EDGE && a.accum) ? a.u[m + nd * i] : 0.0f;
+    apply_lists<NC, EDGE, BIG, false>(a, u, x, y, z, m);
+#pragma unroll
+    for (int i = 0; i < NC; ++i) a.u[m + nd * i] = u[i];
+    return;
+  }
+  float b[NC], vc[NC], sol[NC];
+#pragma unroll
+  for (int i = 0; i < NC; ++i) {
+    b[i] = a.g[m + nd * i];
+    vc[i] = a.v[m + nd * i];
+  }
+  apply_lists<NC, EDGE, BIG, true>(a, b, x, y, z, m);
+  relax_solve<NC>(a, m, b, vc, sol);
 #pragma unroll
   for (int i = 0; i < NC; ++i) a.v[m + nd * i] = vc[i] + sol[i];
 }
 
+// kSpG lanes per voxel: the lanes fetch the voxel's taps (entry e by lane e mod kSpG) into
+// LDS, replace each by its product w_e (f_e - f0), one lane per list then adds its
+// products in list order and finishes sp_conv, and lanes 0..NC-1 combine the lists of
+// their channel (j in order) and write it. Same operations and order as one thread per
+// voxel, with a voxel's memory latency spread over kSpG lanes. The kSpG lanes of a voxel
+// lie in one wave.
+constexpr int kSpG = 16;
+constexpr int kSpVox = kSpBlock / kSpG;
+constexpr int kSpLB = 8;  // entries per lane fetched together
+
+template <int NC, bool EDGE, bool BIG, int OP>
+__global__ __launch_bounds__(kSpBlock) void k_sparse_coop(SpArgs a, SpLattice l) {
+  extern __shared__ float sp_lds[];
+  const int g = threadIdx.x / kSpG, r = threadIdx.x % kSpG;
+  float* sf = sp_lds + g * a.nent;
+  float* cl = sp_lds + kSpVox * a.nent + g * NC * NC;
+  const long long t = (long long)xcd_block(blockIdx.x, gridDim.x) * kSpVox + g;
+  int x = 0, y = 0, z = 0;
+  const bool ok = t < l.n && lattice_voxel(a, l, t, x, y, z);
+  const long long m = x + (long long)a.d[0] * (y + (long long)a.d[1] * z);
+  const long long nd = a.nd;
+  const unsigned mb = BIG ? 0u : (unsigned)m * 4u;
+  float b[NC], vc[NC];
+  if (OP == SP_RELAX && ok && r < NC) {
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+      b[i] = a.g[m + nd * i];
+      vc[i] = a.v[m + nd * i];
+    }
+  }
+  // entries e0 + b kSpG, b < kSpLB, of this lane: all entry loads, then all tap loads
+  if (a.nent <= kSpG * kSpLB) {  // one batch per lane: entries and taps stay in registers
+    SpEntry en[kSpLB];
+    int s0[kSpLB];
+    float f[kSpLB];
+    if (ok) {
+#pragma unroll
+      for (int b = 0; b < kSpLB; ++b) {
+        en[b] = a.ent[min(r + b * kSpG, a.nent - 1)];
+        s0[b] = a.pairs[((en[b].code >> 26) & 7) + NC * ((en[b].code >> 29) & 7)].start;
+      }
+#pragma unroll
+      for (int b = 0; b < kSpLB; ++b) f[b] = fetch<EDGE, BIG>(a, en[b], x, y, z, m, mb);
+#pragma unroll
+      for (int b = 0; b < kSpLB; ++b)
+        if (r + b * kSpG < a.nent) sf[r + b * kSpG] = f[b];
+    }
+    __syncthreads();
+    if (ok) {
+#pragma unroll
+      for (int b = 0; b < kSpLB; ++b) {
+        const int e = r + b * kSpG;
+        if (e < a.nent && !(en[b].code & kSpFirst)) sf[e] = en[b].w * (f[b] - sf[s0[b]]);
+      }
+    }
+  } else {
+    if (ok)
+      for (int e0 = r; e0 < a.nent; e0 += kSpG * kSpLB) {
+        SpEntry en[kSpLB];
+        float f[kSpLB];
+#pragma unroll
+        for (int b = 0; b < kSpLB; ++b) en[b] = a.ent[min(e0 + b * kSpG, a.nent - 1)];
+#pragma unroll
+        for (int b = 0; b < kSpLB; ++b) f[b] = fetch<EDGE, BIG>(a, en[b], x, y, z, m, mb);
+#pragma unroll
+        for (int b = 0; b < kSpLB; ++b)
+          if (e0 + b * kSpG < a.nent) sf[e0 + b * kSpG] = f[b];
+      }
+    __syncthreads();
+    if (ok)
+      for (int e0 = r; e0 < a.nent; e0 += kSpG * kSpLB) {
+        SpEntry en[kSpLB];
+        int s0[kSpLB];
+#pragma unroll
+        for (int b = 0; b < kSpLB; ++b) {
+          en[b] = a.ent[min(e0 + b * kSpG, a.nent - 1)];
+          s0[b] = a.pairs[((en[b].code >> 26) & 7) + NC * ((en[b].code >> 29) & 7)].start;
+        }
+#pragma unroll
+        for (int b = 0; b < kSpLB; ++b) {
+          const int e = e0 + b * kSpG;
+          if (e < a.nent && !(en[b].code & kSpFirst)) sf[e] = en[b].w * (sf[e] - sf[s0[b]]);
+        }
+      }
+  }
+  __syncthreads();
+  if (ok)
+    for (int p = r; p < NC * NC; p += kSpG) {
+      const SpPair q = a.pairs[p];
+      float s = 0.0f;
+      for (int e = q.start + 1; e < q.start + q.len; ++e) s += sf[e];
+      s += q.c0w * sf[q.start];
+      cl[p] = s;
+    }
+  __syncthreads();
+  if (!ok || r >= NC) return;
+  if (OP == SP_VEL2MOM) {
+    float u = (EDGE && a.accum) ? a.u[m + nd * r] : 0.0f;
+#pragma unroll
+    for (int j = 0; j < NC; ++j) u += cl[r + NC * j];
+    a.u[m + nd * r] = u;
+    return;
+  }
+#pragma unroll
+  for (int j = 0; j < NC; ++j)
+#pragma unroll
+    for (int i = 0; i < NC; ++i) b[i] -= cl[i + NC * j];
+  float sol[NC];
+  relax_solve<NC>(a, m, b, vc, sol);
+  float sr = sol[0], vr = vc[0];
+#pragma unroll
+  for (int i = 1; i < NC; ++i)
+    if (r == i) {
+      sr = sol[i];
+      vr = vc[i];
+    }
+  a.v[m + nd * r] = vr + sr;
+}
+
+template <int NC, int OP>
+void launch_op(const SpArgs& a, const SpLattice& l, bool edge, hipStream_t s) {
+  const size_t lds = (size_t)kSpVox * (a.nent + NC * NC) * sizeof(float);
+  if (lds <= kSpCoopLds && l.n <= coop_max_voxels()) {
+    const dim3 grid((unsigned)((l.n + kSpVox - 1) / kSpVox));
+    if (edge)
+      hipLaunchKernelGGL((k_sparse_coop<NC, true, true, OP>), grid, dim3(kSpBlock), lds, s, a, l);
+    else if (a.nd >= (1LL << 30))
+      hipLaunchKernelGGL((k_sparse_coop<NC, false, true, OP>), grid, dim3(kSpBlock), lds, s, a, l);
+    else
+      hipLaunchKernelGGL((k_sparse_coop<NC, false, false, OP>), grid, dim3(kSpBlock), lds, s, a, l);
+    return;
+  }
+  const dim3 grid((unsigned)((l.n + kSpBlock - 1) / kSpBlock));
+  if (edge)
+    hipLaunchKernelGGL((k_sparse<NC, true, true, OP>), grid, dim3(kSpBlock), 0, s, a, l);
+  else if (a.nd >= (1LL << 30))
+    hipLaunchKernelGGL((k_sparse<NC, false, true, OP>), grid, dim3(kSpBlock), 0, s, a, l);
+  else
+    hipLaunchKernelGGL((k_sparse<NC, false, false, OP>), grid, dim3(kSpBlock), 0, s, a, l);
+}
+
 template <int NC>
 hipError_t launch_nc(SpOp op, const SpArgs& a, const SpLattice& l, bool edge, hipStream_t s) {
-  const dim3 grid((unsigned)((l.n + kSpBlock - 1) / kSpBlock));
-  if (op == SP_VEL2MOM) {
-    if (edge)
-      hipLaunchKernelGGL((k_sparse<NC, true, SP_VEL2MOM>), grid, dim3(kSpBlock), 0, s, a, l);
-    else
-      hipLaunchKernelGGL((k_sparse<NC, false, SP_VEL2MOM>), grid, dim3(kSpBlock), 0, s, a, l);
-  } else {
-    if (edge)
-      hipLaunchKernelGGL((k_sparse<NC, true, SP_RELAX>), grid, dim3(kSpBlock), 0, s, a, l);
-    else
-      hipLaunchKernelGGL((k_sparse<NC, false, SP_RELAX>), grid, dim3(kSpBlock), 0, s, a, l);
-  }
+  if (op == SP_VEL2MOM)
+    launch_op<NC, SP_VEL2MOM>(a, l, edge, s);
+  else
+    launch_op<NC, SP_RELAX>(a, l, edge, s);
   return hipGetLastError();
 }
 

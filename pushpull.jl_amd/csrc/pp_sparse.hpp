@@ -11,25 +11,31 @@ namespace pp {
 constexpr int kSpMaxChan = 8;  // MAXN of artifacts/C/sparse_operator_dev.cu:8
 constexpr int kSpMaxBox = 5;   // largest stencil box side (src/operator_sparse.jl:64)
 
-// One (output i, input j) list: entries [start, start + len) of the entry table. Entry
-// `start` is the list's first position (its value is read as f0); w0 is its weight and
-// c0w = w0 + (sum of the list's other weights), rounded as one float add.
+// One (output i, input j) list. w0 is the weight of its first entry (the centre, which
+// make_A reads, sparse_operator_dev.cu:63) and c0w = w0 + (sum of the list's other
+// weights), rounded as one float add (sp_conv's last term, sparse_operator_dev.cu:18).
 struct SpPair {
-  int start, len;
   float w0, c0w;
+  int start, len;  // the list's entries in the entry table
 };
-// One stencil position: off is the linear offset from the centre voxel inside the volume
-// (interior voxels); code packs the box offsets (dx, dy, dz) as bytes biased by 16.
+// The entries of all lists, pair (i, j) = (output, input) at p = i + nc * j, in increasing
+// p: the order in which vel2mom and relax consume them (j outer, i inner). goff = nd * j +
+// the position's linear offset from the centre voxel (interior voxels read v[m + goff]);
+// code packs the box offsets (dx, dy, dz) as bytes biased by 16, then the flags below, the
+// output channel i (bits 26-28) and the input channel j (bits 29-31). A list's first entry
+// is the value read as f0 and carries c0w in w; the others carry their weights.
 struct SpEntry {
-  int off;
-  int code;
+  long long goff;
   float w;
-  int pad;
+  int code;
 };
+constexpr int kSpFirst = 1 << 24;
+constexpr int kSpLast = 1 << 25;
 
 struct SpArgs {
   const PP_GLOBAL SpPair* pairs;  // nc * nc, pair (i, j) at i + nc * j
-  const PP_GLOBAL SpEntry* ent;
+  const PP_GLOBAL SpEntry* ent;   // nent entries
+  int nent;
   PP_GLOBAL float* v;             // vel2mom: input; relax: the estimate, updated in place
   PP_GLOBAL float* u;             // vel2mom: output
   const PP_GLOBAL float* g;       // relax: right-hand side (nc channels)

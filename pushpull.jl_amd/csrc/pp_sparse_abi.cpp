@@ -20,6 +20,7 @@ struct pp_sparse_kernel {
   void* mem;  // one allocation: pairs, then entries
   pp::SpPair* pairs;
   pp::SpEntry* ent;
+  int nent;
 };
 
 namespace {
@@ -57,25 +58,28 @@ int create(pp_sparse_kernel** out, const long long d[3], int nc, const int dp[3]
   }
   if (nvox > kMaxVox) return fail(PP_ERANGE, "volume larger than 2^31-1 voxels");
   const int nbox = dp[0] * dp[1] * dp[2];
+  const long long nd = nvox;
   std::vector<pp::SpPair> pairs(nc * nc);
   std::vector<pp::SpEntry> ent;
-  for (int p = 0; p < nc * nc; ++p) {
+  for (int p = 0; p < nc * nc; ++p) {  // p = i + nc * j: j outer, i inner
+    const int i = p % nc, j = p / nc;
     const long long o = offset[p], n = length[p];
     if (n < 1 || o < 1 || o + n > nvalues)
       return fail(PP_EINVAL, "list %lld: offset %lld / length %lld outside the value array", p, o, n);
-    pairs[p].start = (int)ent.size();
-    pairs[p].len = (int)n;
     pairs[p].w0 = values[o];
     pairs[p].c0w = values[o] + values[o - 1];
+    pairs[p].start = (int)ent.size();
+    pairs[p].len = (int)n;
     for (long long e = o; e < o + n; ++e) {
       const int t = patch[e];
       if (t < 0 || t >= nbox) return fail(PP_EINVAL, "list %lld: patch index %lld outside the stencil box", p, t);
       const int dx = t % dp[0] - dp[0] / 2, dy = (t / dp[0]) % dp[1] - dp[1] / 2, dz = t / (dp[0] * dp[1]) - dp[2] / 2;
       pp::SpEntry s;
-      s.off = (int)(dx + d[0] * (dy + d[1] * (long long)dz));
-      s.code = (dx + 16) | ((dy + 16) << 8) | ((dz + 16) << 16);
-      s.w = values[e];
-      s.pad = 0;
+      s.goff = nd * j + dx + d[0] * (dy + d[1] * (long long)dz);
+      s.code = (dx + 16) | ((dy + 16) << 8) | ((dz + 16) << 16) | (i << 26) | (j << 29);
+      if (e == o) s.code |= pp::kSpFirst;
+      if (e == o + n - 1) s.code |= pp::kSpLast;
+      s.w = (e == o) ? pairs[p].c0w : values[e];
       ent.push_back(s);
     }
   }
@@ -102,6 +106,7 @@ int create(pp_sparse_kernel** out, const long long d[3], int nc, const int dp[3]
   k->mem = mem;
   k->pairs = reinterpret_cast<pp::SpPair*>(base);
   k->ent = reinterpret_cast<pp::SpEntry*>(base + pa);
+  k->nent = (int)ent.size();
   *out = k;
   return pp::abi_ok();
 }
@@ -120,6 +125,7 @@ pp::SpArgs base_args(const pp_sparse_kernel* k, const int* bnd) {
   std::memset(&a, 0, sizeof a);
   a.pairs = k->pairs;
   a.ent = k->ent;
+  a.nent = k->nent;
   a.nd = (long long)k->d[0] * k->d[1] * k->d[2];
   for (int q = 0; q < 3; ++q) a.d[q] = k->d[q];
   a.nc = k->nc;
@@ -198,7 +204,15 @@ int relax_sweep(const pp_sparse_kernel* k, const pp::SpArgs& a, hipStream_t s) {
   for (int q = 0; q < 2; ++q)
     for (int c = 0; c < k->nc; ++c)
       if (a.bnd[3 * c + q] == 0 && k->dp[q] > 1 && k->d[q] % k->dp[q] != 0) wrap |= 1 << q;
+  // The slabs at the two x faces (regions 2 and 3) never read each other's voxels when no
+  // channel folds x circulantly and d_x >= 3 r: then colour k of both runs as one launch
+  // (the sweep's order between them cannot change a value).
+  const int rx = (k->dp[0] - 1) / 2;
+  bool xcirc = false;
+  for (int c = 0; c < k->nc; ++c) xcirc |= a.bnd[3 * c] == 0;
+  const bool merge_x = rx > 0 && !xcirc && k->d[0] >= 3 * rx && b[2].hi[0] - b[2].lo[0] == b[3].hi[0] - b[3].lo[0];
   for (int i = 0; i < 7; ++i) {
+    if (i == 3 && merge_x) continue;
     const Box& r = b[i];
     int split = 0;
     for (int q = 0; q < 2; ++q)
@@ -208,6 +222,11 @@ int relax_sweep(const pp_sparse_kernel* k, const pp::SpArgs& a, hipStream_t s) {
       for (c[1] = r.lo[1]; c[1] < std::min(r.hi[1], r.lo[1] + k->dp[1]); ++c[1])
         for (c[0] = r.lo[0]; c[0] < std::min(r.hi[0], r.lo[0] + k->dp[0]); ++c[0]) {
           pp::SpLattice l = lattice(k, c, k->dp, r.hi);
+          if (i == 2 && merge_x) {  // x in {c0, c0 + e0}: this colour of both x slabs
+            int step[3] = {b[3].lo[0], k->dp[1], k->dp[2]};
+            int hi[3] = {k->d[0], r.hi[1], r.hi[2]};
+            l = lattice(k, c, step, hi);
+          }
           if (l.n == 0) continue;
           l.split = split;
           for (int key = 0; key < 4; ++key) {
