@@ -331,7 +331,7 @@ REG_OPS = {"membrane": [1e-3, 1.0, 0.0, 0.0], "bending+divergence": [1e-3, 0.1, 
 def bench_regulariser(torch, pp, reps=5, seed=8, ops=tuple(REG_OPS)):
     """The sparse regulariser at 256^3 x 3 channels: vel2mom (u = L v) and one Gauss-Seidel
     sweep of (L + H) v = g with a full symmetric H (6 channels), for the multigrid default
-    (membrane, 3^3 stencil, src/multigrid.jl:171-176) and bending + divergence (5^3).
+    (membrane, 3^3 stencil, src/multigrid.jl:175-181) and bending + divergence (5^3).
     Algorithmic bytes: vel2mom reads v and writes u, 24 B/voxel; a sweep reads v, g, H and
     writes v, 60 B/voxel. A sweep is one launch per colour of each region, as the
     reference's sequential order requires (dp^3 colours)."""

@@ -75,8 +75,8 @@ void push_affine(float *f0, const float *f1, const pp_usize *d0, pp_usize n1, co
                  const float *Aff, const int *bnd, const pp_usize *dp, int ext);
 
 /* Sparse regulariser (artifacts/C/sparse_operator.c, built by the reference as its own
- * library, `oplib` of src/operator_sparse.jl). d = (X, Y, Z, nchan, hcode); the stencil
- * is the `sparsify` tuple of src/operator.jl:206-295 (offset, length: nchan x nchan,
+ * library, `oplib`, src/pushpull_init.jl:5). d = (X, Y, Z, nchan, hcode); the stencil
+ * is the `sparsify` tuple of src/operator.jl:206-296 (offset, length: nchan x nchan,
  * column-major; values, patch_indices: the lists). `indices` is accepted and not read:
  * interior offsets are derived from patch_indices and d (sparsify writes both from the
  * same positions). Results are bit-identical to the reference's CPU code. */
@@ -191,7 +191,7 @@ int pp_compose(float *psi, const float *phi, const float *theta, const long long
 int pp_exp(float *out, const float *v, float *work, const long long dim[3], int nbatch, int nsq, void *stream);
 
 /* ---- sparse regulariser (src/operator_sparse.jl, src/operator.jl) ------------
- * A stencil L in the `sparsify` list format (src/operator.jl:206-295) is uploaded once
+ * A stencil L in the `sparsify` list format (src/operator.jl:206-296) is uploaded once
  * for volumes of size d; the calls then take device arrays (X, Y, Z, nchan), column-major,
  * and a host array bnd of 3*nchan boundary codes (channel c's triple at 3c). nchan <= 8,
  * stride odd and <= 5 per axis. Results are bit-identical to artifacts/C/sparse_operator.c.
@@ -216,7 +216,7 @@ int pp_vel2mom(const pp_sparse_kernel *kernel, float *u, const float *v, const i
 /* nit Gauss-Seidel sweeps of (L + H) v = g, updating v in place (src/operator_sparse.jl:43-62).
  * hcode 0: no H (h may be NULL); 1: h holds nchan diagonal channels; 2: nchan*(nchan+1)/2
  * channels, the diagonal first and then the pairs (j, i > j) in order. Boundary codes must
- * be 0, 1 or 2 (src/operator_sparse.jl:79-81). Each sweep runs the reference's colour
+ * be 0, 1 or 2 (src/operator_sparse.jl:82-84). Each sweep runs the reference's colour
  * order as one launch per colour of each of its seven regions. */
 int pp_relax(const pp_sparse_kernel *kernel, float *v, const float *g, const float *h, int hcode,
              const int32_t *bnd, int nit, void *stream);

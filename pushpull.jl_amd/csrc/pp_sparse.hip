@@ -117,7 +117,7 @@ __device__ inline void apply_lists(const SpArgs& a, float (&acc)[NC], int x, int
 template <int NC>
 __device__ inline void chol_solve(float (&A)[NC * NC], float (&b)[NC], float (&x)[NC]) {
   float p[NC];
-  // choldcf (artifacts/C/chol.cu:11-30)
+  // choldcf (artifacts/C/chol.cu:9-30)
 #pragma unroll
   for (int i = 0; i < NC; ++i) {
 #pragma unroll
@@ -131,7 +131,7 @@ __device__ inline void chol_solve(float (&A)[NC * NC], float (&b)[NC], float (&x
         A[j * NC + i] = t / p[i];
     }
   }
-  // chollsf (chol.cu:41-59)
+  // chollsf (chol.cu:40-59)
 #pragma unroll
   for (int i = 0; i < NC; ++i) {
     float t = b[i];

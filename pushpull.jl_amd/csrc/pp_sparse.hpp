@@ -1,6 +1,6 @@
 // pp_sparse.hpp -- host-side view of the sparse-regulariser kernels (internal to
 // libpushpull_hip): u = L v (vel2mom) and the Gauss-Seidel update of (L + H) v = g (relax)
-// for a sparse stencil L in the list format of src/operator.jl:206-295 (`sparsify`).
+// for a sparse stencil L in the list format of src/operator.jl:206-296 (`sparsify`).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -9,7 +9,7 @@
 namespace pp {
 
 constexpr int kSpMaxChan = 8;  // MAXN of artifacts/C/sparse_operator_dev.cu:8
-constexpr int kSpMaxBox = 5;   // largest stencil box side (src/operator_sparse.jl:64)
+constexpr int kSpMaxBox = 5;   // largest stencil box side (src/operator_sparse.jl:66,103-104)
 
 // One (output i, input j) list. w0 is the weight of its first entry (the centre, which
 // make_A reads, sparse_operator_dev.cu:63) and c0w = w0 + (sum of the list's other

@@ -40,7 +40,7 @@ int hip_fail(hipError_t e, const char* where) {
   return pp::abi_fail(PP_EDEVICE, buf);
 }
 
-// Checks the list arrays (the `sparsify` tuple, src/operator.jl:206-295) and builds the
+// Checks the list arrays (the `sparsify` tuple, src/operator.jl:206-296) and builds the
 // device tables: for each pair its entries in list order, the first one being the value
 // read as f0 (sparse_operator_dev.cu:15).
 int create(pp_sparse_kernel** out, const long long d[3], int nc, const int dp[3], const int* offset, const int* length,

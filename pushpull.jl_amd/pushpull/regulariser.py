@@ -2,10 +2,10 @@
 
 Operator construction runs on the host in NumPy, as the reference's does in Julia:
 
-* ``registration_operator(vx, λ, vx0=vx)`` (operator.jl:79-141): the dense stencil L of
+* ``registration_operator(vx, λ, vx0=vx)`` (operator.jl:58-112): the dense stencil L of
   the absolute, membrane, bending and divergence penalties.
-* ``reduce2fit(L, d, bnd)`` (operator.jl:151-196): collapses L along axes of size 1.
-* ``sparsify(L, d, nd=3)`` (operator.jl:206-295): the list format of the kernels.
+* ``reduce2fit(L, d, bnd)`` (operator.jl:120-159): collapses L along axes of size 1.
+* ``sparsify(L, d, nd=3)`` (operator.jl:206-296): the list format of the kernels.
 
 The two operators run on the GPU through libpushpull_hip:
 
@@ -44,7 +44,7 @@ except ImportError:  # pragma: no cover
     torch = None
 
 DEFAULT_BND = ((2, 1, 1), (1, 2, 1), (1, 1, 2))
-MAXD = 5  # largest stencil side (operator_sparse.jl:64)
+MAXD = 5  # largest stencil side (operator_sparse.jl:66,103-104)
 MAXN = 8  # most channels (sparse_operator_dev.cu:8)
 
 
@@ -69,12 +69,12 @@ def _pow4(x, T):
 
 
 def registration_operator(vx, lam, vx0=None) -> np.ndarray:
-    """Dense differential operator of registration (src/operator.jl:79-141).
+    """Dense differential operator of registration (src/operator.jl:58-112).
 
     ``lam`` = (absolute, membrane, bending, divergence) penalties. Returns L as
     ``(D, K, K, K)`` FP32 with K = 1, 3 or 5 and D = 3 (6 when lam[3] != 0, the last three
     channels being the (x,y), (x,z), (y,z) cross terms). The arithmetic runs in FP32 when
-    vx, lam and vx0 are all FP32 (as multigrid passes them, src/multigrid.jl:173-174) and
+    vx, lam and vx0 are all FP32 (as multigrid passes them, src/multigrid.jl:175-178) and
     in FP64 otherwise (exact for integer arguments, as in Julia).
     """
     _req(len(vx) == 3, "Wrong sized vx.")
@@ -131,7 +131,7 @@ def registration_operator(vx, lam, vx0=None) -> np.ndarray:
 
 
 def reduce2fit(L, d, bnd=DEFAULT_BND) -> np.ndarray:
-    """Collapse L along every axis where the volume has size 1 (src/operator.jl:151-196):
+    """Collapse L along every axis where the volume has size 1 (src/operator.jl:120-159):
     the stencil is summed along that axis, with alternating signs for channels whose
     boundary code there is 2. ``d`` is ``(Z, Y, X)``."""
     L = np.asarray(L)
@@ -157,7 +157,7 @@ def reduce2fit(L, d, bnd=DEFAULT_BND) -> np.ndarray:
 
 @dataclass(eq=False)
 class SparseKernel:
-    """The `sparsify` tuple (src/operator.jl:167-205) in this package's array order.
+    """The `sparsify` tuple (src/operator.jl:162-205) in this package's array order.
 
     ``offset[j, i]`` / ``length[j, i]``: the list of output channel i and input channel j
     (Julia ``offset[i, j]``); its entries are ``values[offset : offset + length]`` and
@@ -202,7 +202,7 @@ def _hptr(a: np.ndarray):
 
 
 def sparsify(L, d, nd: int = 3) -> SparseKernel:
-    """List form of the dense stencil L for volumes of size d (src/operator.jl:206-295)."""
+    """List form of the dense stencil L for volumes of size d (src/operator.jl:206-296)."""
     L = np.asarray(L)
     D, kz, ky, kx = L.shape
     dp = (kx, ky, kz)
@@ -250,7 +250,7 @@ def sparsify(L, d, nd: int = 3) -> SparseKernel:
             lists[c1, c2].append((np.float32(0.0), image_offset(*cv, c2), patch_offset(*cv)))
             lists[c2, c1].append((np.float32(0.0), image_offset(*cv, c1), patch_offset(*cv)))
 
-    # rearrange (operator.jl:210-224): column-major over (i, j), a sum slot before each list
+    # rearrange (operator.jl:211-224): column-major over (i, j), a sum slot before each list
     total = sum(len(v) for v in lists.values()) + nd * nd
     values = np.zeros(total, dtype=np.float32)
     indices = np.zeros(total, dtype=np.int32)
@@ -334,7 +334,7 @@ def vel2mom(v, kernel: SparseKernel, bnd=DEFAULT_BND):
 
 
 def _hcode(g, h, nc):
-    """The Hessian layout (src/operator_sparse.jl:92-104)."""
+    """The Hessian layout (src/operator_sparse.jl:90-101)."""
     if h is None or (h.numel() if _is_dev(h) else np.asarray(h).size) == 0:
         return 0
     _req(h.ndim == g.ndim, "incompatible dimensionality")
