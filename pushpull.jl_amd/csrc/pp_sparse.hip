@@ -365,6 +365,148 @@ __global__ __launch_bounds__(kSpBlock) void k_sparse_coop(SpArgs a, SpLattice l)
   a.v[m + nd * r] = vr + sr;
 }
 
+// vel2mom over the whole volume, LDS-tiled: a workgroup takes a tile of kVTX x kVTY x kVTZ
+// voxels, stages each input channel's tile plus an R-voxel halo in LDS -- every staged
+// position folded and signed by that channel's boundary codes, as the reference's
+// get_patch does for its edge voxels (and an identity inside) -- and each thread runs kVPT
+// voxels (one x, two y, eight z) through the lists of that channel entry by entry, so an
+// entry's scalar decoding is shared by kVPT voxels. Per voxel the operations and their
+// order are those of the one-thread kernels; with accum, voxels outside the interior block
+// [l.start, l.start + l.cnt) start from the u they are given.
+constexpr int kVTX = 64, kVTY = 8, kVTZ = 8;
+constexpr int kVPT = kVTY * kVTZ / (kSpBlock / kVTX);
+
+template <int NC, int R>
+__global__ __launch_bounds__(kSpBlock) void k_vel2mom_tile(SpArgs a, SpLattice l) {
+  constexpr int BX = kVTX + 2 * R, BY = kVTY + 2 * R, BZ = kVTZ + 2 * R, NB = BX * BY * BZ;
+  __shared__ float box[NB];
+  __shared__ int fold[BX + BY + BZ];  // folded coordinate * 2 + sign bit, x then y then z
+  const int ntx = (a.d[0] + kVTX - 1) / kVTX, nty = (a.d[1] + kVTY - 1) / kVTY;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int tx = b % ntx, ty = (b / ntx) % nty, tz = b / (ntx * nty);
+  const int x0 = tx * kVTX, y0 = ty * kVTY, z0 = tz * kVTZ;
+  const int lx = threadIdx.x % kVTX, lg = threadIdx.x / kVTX;  // lg: y rows 2 lg, 2 lg + 1
+  const long long nd = a.nd;
+  const int x = x0 + lx;
+  float u[kVPT][NC];
+#pragma unroll
+  for (int v = 0; v < kVPT; ++v) {
+    const int y = y0 + 2 * lg + v % 2, z = z0 + v / 2;
+    const bool edge = x < l.start[0] || x >= l.start[0] + l.cnt[0] || y < l.start[1] || y >= l.start[1] + l.cnt[1] ||
+                      z < l.start[2] || z >= l.start[2] + l.cnt[2];
+    const bool in = x < a.d[0] && y < a.d[1] && z < a.d[2];
+    const long long m = x + (long long)a.d[0] * (y + (long long)a.d[1] * z);
+#pragma unroll
+    for (int i = 0; i < NC; ++i) u[v][i] = (a.accum && edge && in) ? a.u[m + nd * i] : 0.0f;
+  }
+  const int base = lx + BX * 2 * lg;  // box corner of voxel 0 of this thread
+  // the box lies inside the volume: every fold is the identity
+  const bool inside = x0 >= R && y0 >= R && z0 >= R && x0 + BX - R <= a.d[0] && y0 + BY - R <= a.d[1] &&
+                      z0 + BZ - R <= a.d[2];
+  constexpr int kStage = 16, kIter = (NB + kSpBlock - 1) / kSpBlock;
+  for (int j = 0; j < NC; ++j) {
+    if (j) __syncthreads();
+    const PP_GLOBAL float* vj = a.v + nd * j;
+    if (!inside) {  // fold tables of this channel's codes for the box's x, y and z ranges
+      for (int t = threadIdx.x; t < BX + BY + BZ; t += kSpBlock) {
+        const int ax = t < BX ? 0 : (t < BX + BY ? 1 : 2);
+        const int c = ax == 0 ? x0 - R + t : (ax == 1 ? y0 - R + (t - BX) : z0 - R + (t - BX - BY));
+        bool neg = false;
+        const int g = fold_axis(c, a.d[ax], a.bnd[3 * j + ax], neg);
+        fold[t] = 2 * g + (neg ? 1 : 0);
+      }
+      __syncthreads();
+    }
+    // staging: kStage loads in flight per thread, then their LDS stores
+    if (inside) {
+      const PP_GLOBAL float* o = vj + (x0 - R) + (long long)a.d[0] * ((y0 - R) + (long long)a.d[1] * (z0 - R));
+#pragma unroll 1
+      for (int it = 0; it < kIter; it += kStage) {
+        float t[kStage];
+#pragma unroll
+        for (int q = 0; q < kStage; ++q) {
+          const int e = min((it + q) * kSpBlock + (int)threadIdx.x, NB - 1);
+          const int bx = e % BX, by = (e / BX) % BY, bz = e / (BX * BY);
+          t[q] = o[bx + (long long)a.d[0] * (by + (long long)a.d[1] * bz)];
+        }
+#pragma unroll
+        for (int q = 0; q < kStage; ++q) {
+          const int e = (it + q) * kSpBlock + (int)threadIdx.x;
+          if (it + q < kIter && e < NB) box[e] = t[q];
+        }
+      }
+    } else {
+#pragma unroll 1
+      for (int it = 0; it < kIter; it += kStage) {
+        float t[kStage];
+#pragma unroll
+        for (int q = 0; q < kStage; ++q) {
+          const int e = min((it + q) * kSpBlock + (int)threadIdx.x, NB - 1);
+          const int bx = e % BX, by = (e / BX) % BY, bz = e / (BX * BY);
+          const int fx = fold[bx], fy = fold[BX + by], fz = fold[BX + BY + bz];
+          const float f = vj[(fx >> 1) + (long long)a.d[0] * ((fy >> 1) + (long long)a.d[1] * (fz >> 1))];
+          t[q] = ((fx ^ fy ^ fz) & 1) ? -f : f;
+        }
+#pragma unroll
+        for (int q = 0; q < kStage; ++q) {
+          const int e = (it + q) * kSpBlock + (int)threadIdx.x;
+          if (it + q < kIter && e < NB) box[e] = t[q];
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int i = 0; i < NC; ++i) {
+      const SpPair q = a.pairs[i + NC * j];
+      float f0[kVPT], sv[kVPT];
+      {
+        const int c = a.ent[q.start].code;
+        const int o = ((c & 255) - 16 + R) + BX * ((((c >> 8) & 255) - 16 + R) + BY * (((c >> 16) & 255) - 16 + R));
+#pragma unroll
+        for (int v = 0; v < kVPT; ++v) {
+          f0[v] = box[base + o + BX * ((v % 2) + BY * (v / 2))];
+          sv[v] = 0.0f;
+        }
+      }
+#pragma unroll 2
+      for (int e = q.start + 1; e < q.start + q.len; ++e) {
+        const SpEntry t = a.ent[e];
+        const int c = t.code;
+        const int o = ((c & 255) - 16 + R) + BX * ((((c >> 8) & 255) - 16 + R) + BY * (((c >> 16) & 255) - 16 + R));
+#pragma unroll
+        for (int v = 0; v < kVPT; ++v) sv[v] += t.w * (box[base + o + BX * ((v % 2) + BY * (v / 2))] - f0[v]);
+      }
+#pragma unroll
+      for (int v = 0; v < kVPT; ++v) {
+        sv[v] += q.c0w * f0[v];
+#pragma unroll
+        for (int ii = 0; ii < NC; ++ii)
+          if (ii == i) u[v][ii] += sv[v];
+      }
+    }
+  }
+  if (x >= a.d[0]) return;
+#pragma unroll
+  for (int v = 0; v < kVPT; ++v) {
+    const int y = y0 + 2 * lg + v % 2, z = z0 + v / 2;
+    if (y < a.d[1] && z < a.d[2]) {
+      const long long m = x + (long long)a.d[0] * (y + (long long)a.d[1] * z);
+#pragma unroll
+      for (int i = 0; i < NC; ++i) a.u[m + nd * i] = u[v][i];
+    }
+  }
+}
+
+template <int NC>
+void launch_vtile(const SpArgs& a, const SpLattice& l, int r, hipStream_t s) {
+  const long long nt = (long long)((a.d[0] + kVTX - 1) / kVTX) * ((a.d[1] + kVTY - 1) / kVTY) *
+                       ((a.d[2] + kVTZ - 1) / kVTZ);
+  if (r == 1)
+    hipLaunchKernelGGL((k_vel2mom_tile<NC, 1>), dim3((unsigned)nt), dim3(kSpBlock), 0, s, a, l);
+  else
+    hipLaunchKernelGGL((k_vel2mom_tile<NC, 2>), dim3((unsigned)nt), dim3(kSpBlock), 0, s, a, l);
+}
+
 template <int NC, int OP>
 void launch_op(const SpArgs& a, const SpLattice& l, bool edge, hipStream_t s) {
   const size_t lds = (size_t)kSpVox * (a.nent + NC * NC) * sizeof(float);
@@ -397,6 +539,18 @@ hipError_t launch_nc(SpOp op, const SpArgs& a, const SpLattice& l, bool edge, hi
 }
 
 }  // namespace
+
+hipError_t launch_vel2mom_tiled(const SpArgs& a, const SpLattice& interior, int r, hipStream_t s) {
+  if (r < 1 || r > 2) return hipErrorNotSupported;
+  const SpLattice& l = interior;
+  switch (a.nc) {
+    case 1: launch_vtile<1>(a, l, r, s); break;
+    case 2: launch_vtile<2>(a, l, r, s); break;
+    case 3: launch_vtile<3>(a, l, r, s); break;
+    default: return hipErrorNotSupported;
+  }
+  return hipGetLastError();
+}
 
 hipError_t launch_sparse(SpOp op, const SpArgs& a, const SpLattice& l, bool edge, hipStream_t s) {
   if (l.n <= 0) return hipSuccess;

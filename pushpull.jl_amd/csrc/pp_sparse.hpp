@@ -67,6 +67,12 @@ enum SpOp : int { SP_VEL2MOM = 0, SP_RELAX = 1 };
 // otherwise the lattice lies in the interior and reads use the entries' linear offsets.
 hipError_t launch_sparse(SpOp op, const SpArgs& a, const SpLattice& l, bool edge, hipStream_t stream);
 
+// vel2mom over the whole volume in one LDS-tiled launch, for stencils of radius 1 <= r <= 2
+// and nc <= 3; `interior` is the reference's interior block (step 1), outside which u
+// accumulates when a.accum is set. hipErrorNotSupported otherwise (then launch_sparse
+// covers the seven regions).
+hipError_t launch_vel2mom_tiled(const SpArgs& a, const SpLattice& interior, int r, hipStream_t stream);
+
 // Error state of the C ABI (pp_last_error / pp_last_status), shared by its source files.
 int abi_fail(int code, const char* msg);
 int abi_ok();

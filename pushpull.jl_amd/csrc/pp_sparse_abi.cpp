@@ -184,10 +184,13 @@ int vel2mom_impl(const pp_sparse_kernel* k, float* u, const float* v, const int*
   Box b[7];
   regions(k, b);
   const int one[3] = {1, 1, 1};
+  const int rmax = std::max((k->dp[0] - 1) / 2, std::max((k->dp[1] - 1) / 2, (k->dp[2] - 1) / 2));
+  hipError_t e = pp::launch_vel2mom_tiled(a, lattice(k, b[6].lo, one, b[6].hi), rmax, s);
+  if (e == hipSuccess) return pp::abi_ok();
+  if (e != hipErrorNotSupported) return hip_fail(e, "vel2mom launch");
   for (int i = 0; i < 7; ++i) {
     const pp::SpLattice l = lattice(k, b[i].lo, one, b[i].hi);
-    hipError_t e = pp::launch_sparse(pp::SP_VEL2MOM, a, l, i < 6, s);
-    if (e != hipSuccess) return hip_fail(e, "vel2mom launch");
+    if ((e = pp::launch_sparse(pp::SP_VEL2MOM, a, l, i < 6, s)) != hipSuccess) return hip_fail(e, "vel2mom launch");
   }
   return pp::abi_ok();
 }
