@@ -214,3 +214,21 @@ def test_graph_replays():
         graph.stream.synchronize()
         assert bits_equal(H(v), want)
     graph.close()
+
+
+@pytest.mark.parametrize("nd", [2, 4])
+def test_other_channel_counts_bit_exact(nd):
+    d = (10, 11, 70)
+    k = pp.sparsify(op([1e-3, 0.2, 0.01, 0.05]), d, nd=nd)
+    rng = np.random.default_rng(40 + nd)
+    bnd = [(1, 2, 0), (2, 1, 1), (0, 0, 1), (1, 1, 2)][:nd]
+    v = rng.standard_normal((nd,) + d).astype(np.float32)
+    print(f"channels {nd}", flush=True)
+    assert bits_equal(H(pp.vel2mom(T(v), k, bnd)), osp.vel2mom(v, k, bnd))
+    g = rng.standard_normal((nd,) + d).astype(np.float32)
+    h = np.zeros((nd * (nd + 1) // 2,) + d, np.float32)
+    h[:nd] = 1.0 + rng.random((nd,) + d)
+    h[nd:] = 0.1 * rng.standard_normal(h[nd:].shape)
+    vt = T(v)
+    pp.relax_(T(g), T(h), k, bnd, 2, vt)
+    assert bits_equal(H(vt), osp.relax(v, g, h, k, bnd, 2))

@@ -312,3 +312,18 @@ def test_single_channel_kernel_matches_reference_build():
     assert bits_equal(osp.vel2mom(v, k, bnd), osp.ref_vel2mom(v, k, bnd))
     h = 1.0 + rng.random((1,) + d).astype(np.float32)
     assert bits_equal(osp.relax(v, v, h, k, bnd, 2), osp.ref_relax(v, v, h, k, bnd, 2))
+
+
+@ref_only
+@pytest.mark.parametrize("nd", [2, 4])
+def test_other_channel_counts_match_reference_build(nd):
+    # sparsify takes the first nd channels of L as the diagonal lists (src/operator.jl:237-253)
+    rng = np.random.default_rng(30 + nd)
+    d = (7, 8, 9)
+    k = pp.sparsify(op([1e-3, 0.2, 0.01, 0.05]), d, nd=nd)
+    assert k.nchan == nd
+    bnd = [(1, 2, 0), (2, 1, 1), (0, 0, 1), (1, 1, 2)][:nd]
+    v = rng.standard_normal((nd,) + d).astype(np.float32)
+    assert bits_equal(osp.vel2mom(v, k, bnd), osp.ref_vel2mom(v, k, bnd))
+    h = 1.0 + rng.random((nd,) + d).astype(np.float32)
+    assert bits_equal(osp.relax(v, v, h, k, bnd, 2), osp.ref_relax(v, v, h, k, bnd, 2))
