@@ -232,3 +232,23 @@ def test_other_channel_counts_bit_exact(nd):
     vt = T(v)
     pp.relax_(T(g), T(h), k, bnd, 2, vt)
     assert bits_equal(H(vt), osp.relax(v, g, h, k, bnd, 2))
+
+
+def test_overwrite_mode_and_host_hessian_layouts():
+    d = (9, 10, 67)
+    k = pp.sparsify(op([1e-3, 0.1, 0.01, 0.05]), d)
+    rng = np.random.default_rng(50)
+    v = rng.standard_normal((3,) + d).astype(np.float32)
+    # pp_vel2mom(..., accumulate_edges = 0) overwrites u everywhere
+    ut = T(rng.standard_normal((3,) + d).astype(np.float32))
+    b = pp.regulariser._bnd(DEFAULT_BND, 3)
+    L = pp._lib.lib()
+    pp._lib.check(L.pp_vel2mom(k.handle(0), pp.ops._ptr(ut), pp.ops._ptr(T(v)), b, 0, pp.ops._stream(ut)), "pp_vel2mom")
+    assert bits_equal(H(ut), osp.vel2mom(v, k, DEFAULT_BND))
+    # the host entry point with no H and with a diagonal H
+    g = rng.standard_normal((3,) + d).astype(np.float32)
+    for hcode in (0, 1):
+        h = _h(rng, hcode, d)
+        vv = v.copy()
+        pp.relax_(g, h, k, DEFAULT_BND, 2, vv)
+        assert bits_equal(vv, osp.relax(v, g, h, k, DEFAULT_BND, 2))
