@@ -13,8 +13,6 @@
 #define PP_KERNEL_SOURCE 1
 #include <hip/hip_runtime.h>
 
-#include <cstdlib>
-
 #include "pp_sparse.hpp"
 
 #pragma clang fp contract(off)
@@ -27,13 +25,7 @@ constexpr size_t kSpCoopLds = 48 * 1024;  // LDS budget of the cooperative kerne
 
 // Launches of at most this many voxels run kSpG lanes per voxel: they are too small to
 // fill the GPU, and their time is one voxel's latency. Larger ones run a thread per voxel.
-long long coop_max_voxels() {
-  static const long long v = [] {
-    const char* e = getenv("PP_SPARSE_COOP_MAX");
-    return e ? atoll(e) : 32768LL;
-  }();
-  return v;
-}
+constexpr long long kSpCoopMaxVoxels = 32768;
 
 // Fold coordinate c into [0, d) under boundary code `code` (0 circulant, 1 reflect,
 // 2 reflect with sign change on the mirrored copies); flips `neg` for a sign change.
@@ -510,7 +502,7 @@ void launch_vtile(const SpArgs& a, const SpLattice& l, int r, hipStream_t s) {
 template <int NC, int OP>
 void launch_op(const SpArgs& a, const SpLattice& l, bool edge, hipStream_t s) {
   const size_t lds = (size_t)kSpVox * (a.nent + NC * NC) * sizeof(float);
-  if (lds <= kSpCoopLds && l.n <= coop_max_voxels()) {
+  if (lds <= kSpCoopLds && l.n <= kSpCoopMaxVoxels) {
     const dim3 grid((unsigned)((l.n + kSpVox - 1) / kSpVox));
     if (edge)
       hipLaunchKernelGGL((k_sparse_coop<NC, true, true, OP>), grid, dim3(kSpBlock), lds, s, a, l);
