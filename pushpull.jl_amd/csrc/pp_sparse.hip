@@ -140,26 +140,30 @@ __device__ inline void chol_solve(float (&A)[NC * NC], float (&b)[NC], float (&x
   }
 }
 
-// relax1 / relax_padded1 after the stencil (sparse_operator_dev.cu:117-131): b -= H v,
-// A = the lists' centres (+ H), sol = A \ b by Cholesky. vc: v at the voxel.
+// H at the voxel (nc or nc (nc + 1) / 2 channels), loaded with g and v before the stencil.
 template <int NC>
-__device__ inline void relax_solve(const SpArgs& a, long long m, float (&b)[NC], const float (&vc)[NC],
-                                   float (&sol)[NC]) {
-  const long long nd = a.nd;
-  float hv[NC * (NC + 1) / 2];
+__device__ inline void load_h(const SpArgs& a, long long m, float (&hv)[NC * (NC + 1) / 2]) {
+  if (!a.hcode) return;
+  const int nh = a.hcode == 2 ? NC * (NC + 1) / 2 : NC;
+#pragma unroll
+  for (int i = 0; i < NC * (NC + 1) / 2; ++i)
+    if (i < nh) hv[i] = a.h[m + a.nd * i];
+}
+
+// relax1 / relax_padded1 after the stencil (sparse_operator_dev.cu:117-131): b -= H v,
+// A = the lists' centres (+ H), sol = A \ b by Cholesky. vc: v at the voxel, hv: H there.
+template <int NC>
+__device__ inline void relax_solve(const SpArgs& a, float (&b)[NC], const float (&vc)[NC],
+                                   const float (&hv)[NC * (NC + 1) / 2], float (&sol)[NC]) {
   if (a.hcode) {  // Hv (sparse_operator_dev.cu:32-54)
 #pragma unroll
-    for (int i = 0; i < NC; ++i) {
-      hv[i] = a.h[m + nd * i];
-      b[i] -= hv[i] * vc[i];
-    }
+    for (int i = 0; i < NC; ++i) b[i] -= hv[i] * vc[i];
     if (a.hcode == 2) {
       int ii = NC;
 #pragma unroll
       for (int j = 0; j < NC; ++j)
 #pragma unroll
         for (int i = j + 1; i < NC; ++i, ++ii) {
-          hv[ii] = a.h[m + nd * ii];
           b[j] -= hv[ii] * vc[i];
           b[i] -= hv[ii] * vc[j];
         }
@@ -226,14 +230,15 @@ __global__ __launch_bounds__(kSpBlock) void k_sparse(SpArgs a, SpLattice l) {
     for (int i = 0; i < NC; ++i) a.u[m + nd * i] = u[i];
     return;
   }
-  float b[NC], vc[NC], sol[NC];
+  float b[NC], vc[NC], sol[NC], hv[NC * (NC + 1) / 2];
 #pragma unroll
   for (int i = 0; i < NC; ++i) {
     b[i] = a.g[m + nd * i];
     vc[i] = a.v[m + nd * i];
   }
+  load_h<NC>(a, m, hv);
   apply_lists<NC, EDGE, BIG, true>(a, b, x, y, z, m);
-  relax_solve<NC>(a, m, b, vc, sol);
+  relax_solve<NC>(a, b, vc, hv, sol);
 #pragma unroll
   for (int i = 0; i < NC; ++i) a.v[m + nd * i] = vc[i] + sol[i];
 }
@@ -260,13 +265,14 @@ __global__ __launch_bounds__(kSpBlock) void k_sparse_coop(SpArgs a, SpLattice l)
   const long long m = x + (long long)a.d[0] * (y + (long long)a.d[1] * z);
   const long long nd = a.nd;
   const unsigned mb = BIG ? 0u : (unsigned)m * 4u;
-  float b[NC], vc[NC];
+  float b[NC], vc[NC], hv[NC * (NC + 1) / 2];
   if (OP == SP_RELAX && ok && r < NC) {
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
       b[i] = a.g[m + nd * i];
       vc[i] = a.v[m + nd * i];
     }
+    load_h<NC>(a, m, hv);
   }
   // entries e0 + b kSpG, b < kSpLB, of this lane: all entry loads, then all tap loads
   if (a.nent <= kSpG * kSpLB) {  // one batch per lane: entries and taps stay in registers
@@ -346,7 +352,7 @@ __global__ __launch_bounds__(kSpBlock) void k_sparse_coop(SpArgs a, SpLattice l)
 #pragma unroll
     for (int i = 0; i < NC; ++i) b[i] -= cl[i + NC * j];
   float sol[NC];
-  relax_solve<NC>(a, m, b, vc, sol);
+  relax_solve<NC>(a, b, vc, hv, sol);
   float sr = sol[0], vr = vc[0];
 #pragma unroll
   for (int i = 1; i < NC; ++i)
