@@ -8,6 +8,12 @@ smooth deformation (identity + displacement of max 6 voxels, reflect boundary, e
 i.e. 48 + 24C = 72 bytes/voxel of compulsory HBM traffic per degree. Inputs are resident
 in HBM before timing. value = 3 * 256^3 voxels / step time, in Gvoxel/s.
 
+The three ops of a degree are independent, so by default (--schedule overlap) the pullback
+runs on a second HIP stream beside pull and push, joined at the end of each step: the
+gathers' VALU-bound tiles then share the CUs with push's LDS-bound tiles. The per-op
+breakdown comes from the same number of serial steps run afterwards (--schedule serial
+times those instead).
+
 The CPU baseline is the reference's own C (artifacts/C/pushpull.c, compiled from source
 into oracle/_ref by oracle/Makefile, single-threaded as shipped) timed on this host on a
 z-slab of the same workload; when that build is absent the project's FP32 restatement
@@ -492,6 +498,9 @@ def main():
                     help="skip the batched-uneven and matching-loop legs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run k_fill and k_axpy over 64 Mi floats (reference kernels for HBM counter scaling)")
+    ap.add_argument("--schedule", choices=["overlap", "serial"], default="overlap",
+                    help="overlap: the pullback runs on a second stream beside pull and push; "
+                         "serial: the nine launches in order on one stream")
     ap.add_argument("--leg", choices=["regulariser"], default=None,
                     help="run only this secondary leg and print its JSON (for profiling)")
     ap.add_argument("--reg-op", choices=["all", *REG_OPS], default="all",
@@ -538,6 +547,26 @@ def main():
     P = lambda t: ct.c_void_p(t.data_ptr())  # noqa: E731
     stream = ct.c_void_p(torch.cuda.current_stream().cuda_stream)
 
+    side = torch.cuda.Stream()  # default priority
+    side_h = ct.c_void_p(side.cuda_stream)
+    main_s = torch.cuda.current_stream()
+
+    def step_overlap():
+        # pull and push on the launch stream, the pullback on a second stream: the three ops
+        # of a degree are independent (they write f1, df, dphi), and the gathers' VALU work
+        # overlaps push's LDS-bound work on the same CUs; joined at the end of the step
+        e = torch.cuda.Event()
+        e.record(main_s)
+        side.wait_event(e)
+        for deg in DEGS:
+            d = ct.byref(descs[deg])
+            _lib.check(L.pp_pull(d, P(f1), P(phi), P(f), stream), "pull")
+            _lib.check(L.pp_push(d, P(df), P(phi), P(delta), 0, stream), "push")
+            _lib.check(L.pp_pull_backward(d, P(dphi), P(phi), P(f), P(delta), side_h), "backward")
+        e2 = torch.cuda.Event()
+        e2.record(side)
+        main_s.wait_event(e2)
+
     def step(events=None):
         for deg in DEGS:
             d = ct.byref(descs[deg])
@@ -567,9 +596,10 @@ def main():
         def add(self, deg, m):
             self.rows.append((deg, m))
 
+    timed_step = step_overlap if args.schedule == "overlap" else step
     _progress("warm-up")
     for _ in range(args.warmup):
-        step()
+        timed_step()
     torch.cuda.synchronize()
     _progress("timed steps")
     ev = Ev()
@@ -579,11 +609,20 @@ def main():
     t0 = time.perf_counter()
     start.record()
     for _ in range(args.steps):
-        step(ev)
+        if args.schedule == "overlap":
+            step_overlap()
+        else:
+            step(ev)
     end.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_step = start.elapsed_time(end) / args.steps
+    if args.schedule == "overlap":  # per-op times come from separate serial steps
+        _progress("serial steps for the per-op breakdown")
+        for _ in range(args.steps):
+            step(ev)
+        torch.cuda.synchronize()
+    ms_serial = float(np.sum([m[0].elapsed_time(m[3]) for _, m in ev.rows])) / args.steps
 
     per = {deg: {"pull": [], "push": [], "backward": []} for deg in DEGS}
     for deg, m in ev.rows:
@@ -634,11 +673,15 @@ def main():
         "data": "synthetic (smoothed Gaussian noise image and upstream gradient; identity + smooth displacement, max 6 voxels)",
         "config": {"workload": "pull+push+backward, 256^3 x 1 channel, degrees 1,2,3 (one triple each per step), "
                                "reflect boundary, ext=1", "volume": [N, N, N], "channels": 1, "degrees": list(DEGS),
-                   "kernels": {0: "per-sample", 1: "automatic", 2: "tiled"}[args.tiling], "tile_shape": args.shape},
+                   "kernels": {0: "per-sample", 1: "automatic", 2: "tiled"}[args.tiling], "tile_shape": args.shape,
+                   "schedule": {"overlap": "pull and push on the launch stream, pullback on a second stream, "
+                                           "joined per step",
+                                "serial": "nine launches in order on one stream"}[args.schedule]},
         "roofline": {"bound": "hbm", "kernel": f"{kernel_name(op, deg, args.tiling)} deg{deg}", "achieved": round(gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
                      "traffic": _traffic(kernel_name(op, deg, args.tiling), deg), "bytes_per_launch": BYTES[op] * n1,
                      "avg_ms": round(ms, 4), "op_ms_with_fill": round(ms_op, 4)},
+        "serial_ms_per_step": round(ms_serial, 4),
         "per_degree": detail,
         "wall_s": round(wall, 3),
     }
