@@ -7,7 +7,9 @@ ChainRulesCore rrules in src/pushpull_pb.jl:11-51.
 
 The phi pullbacks are single fused kernels (pp_pull_backward / pp_pull_grad_backward):
 the 3C-channel pull_grad the Julia rule materialises before contracting
-(src/pushpull_pb.jl:113-117) is never written to memory.
+(src/pushpull_pb.jl:113-117) is never written to memory. When both gradients are wanted
+on the GPU, the pullback runs on a second stream while the other gradient runs on the
+current one (the two are independent).
 """
 from __future__ import annotations
 
@@ -19,6 +21,37 @@ from .settings import Settings
 
 def _spatial(t):
     return tuple(t.shape[-3:])
+
+
+_side_streams = {}
+
+
+def _side_stream(device):
+    """A second stream per device (default priority) for the φ pullback."""
+    s = _side_streams.get(device.index)
+    if s is None:
+        s = _side_streams[device.index] = torch.cuda.Stream(device=device)
+    return s
+
+
+def _both(first, second, tensors):
+    """Run first() on the current stream and second() on a side stream at the same time;
+    the two are independent ops of one rrule (they write different arrays). Returns both
+    results, the second recorded on the current stream, which waits for the side stream."""
+    if not (tensors[0].is_cuda and torch.cuda.is_available()):
+        return first(), second()
+    dev = tensors[0].device
+    cur = torch.cuda.current_stream(dev)
+    side = _side_stream(dev)
+    side.wait_stream(cur)
+    with torch.cuda.stream(side):
+        b = second()
+    a = first()
+    cur.wait_stream(side)
+    b.record_stream(cur)
+    for t in tensors:  # inputs read on the side stream
+        t.record_stream(side)
+    return a, b
 
 
 class _Pull(torch.autograd.Function):
@@ -33,11 +66,20 @@ class _Pull(torch.autograd.Function):
         f0, phi = ctx.saved_tensors
         delta = delta.contiguous()
         df0 = dphi = None
-        if ctx.needs_input_grad[0]:
-            df0 = ops.push_raw(delta, phi, _spatial(f0), ctx.sett).reshape(f0.shape)
-        if ctx.needs_input_grad[1]:
-            dphi = ops.pull_backward(f0.reshape(_as5(f0)), phi.reshape(_as5(phi)),
+
+        def d_f0():
+            return ops.push_raw(delta, phi, _spatial(f0), ctx.sett).reshape(f0.shape)
+
+        def d_phi():
+            return ops.pull_backward(f0.reshape(_as5(f0)), phi.reshape(_as5(phi)),
                                      delta.reshape(_as5(delta)), ctx.sett).reshape(phi.shape)
+
+        if ctx.needs_input_grad[0] and ctx.needs_input_grad[1]:
+            df0, dphi = _both(d_f0, d_phi, (f0, phi, delta))
+        elif ctx.needs_input_grad[0]:
+            df0 = d_f0()
+        elif ctx.needs_input_grad[1]:
+            dphi = d_phi()
         return df0, dphi, None
 
 
@@ -53,12 +95,21 @@ class _PullGrad(torch.autograd.Function):
         f0, phi = ctx.saved_tensors
         delta = delta.contiguous()
         df0 = dphi = None
-        if ctx.needs_input_grad[0]:
-            df0 = ops.push_grad(delta, phi, _spatial(f0), ctx.sett).reshape(f0.shape)
-        if ctx.needs_input_grad[1]:
+
+        def d_f0():
+            return ops.push_grad(delta, phi, _spatial(f0), ctx.sett).reshape(f0.shape)
+
+        def d_phi():
             f5 = f0.reshape(_as5(f0))
             d6 = delta.reshape((f5.shape[0], f5.shape[1], 3) + _spatial(delta))
-            dphi = ops.pull_grad_backward(f5, phi.reshape(_as5(phi)), d6, ctx.sett).reshape(phi.shape)
+            return ops.pull_grad_backward(f5, phi.reshape(_as5(phi)), d6, ctx.sett).reshape(phi.shape)
+
+        if ctx.needs_input_grad[0] and ctx.needs_input_grad[1]:
+            df0, dphi = _both(d_f0, d_phi, (f0, phi, delta))
+        elif ctx.needs_input_grad[0]:
+            df0 = d_f0()
+        elif ctx.needs_input_grad[1]:
+            dphi = d_phi()
         return df0, dphi, None
 
 
@@ -74,11 +125,20 @@ class _Push(torch.autograd.Function):
         f1, phi = ctx.saved_tensors
         delta = delta.contiguous()
         df1 = dphi = None
-        if ctx.needs_input_grad[0]:
-            df1 = ops.pull_raw(delta, phi, ctx.sett).reshape(f1.shape)
-        if ctx.needs_input_grad[1]:
-            dphi = ops.pull_backward(delta.reshape(_as5(delta)), phi.reshape(_as5(phi)),
+
+        def d_f1():
+            return ops.pull_raw(delta, phi, ctx.sett).reshape(f1.shape)
+
+        def d_phi():
+            return ops.pull_backward(delta.reshape(_as5(delta)), phi.reshape(_as5(phi)),
                                      f1.reshape(_as5(f1)), ctx.sett).reshape(phi.shape)
+
+        if ctx.needs_input_grad[0] and ctx.needs_input_grad[1]:
+            df1, dphi = _both(d_f1, d_phi, (f1, phi, delta))
+        elif ctx.needs_input_grad[0]:
+            df1 = d_f1()
+        elif ctx.needs_input_grad[1]:
+            dphi = d_phi()
         return df1, dphi, None, None
 
 
