@@ -331,6 +331,50 @@ def bench_invert_def(torch, pp, reps=5, seed=7):
             "GB_s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
+def bench_affine(torch, pp, reps=5, seed=9):
+    """affine_pull / affine_push (src/pushpull_cpu.jl:268-362) at 256^3 x 1 channel, degrees
+    1-3, reflect boundary, ext=1, through a 10-degree rotation about z with 0.95 scaling and
+    a shift (the sample coordinates come from the matrix; no deformation is read).
+    Algorithmic bytes: pull reads f0 and writes f1, push reads f1 and writes f0: 8 B/voxel."""
+    import ctypes as ct
+
+    dev = torch.device("cuda:0")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    d = (N, N, N)
+    f0 = _smooth_noise(torch, d, 3.0, gen, dev).contiguous()
+    f1 = torch.empty_like(f0)
+    th = np.deg2rad(10.0)
+    c, sn = 0.95 * np.cos(th), 0.95 * np.sin(th)
+    Aff = np.array([[c, -sn, 0, 12.0], [sn, c, 0, -9.0], [0, 0, 0.95, 6.0], [0, 0, 0, 1]], np.float32)
+    a12 = pp.affine._aff12(Aff)
+    L = pp._lib.lib()
+    P = lambda t: ct.c_void_p(t.data_ptr())  # noqa: E731
+    st = ct.c_void_p(torch.cuda.current_stream().cuda_stream)
+    n = N ** 3
+    out = {"volume": [N, N, N], "affine": "0.95 x rotation of 10 degrees about z, shift (12, -9, 6)"}
+    for deg in DEGS:
+        dsc = pp.ops._desc(pp.Settings(deg, 1, 1), d, d, 1, 1)
+        ha = pp.ops._hostptr(a12)
+
+        def pull():
+            pp._lib.check(L.pp_affine_pull(ct.byref(dsc), P(f1), P(f0), ha, st), "pp_affine_pull")
+
+        def push():
+            pp._lib.check(L.pp_affine_push(ct.byref(dsc), P(f0), P(f1), ha, 1, st), "pp_affine_push")
+
+        pull()
+        push()
+        ms_pull = _time_ms(torch, pull, reps, None)
+        ms_push = _time_ms(torch, push, reps, None)
+        gp, gq = 8 * n / (ms_pull * 1e-3) / 1e9, 8 * n / (ms_push * 1e-3) / 1e9
+        out[f"deg{deg}"] = {"pull_ms": round(ms_pull, 4), "pull_GB_s": round(gp, 1),
+                            "pull_hbm_frac": round(gp / HBM_PEAK_GBS, 4), "push_ms": round(ms_push, 4),
+                            "push_GB_s": round(gq, 1), "push_hbm_frac": round(gq / HBM_PEAK_GBS, 4),
+                            "pull_Gvoxel_s": round(n / (ms_pull * 1e-3) / 1e9, 2)}
+    return out
+
+
 REG_OPS = {"membrane": [1e-3, 1.0, 0.0, 0.0], "bending+divergence": [1e-3, 0.1, 0.01, 0.05]}
 
 
@@ -501,7 +545,7 @@ def main():
     ap.add_argument("--schedule", choices=["overlap", "serial"], default="overlap",
                     help="overlap: the pullback runs on a second stream beside pull and push; "
                          "serial: the nine launches in order on one stream")
-    ap.add_argument("--leg", choices=["regulariser"], default=None,
+    ap.add_argument("--leg", choices=["regulariser", "affine", "exp", "invert_def", "batched"], default=None,
                     help="run only this secondary leg and print its JSON (for profiling)")
     ap.add_argument("--reg-op", choices=["all", *REG_OPS], default="all",
                     help="operators of the regulariser leg")
@@ -519,6 +563,10 @@ def main():
     if args.leg == "regulariser":
         ops_sel = list(REG_OPS) if args.reg_op == "all" else [args.reg_op]
         print(json.dumps({"regulariser_256": bench_regulariser(torch, pp, ops=ops_sel)}))
+        return
+    if args.leg is not None:
+        leg = {"affine": bench_affine, "exp": bench_exp, "invert_def": bench_invert_def, "batched": bench_batched}
+        print(json.dumps({args.leg: leg[args.leg](torch, pp)}))
         return
 
     dev = torch.device("cuda:0")
@@ -698,6 +746,9 @@ def main():
         torch.cuda.empty_cache()
         _progress("invert_def pushes (push with count)")
         res["invert_def_push_count_256"] = bench_invert_def(torch, pp)
+        torch.cuda.empty_cache()
+        _progress("affine pull / push")
+        res["affine_256"] = bench_affine(torch, pp)
         torch.cuda.empty_cache()
         _progress("sparse regulariser (vel2mom, relax)")
         res["regulariser_256"] = bench_regulariser(torch, pp)
