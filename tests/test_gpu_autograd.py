@@ -85,3 +85,30 @@ def test_push_gradients():
     (pp.push(tf1, tp, f0.shape[-3:], s) * T(delta)).sum().backward()
     _close(H(tf1.grad), oracle.pull(delta, phi, so, "f64"), "df1")
     _close(H(tp.grad), oracle.backward_phi(delta, phi[None], f1, so)[0], "dphi")
+
+
+def test_gradients_match_finite_differences():
+    # the rrules against calculus rather than the oracle's formulas: central differences of
+    # L = <pull(f0, φ), δ> in FP64 on the host oracle, at a few entries of f0 and φ
+    rng = np.random.default_rng(90)
+    f0, phi = _inputs(rng)
+    s, so = pp.Settings(3, 1, 1), S((3, 3, 3), (1, 1, 1), 1)
+    delta = rng.standard_normal((1, 1) + phi.shape[1:]).astype(np.float32)
+    tf, tp = T(f0, True), T(phi, True)
+    (pp.pull(tf, tp, s) * T(delta)).sum().backward()
+    gf, gp = H(tf.grad), H(tp.grad)
+
+    def loss(f, p):
+        return float((oracle.pull(f, p, so, "f64") * delta).sum())
+
+    for idx in [(0, 0, 6, 6, 7), (0, 0, 3, 9, 2), (0, 0, 10, 1, 12)]:
+        e = np.zeros_like(f0, dtype=np.float64)
+        e[idx] = 1.0
+        fd = (loss(f0 + e, phi) - loss(f0 - e, phi)) / 2  # L is linear in f0
+        assert abs(fd - gf[idx]) <= 1e-4 * max(1.0, abs(fd)), (idx, fd, gf[idx])
+    h = 1e-2
+    for idx in [(0, 5, 6, 6), (1, 4, 7, 3), (2, 8, 2, 9)]:
+        e = np.zeros_like(phi)
+        e[idx] = h
+        fd = (loss(f0, phi + e) - loss(f0, phi - e)) / (2 * h)
+        assert abs(fd - gp[idx]) <= 2e-3 * max(1.0, abs(fd)), (idx, fd, gp[idx])
