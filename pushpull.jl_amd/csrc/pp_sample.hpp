@@ -79,6 +79,25 @@ __device__ __forceinline__ bool setup(const KArgs& a, const float* phi, int v, A
 
 #define PP_TAP(DP, ax, i) if (DP > 0 || (i) < (ax).n)
 
+// The x taps of one row. Trilinear taps that are adjacent in memory (every x knot but the
+// folded ones at the faces) come in one 8-byte load, halving the gather instructions.
+template <int DP>
+__device__ __forceinline__ void row_taps(const float* __restrict__ row, const Axis<DP>& ax,
+                                         float (&v)[Axis<DP>::NP]) {
+  constexpr int NP = Axis<DP>::NP;
+  if constexpr (DP == 2) {
+    if (ax.off[1] == ax.off[0] + 1) {
+      typedef float pair_t __attribute__((ext_vector_type(2), aligned(4)));
+      const pair_t p = *reinterpret_cast<const pair_t*>(row + ax.off[0]);
+      v[0] = p.x;
+      v[1] = p.y;
+      return;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NP; ++i) PP_TAP(DP, ax, i) v[i] = row[ax.off[i]];
+}
+
 // Value: sum_ijk wx_i wy_j wz_k f[...]
 template <int DP, bool LEAN = false>
 __device__ __forceinline__ float gather_val(const float* __restrict__ f, const Axis<DP>& ax, const Axis<DP>& ay,
@@ -90,10 +109,11 @@ __device__ __forceinline__ float gather_val(const float* __restrict__ f, const A
   for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
 #pragma unroll
       for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
-          const float* row = f + az.off[k] + ay.off[j];
+          float v[NP];
+          row_taps<DP>(f + az.off[k] + ay.off[j], ax, v);
           float r = 0.f;
 #pragma unroll
-          for (int i = 0; i < NP; ++i) PP_TAP(DP, ax, i) r += ax.w[i] * row[ax.off[i]];
+          for (int i = 0; i < NP; ++i) PP_TAP(DP, ax, i) r += ax.w[i] * v[i];
           acc += (az.w[k] * ay.w[j]) * r;
         }
     }
@@ -111,13 +131,13 @@ __device__ __forceinline__ void gather_grad(const float* __restrict__ f, const A
   for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
 #pragma unroll
       for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
-          const float* row = f + az.off[k] + ay.off[j];
+          float v[NP];
+          row_taps<DP>(f + az.off[k] + ay.off[j], ax, v);
           float r = 0.f, rd = 0.f;
 #pragma unroll
           for (int i = 0; i < NP; ++i) PP_TAP(DP, ax, i) {
-              const float v = row[ax.off[i]];
-              r += ax.w[i] * v;
-              rd += ax.dw[i] * v;
+              r += ax.w[i] * v[i];
+              rd += ax.dw[i] * v[i];
             }
           gx += (az.w[k] * ay.w[j]) * rd;
           gy += (az.w[k] * ay.dw[j]) * r;
@@ -137,13 +157,13 @@ __device__ __forceinline__ void gather_val_grad(const float* __restrict__ f, con
   for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
 #pragma unroll
       for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
-          const float* row = f + az.off[k] + ay.off[j];
+          float v[NP];
+          row_taps<DP>(f + az.off[k] + ay.off[j], ax, v);
           float r = 0.f, rd = 0.f;
 #pragma unroll
           for (int i = 0; i < NP; ++i) PP_TAP(DP, ax, i) {
-              const float v = row[ax.off[i]];
-              r += ax.w[i] * v;
-              rd += ax.dw[i] * v;
+              r += ax.w[i] * v[i];
+              rd += ax.dw[i] * v[i];
             }
           const float wyz = az.w[k] * ay.w[j];
           val += wyz * r;
