@@ -23,6 +23,7 @@ from __future__ import annotations
 
 import argparse
 import ctypes as ct
+import gc
 import json
 import os
 import sys
@@ -155,6 +156,22 @@ def _time_ms(torch, fn, reps, stream):
     e1.record(stream)
     e1.synchronize()
     return e0.elapsed_time(e1) / reps
+
+
+def _time_ms_median(torch, fn, reps):
+    """Median ms of fn() over reps calls, one pair of HIP events around each call on the
+    current stream: a host stall between launches (which shows as idle GPU time inside
+    that call) moves one sample, not the result."""
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    gc.collect()
+    gc.disable()
+    for a, b in ev:
+        a.record()
+        fn()
+        b.record()
+    ev[-1][1].synchronize()
+    gc.enable()
+    return float(np.median([a.elapsed_time(b) for a, b in ev]))
 
 
 def bench_batched(torch, pp, reps=5, seed=4):
@@ -384,7 +401,7 @@ def bench_regulariser(torch, pp, reps=5, seed=8, ops=tuple(REG_OPS)):
     (membrane, 3^3 stencil, src/multigrid.jl:175-181) and bending + divergence (5^3).
     Algorithmic bytes: vel2mom reads v and writes u, 24 B/voxel; a sweep reads v, g, H and
     writes v, 60 B/voxel. A sweep is one launch per colour of each region, as the
-    reference's sequential order requires (dp^3 colours)."""
+    reference's sequential order requires (dp^3 colours). Times are medians over calls."""
     from pushpull.regulariser import DEFAULT_BND
 
     dev = torch.device("cuda:0")
@@ -401,10 +418,10 @@ def bench_regulariser(torch, pp, reps=5, seed=8, ops=tuple(REG_OPS)):
     for name in ops:
         k = pp.sparsify(pp.registration_operator(np.float32([1, 1, 1]), np.float32(REG_OPS[name])), d)
         pp.vel2mom_(u, v, k)  # uploads the kernel and warms up
-        ms_v = _time_ms(torch, lambda: pp.vel2mom_(u, v, k), reps, None)
+        ms_v = _time_ms_median(torch, lambda: pp.vel2mom_(u, v, k), reps)
         w = v.clone()
         pp.relax_(g, h, k, DEFAULT_BND, 1, w)
-        ms_r = _time_ms(torch, lambda: pp.relax_(g, h, k, DEFAULT_BND, 1, w), reps, None)
+        ms_r = _time_ms_median(torch, lambda: pp.relax_(g, h, k, DEFAULT_BND, 1, w), reps)
         gv, gr = 24 * n / (ms_v * 1e-3) / 1e9, 60 * n / (ms_r * 1e-3) / 1e9
         out[name] = {"stencil": list(k.stride), "entries": int(k.length.sum()),
                      "vel2mom_ms": round(ms_v, 4), "vel2mom_GB_s": round(gv, 1), "vel2mom_hbm_frac": round(gv / HBM_PEAK_GBS, 4),
@@ -653,6 +670,8 @@ def main():
     ev = Ev()
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
+    gc.collect()
+    gc.disable()  # no collector pause inside the timed launches
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     start.record()
@@ -664,6 +683,7 @@ def main():
     end.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    gc.enable()
     ms_step = start.elapsed_time(end) / args.steps
     if args.schedule == "overlap":  # per-op times come from separate serial steps
         _progress("serial steps for the per-op breakdown")
