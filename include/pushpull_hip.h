@@ -155,6 +155,15 @@ int pp_pull_grad_backward(const pp_desc *d, float *dphi, const float *phi, const
 int pp_objective_grad(const pp_desc *d, float *dphi, float *df0, double *loss, const float *phi,
                       const float *f0, const float *g, int accumulate, void *stream);
 
+/* Both cotangents of pull in one pass (the rrule of pull, src/pushpull_pb.jl:11-17, whose
+ * two thunks call push and pull_grad separately; the reference has no fused entry point):
+ *   dphi (d1,3,B) = sum_c delta_c grad pull(f0)_c, NaN where the sample is invalid
+ *                   (= pp_pull_backward),
+ *   df0  (d0,C,B) = push(delta) (zeroed first unless accumulate; = pp_push).
+ * One launch stages each tile of f0 once and reads phi and delta once for both. */
+int pp_pull_vjp(const pp_desc *d, float *dphi, float *df0, const float *phi, const float *f0,
+                const float *delta, int accumulate, void *stream);
+
 /* ---- push with a count image ------------------------------------------------
  * pp_push_count replaces pushc (pushpull.cu:128-142: push1 with its count at
  * pushpull_dev.cu:140-154) and the push pairs of its callers: invert_def pushes the
@@ -233,7 +242,8 @@ enum {
   PP_OP_PUSH = 3,          /* out = push(f, phi) (zeroed first)                    */
   PP_OP_PUSH_GRAD = 4,     /* out = push_grad(f, phi) (zeroed first)               */
   PP_OP_PULL_BACKWARD = 5, /* out = dphi = sum_c aux_c grad pull(f)_c              */
-  PP_OP_OBJECTIVE = 7      /* out = dphi, out2 = dL/df (zeroed first), loss[0] = L */
+  PP_OP_OBJECTIVE = 7,     /* out = dphi, out2 = dL/df (zeroed first), loss[0] = L */
+  PP_OP_PULL_VJP = 8       /* out = dphi, out2 = push(aux) (zeroed first)          */
 };
 
 typedef struct pp_volume {

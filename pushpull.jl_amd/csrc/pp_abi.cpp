@@ -172,7 +172,8 @@ bool sign_split(const pp_desc* d, int base[3], unsigned& mask) {
 // the tiled pull and push, differ only in sign), else one per channel (pointers offset by
 // the channel strides; channel-reducing outputs accumulate after the first launch).
 int run(pp::Op op, const pp_desc* d, const Geo& g, pp::KArgs a, hipStream_t s) {
-  const bool reduce = (op == pp::OP_PULL_BWD || op == pp::OP_PULLG_BWD || op == pp::OP_OBJGRAD);
+  const bool reduce =
+      (op == pp::OP_PULL_BWD || op == pp::OP_PULLG_BWD || op == pp::OP_OBJGRAD || op == pp::OP_PULL_VJP);
   if (uniform_bnd(d)) {
     a.bx = d->bnd[0];
     a.by = d->bnd[1];
@@ -538,6 +539,32 @@ int pp_objective_grad(const pp_desc* d, float* dphi, float* df0, double* loss, c
   a.out2_bs = g.n0 * d->nchan;
   a.red = loss;
   return run(pp::OP_OBJGRAD, d, g, a, S(s));
+}
+
+int pp_pull_vjp(const pp_desc* d, float* dphi, float* df0, const float* phi, const float* f0, const float* delta,
+                int accumulate, void* s) {
+  Geo g;
+  if (int r = check_desc(d, g)) return r;
+  if (!dphi || !df0 || !phi || !f0 || !delta) return fail(PP_EINVAL, "null array");
+  if (!accumulate) {
+    hipError_t e = pp::launch_fill(df0, g.n0 * d->nchan * d->nbatch, 0.0f, S(s));
+    if (e != hipSuccess) return hip_fail(e, "zero fill");
+  }
+  pp::KArgs a;
+  base_args(d, a, g);
+  a.f = f0;
+  a.f_cs = g.n0;
+  a.f_bs = g.n0 * d->nchan;
+  a.phi = phi;
+  a.aux = delta;
+  a.aux_cs = g.n1;
+  a.aux_bs = g.n1 * d->nchan;
+  a.out = dphi;
+  a.out_bs = 3 * g.n1;
+  a.out2 = df0;
+  a.out2_cs = g.n0;
+  a.out2_bs = g.n0 * d->nchan;
+  return run(pp::OP_PULL_VJP, d, g, a, S(s));
 }
 
 // ---- push with a count image ----------------------------------------------------------
@@ -957,6 +984,7 @@ int pp_tune(int op, const pp_desc* d, float* out, float* out2, double* loss, con
       case PP_OP_PUSH_GRAD: return pp_push_grad(d, out, phi, f, 0, s);
       case PP_OP_PULL_BACKWARD: return pp_pull_backward(d, out, phi, f, aux, s);
       case PP_OP_OBJECTIVE: return pp_objective_grad(d, out, out2, loss, phi, f, aux, 0, s);
+      case PP_OP_PULL_VJP: return pp_pull_vjp(d, out, out2, phi, f, aux, 0, s);
       default: return fail(PP_EINVAL, "op %d has a single kernel variant", op);
     }
   };

@@ -217,6 +217,37 @@ __global__ __launch_bounds__(256) void k_objgrad(KArgs a) {
   }
 }
 
+// Vector-Jacobian product of pull (both cotangents of the rrule, src/pushpull_pb.jl:11-17):
+// dphi = sum_c delta_c grad f_c (NaN where invalid); df0 += push(delta) (skips non-finite
+// delta, pushpull.c:155).
+template <int DP>
+__global__ __launch_bounds__(256) void k_pull_vjp(KArgs a) {
+  const int bb = blockIdx.y;
+  const int v = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (v >= a.n1) return;
+  const float* phi = a.phi + bb * a.phi_bs;
+  Axis<DP> ax, ay, az;
+  float* out = a.out + bb * a.out_bs + v;
+  const long long n1 = a.n1;
+  if (!setup<DP, kW | kDW>(a, phi, v, ax, ay, az)) {
+    const float nan = __builtin_nanf("");
+    out[0] = nan; out[n1] = nan; out[2 * n1] = nan;
+    return;
+  }
+  const float* f = a.f + bb * a.f_bs;
+  const float* d = a.aux + bb * a.aux_bs + v;
+  float* df = a.out2 + bb * a.out2_bs;
+  float sx = 0.f, sy = 0.f, sz = 0.f;
+  for (int c = 0; c < a.nchan; ++c) {
+    float gx, gy, gz;
+    sample_grad<DP>(f + c * a.f_cs, ax, ay, az, gx, gy, gz);
+    const float w = d[c * a.aux_cs];
+    sx += w * gx; sy += w * gy; sz += w * gz;
+    if (isfinite(w)) scatter_val<DP>(df + c * a.out2_cs, ax, ay, az, w);
+  }
+  put3(out, n1, sx, sy, sz, a.accum);
+}
+
 __global__ void k_fill(float* p, long long n, float v) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long stride = (long long)gridDim.x * blockDim.x;
@@ -248,6 +279,7 @@ static hipError_t launch_dp(Op op, const KArgs& a, int nbatch, hipStream_t s, in
     case OP_PULL_BWD: hipLaunchKernelGGL(k_pull_bwd<DP>, grid, dim3(block), 0, s, a); break;
     case OP_PULLG_BWD: hipLaunchKernelGGL(k_pullg_bwd<DP>, grid, dim3(block), 0, s, a); break;
     case OP_OBJGRAD: hipLaunchKernelGGL(k_objgrad<DP>, grid, dim3(block), 0, s, a); break;
+    case OP_PULL_VJP: hipLaunchKernelGGL(k_pull_vjp<DP>, grid, dim3(block), 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

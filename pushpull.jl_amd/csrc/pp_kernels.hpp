@@ -15,7 +15,8 @@ enum Op : int {
   OP_PULL_BWD = 5,  // dphi = sum_c delta_c grad f_c               pushpull_pb.jl:13-17
   OP_PULLG_BWD = 6, // dphi = sum_{a,c} delta_{a,c} H_{a,.,c}      pushpull_pb.jl:128-132
   OP_OBJGRAD = 7,   // r = pull(f0)-g; dphi = sum_c r_c grad f_c; df0 += push(r); loss += r^2/2
-  OP_COUNT = 8
+  OP_PULL_VJP = 8,  // dphi = sum_c delta_c grad f_c; df0 += push(delta)  pushpull_pb.jl:11-17
+  OP_COUNT = 9
 };
 
 // Every array in the kernel arguments lives in device global memory. In the kernel

@@ -538,7 +538,8 @@ __global__ __launch_bounds__(kTB) void k_push_tile(KArgs a) {
   push_tile<DP, TX, TY, TZ, CAP, GRAD>(a, xcd_block(blockIdx.x, gridDim.x), blockIdx.y, box, wsum, S);
 }
 
-// ---- gathers: pull (KIND 0), phi pullback of pull (1), fused objective (2) ----------
+// ---- gathers: pull (KIND 0), phi pullback of pull (1), fused objective (2) and the --
+// ---- vector-Jacobian product of pull (3): push(delta) and the phi pullback together --
 template <int DP>
 __device__ __forceinline__ float lds_val(const float* box, const LSample<DP>& L, int BX, int BXY) {
   float acc = 0.f;
@@ -580,8 +581,8 @@ __device__ __forceinline__ void lds_grad(const float* box, const LSample<DP>& L,
 }
 
 // Gradient gather of one sample outside the tile's box, through global memory; for the
-// objective (KIND 2) it also scatters the residual. Returns the weight w of the pullback
-// (the upstream gradient, or the residual). Rarely taken, so it gathers lean: it must not
+// objective (KIND 2) it also scatters the residual, for the VJP (3) the upstream gradient.
+// Returns the weight w of the pullback (the upstream gradient, or the residual). Rarely taken, so it gathers lean: it must not
 // set the register budget of the whole kernel.
 template <int DP, int KIND>
 __device__ __forceinline__ float outlier_grad(const KArgs& a, const float* __restrict__ src, float* __restrict__ dst2,
@@ -593,19 +594,20 @@ __device__ __forceinline__ float outlier_grad(const KArgs& a, const float* __res
   float val = 0.f;
   if (KIND == 2) sample_val_grad<DP, true>(src, ax, ay, az, val, gx, gy, gz);
   else sample_grad<DP, true>(src, ax, ay, az, gx, gy, gz);
-  const float w = (KIND == 1) ? aux : val - aux;
-  if (KIND == 2 && isfinite(w)) scatter_val<DP>(dst2, ax, ay, az, w);
+  const float w = (KIND == 2) ? val - aux : aux;
+  if (KIND >= 2 && isfinite(w)) scatter_val<DP>(dst2, ax, ay, az, w);
   return w;
 }
 
-// LDS box of a gather kind, in floats. The objective's box is twice as large: it first
-// holds the staged image (float) and then, reused, the int64 scatter accumulator.
+// LDS box of a gather kind, in floats. The scattering kinds' box is twice as large: it
+// first holds the staged image (float) and then, reused, the int64 scatter accumulator.
 template <int CAP, int KIND>
 constexpr int box_floats() {
-  return KIND == 2 ? 2 * CAP : CAP;
+  return KIND >= 2 ? 2 * CAP : CAP;
 }
 
-// One tile of pull (KIND 0), the pull pullback (1) or the fused objective (2).
+// One tile of pull (KIND 0), the pull pullback (1), the fused objective (2) or the VJP
+// of pull (3: dphi = sum_c delta_c grad f_c and df0 += push(delta) from one staging).
 // box: box_floats<CAP, KIND>() floats (8-byte aligned); wsum: kTB/64 floats.
 // The pullback of every sample is summed over channels in registers (gsum), and the
 // objective keeps each sample's residual in registers between its two passes.
@@ -631,8 +633,8 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
   float lsum = 0.f;
   for (int c = 0; c < a.nchan; ++c) {
     const float* src = a.f + bb * a.f_bs + c * a.f_cs;
-    unsigned rfin = 0u;  // KIND 2: in-window samples with a finite residual (scattered via LDS)
-    float res[VPT];      // KIND 2: residual of each sample
+    unsigned rfin = 0u;  // KIND 2/3: in-window samples with a finite weight (scattered via LDS)
+    float res[VPT];      // KIND 2/3: residual / upstream gradient of each sample
 #pragma unroll
     for (int q = 0; q < VPT; ++q) {
       if (c > 0)  // loaded before staging so both overlap
@@ -679,8 +681,8 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
         LSample<DP> L;
         lsample<DP, true>(x, y, z, S, L);
         lds_grad<DP, KIND == 2>(box, L, BX, BXY, val, gx, gy, gz);
-        w = (KIND == 1) ? aux : val - aux;
-        if (KIND == 2 && isfinite(w)) {
+        w = (KIND == 2) ? val - aux : aux;
+        if (KIND >= 2 && isfinite(w)) {
           res[0] = w;
           rfin |= 1u << q;
         }
@@ -694,7 +696,7 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
     }
     if (lds) {
       __syncthreads();
-      if (KIND == 2) {  // scatter the residuals of the in-window samples through the int64 box
+      if (KIND >= 2) {  // scatter the residuals (gradients) of the in-window samples via the int64 box
         float vmax = 0.f;
 #pragma unroll
         for (int q = 0; q < VPT; ++q)
@@ -784,6 +786,9 @@ static hipError_t launch_shape(Op op, const KArgs& a, int nbatch, hipStream_t s)
       break;
     case OP_OBJGRAD:
       hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, cap_cells<DP, 2>(), 2>), grid, block, 0, s, a);
+      break;
+    case OP_PULL_VJP:
+      hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, cap_cells<DP, 2>(), 3>), grid, block, 0, s, a);
       break;
     case OP_PUSH:
       hipLaunchKernelGGL((k_push_tile<DP, TX, TY, TZ, cap_cells<DP, 3>(), false>), grid, block, 0, s, a);
@@ -894,7 +899,8 @@ hipError_t launch_batch_group(Op op, int dp, const BatchVol* dvols, int nvol, co
 }
 
 hipError_t launch_tiled(Op op, const KArgs& a, int nbatch, hipStream_t s, TileCfg cfg) {
-  if (op != OP_PULL && op != OP_PULL_BWD && op != OP_OBJGRAD && op != OP_PUSH && op != OP_PUSHG)
+  if (op != OP_PULL && op != OP_PULL_BWD && op != OP_OBJGRAD && op != OP_PULL_VJP && op != OP_PUSH &&
+      op != OP_PUSHG)
     return hipErrorNotSupported;
   if (a.n1 <= 0 || nbatch <= 0) return hipSuccess;
   if (a.dpx != a.dpy || a.dpx != a.dpz) return hipErrorNotSupported;

@@ -7,9 +7,11 @@ ChainRulesCore rrules in src/pushpull_pb.jl:11-51.
 
 The phi pullbacks are single fused kernels (pp_pull_backward / pp_pull_grad_backward):
 the 3C-channel pull_grad the Julia rule materialises before contracting
-(src/pushpull_pb.jl:113-117) is never written to memory. When both gradients are wanted
-on the GPU, the pullback runs on a second stream while the other gradient runs on the
-current one (the two are independent).
+(src/pushpull_pb.jl:113-117) is never written to memory. When both gradients of pull are
+wanted, one launch computes them (pp_pull_vjp: each tile of f0 is staged once, phi and
+delta are read once). For pull_grad and push, when both gradients are wanted on the GPU,
+the pullback runs on a second stream while the other gradient runs on the current one
+(the two are independent).
 """
 from __future__ import annotations
 
@@ -75,7 +77,12 @@ class _Pull(torch.autograd.Function):
                                      delta.reshape(_as5(delta)), ctx.sett).reshape(phi.shape)
 
         if ctx.needs_input_grad[0] and ctx.needs_input_grad[1]:
-            df0, dphi = _both(d_f0, d_phi, (f0, phi, delta))
+            if f0.is_cuda:  # both cotangents from one launch (pp_pull_vjp)
+                df0, dphi = ops.pull_vjp(f0.reshape(_as5(f0)), phi.reshape(_as5(phi)),
+                                         delta.reshape(_as5(delta)), ctx.sett)
+                df0, dphi = df0.reshape(f0.shape), dphi.reshape(phi.shape)
+            else:
+                df0, dphi = _both(d_f0, d_phi, (f0, phi, delta))
         elif ctx.needs_input_grad[0]:
             df0 = d_f0()
         elif ctx.needs_input_grad[1]:

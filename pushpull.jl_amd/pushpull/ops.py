@@ -238,6 +238,28 @@ def pull_grad_backward(f0, phi, delta, sett=None):
     return _backward(3, f0, phi, delta, sett or Settings())
 
 
+def pull_vjp(f0, phi, delta, sett=None, df0=None, accumulate=False):
+    """Both cotangents of pull (the rrule, src/pushpull_pb.jl:11-17) from one launch:
+    df0 = push(delta, phi, size(f0)) and dphi = sum_c delta_c grad pull(f0)_c (device
+    tensors; df0 is zeroed first unless accumulate). Returns (df0, dphi)."""
+    sett = sett or Settings()
+    f0 = _f32(f0, "f₀")
+    phi = _f32(phi, "ϕ")
+    delta = _f32(delta, "δ")
+    _req(_is_dev(f0) and _is_dev(phi) and _is_dev(delta), "pull_vjp takes device tensors")
+    lead, d0 = _dims5(f0.shape, 3, 5, "f₀")
+    nb, d1 = _phi_dims(phi)
+    nc = f0.shape[-4] if f0.ndim >= 4 else 1
+    _req(tuple(delta.shape) == lead + d1, "δ must have the shape of pull(f0, phi)")
+    dphi = torch.empty(tuple(phi.shape), dtype=torch.float32, device=f0.device)
+    if df0 is None:
+        df0 = torch.empty(tuple(f0.shape), dtype=torch.float32, device=f0.device)
+    d = _desc(sett, d0, d1, nc, nb)
+    _lib.check(_lib.lib().pp_pull_vjp(ct.byref(d), _ptr(dphi), _ptr(df0), _ptr(phi), _ptr(f0), _ptr(delta),
+                                      int(accumulate), _stream(f0)), "pp_pull_vjp")
+    return df0, dphi
+
+
 def objective_grad(f0, phi, g, sett=None, df0=None, accumulate=False):
     """Fused L = 0.5*||pull(f0,phi) - g||^2 with dL/dphi and dL/df0 (device tensors).
 

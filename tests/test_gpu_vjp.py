@@ -1,0 +1,107 @@
+"""pp_pull_vjp (both cotangents of pull from one launch) against the FP64 oracle's push and
+phi pullback, and against the separate GPU entry points (needs a GPU).
+
+Bar: NaN masks identical; finite entries within 1e-5 norm-wise relative error of the FP64
+oracle; the fused and the separate GPU calls agree to the same bar.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import S, check_close, identity, smooth_field
+
+torch = pytest.importorskip("torch")
+pp = pytest.importorskip("pushpull")
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def T(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+def H(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.fixture(params=["tiled", "per-sample"])
+def path(request):
+    old = pp._lib.get_option(pp._lib.PP_OPT_TILING)
+    old_t = pp._lib.get_option(pp._lib.PP_OPT_USE_TUNED)
+    pp._lib.set_option(pp._lib.PP_OPT_TILING, 2 if request.param == "tiled" else 0)
+    pp._lib.set_option(pp._lib.PP_OPT_USE_TUNED, 0)
+    yield request.param
+    pp._lib.set_option(pp._lib.PP_OPT_TILING, old)
+    pp._lib.set_option(pp._lib.PP_OPT_USE_TUNED, old_t)
+
+
+def _case(rng, d, C, maxdisp, out_frac=0.0):
+    f = np.stack([smooth_field(rng, d, 2.0, 1.0)[0] for _ in range(C)])[None].astype(np.float32)
+    phi = (identity(d) + smooth_field(rng, d, 4.0, maxdisp))[None].astype(np.float32)
+    if out_frac:
+        flat = phi.reshape(3, -1)
+        idx = rng.choice(flat.shape[1], size=int(out_frac * flat.shape[1]), replace=False)
+        flat[rng.integers(0, 3, size=idx.size), idx] = rng.choice([-20.0, 70.0], size=idx.size).astype(np.float32)
+    delta = np.stack([smooth_field(rng, d, 2.0, 1.0)[0] for _ in range(C)])[None].astype(np.float32)
+    return f, phi, delta
+
+
+@pytest.mark.parametrize("deg", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("bnd", [0, 1, 2])
+def test_pull_vjp_matches_oracle(deg, bnd, path):
+    rng = np.random.default_rng(100 + 7 * deg + bnd)
+    d = (30, 34, 38)
+    f, phi, delta = _case(rng, d, 2, 5.0, out_frac=0.03)
+    s, so = pp.Settings(deg, bnd, 1), S(deg, bnd, 1)
+    df0, dphi = pp.pull_vjp(T(f), T(phi), T(delta), s)
+    sc = float(np.abs(delta).max())
+    check_close(H(df0), oracle.push(delta, phi, d, so, prec="f64"), f"vjp df0 deg {deg} bnd {bnd}", scale=sc)
+    check_close(H(dphi), oracle.backward_phi(f, phi, delta, so), f"vjp dphi deg {deg} bnd {bnd}")
+
+
+@pytest.mark.parametrize("deg", [1, 2, 3])
+def test_pull_vjp_matches_separate_calls(deg, path):
+    """Fused = pp_push + pp_pull_backward on the same inputs, with ext=0 (invalid samples
+    give NaN in dphi and are skipped by push), non-finite entries of delta (skipped by
+    push, NaN in dphi), per-channel boundary triples and a batch of two."""
+    rng = np.random.default_rng(300 + deg)
+    d = (24, 28, 33)
+    f, phi, delta = _case(rng, d, 3, 6.0)
+    f = np.concatenate([f, -0.5 * f])
+    phi = np.concatenate([phi, phi[:, :, ::-1].copy()])
+    delta = np.concatenate([delta, delta[:, :, :, ::-1].copy()])
+    delta[0, 1, 3, 4, 5] = np.nan
+    delta[1, 2, 7, 8, 9] = np.inf
+    s = pp.Settings(deg, [(1, 1, 1), (0, 2, 1), (2, 2, 2)], 0)
+    df0, dphi = pp.pull_vjp(T(f), T(phi), T(delta), s)
+    ref_df0 = pp.push(T(delta), T(phi), d, s)
+    ref_dphi = pp.pull_backward(T(f), T(phi), T(delta), s)
+    check_close(H(df0), H(ref_df0), f"vjp df0 vs push deg {deg}", scale=1.0)
+    check_close(H(dphi), H(ref_dphi), f"vjp dphi vs pull_backward deg {deg}")
+    assert np.isnan(H(dphi)).any() and np.isfinite(H(df0)).all()
+
+
+def test_pull_vjp_accumulates():
+    rng = np.random.default_rng(5)
+    d = (20, 21, 22)
+    f, phi, delta = _case(rng, d, 1, 4.0)
+    s = pp.Settings(3, 1, 1)
+    base = torch.full(f.shape, 2.0, device=DEV)
+    df0, _ = pp.pull_vjp(T(f), T(phi), T(delta), s, df0=base.clone(), accumulate=True)
+    ref, _ = pp.pull_vjp(T(f), T(phi), T(delta), s)
+    check_close(H(df0), H(ref) + 2.0, "vjp accumulate", scale=2.0)
+
+
+def test_autograd_pull_uses_vjp_and_matches_oracle():
+    """torch.autograd through pull with both inputs requiring gradients (the fused
+    path) against the FP64 oracle's push and pullback."""
+    rng = np.random.default_rng(9)
+    d = (26, 25, 27)
+    f, phi, delta = _case(rng, d, 2, 4.0)
+    s, so = pp.Settings(2, 1, 1), S(2, 1, 1)
+    tf = T(f).requires_grad_(True)
+    tp = T(phi).requires_grad_(True)
+    pp.pull(tf, tp, s).backward(T(delta))
+    check_close(H(tf.grad), oracle.push(delta, phi, d, so, prec="f64"), "autograd df0", scale=1.0)
+    check_close(H(tp.grad), oracle.backward_phi(f, phi, delta, so), "autograd dphi")
