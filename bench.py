@@ -8,11 +8,13 @@ smooth deformation (identity + displacement of max 6 voxels, reflect boundary, e
 i.e. 48 + 24C = 72 bytes/voxel of compulsory HBM traffic per degree. Inputs are resident
 in HBM before timing. value = 3 * 256^3 voxels / step time, in Gvoxel/s.
 
-The three ops of a degree are independent, so by default (--schedule overlap) the pullback
-runs on a second HIP stream beside pull and push, joined at the end of each step: the
-gathers' VALU-bound tiles then share the CUs with push's LDS-bound tiles. The per-op
-breakdown comes from the same number of serial steps run afterwards (--schedule serial
-times those instead).
+push(delta) and the pullback are the two cotangents of pull's rrule (src/pushpull_pb.jl:11-17):
+by default (--schedule vjp) they come from one launch (pp_pull_vjp, which stages each tile of
+f once and reads phi and delta once for both), on a second HIP stream beside pull (the two
+are independent), joined at the end of each step. --schedule overlap runs the three
+separate entry points (pull and push on the launch stream, the pullback beside them);
+--schedule serial the nine separate launches in order. The per-op breakdown (separate
+entry points, and the VJP) comes from serial steps run after the timed ones.
 
 The CPU baseline is the reference's own C (artifacts/C/pushpull.c, compiled from source
 into oracle/_ref by oracle/Makefile, single-threaded as shipped) timed on this host on a
@@ -37,9 +39,12 @@ sys.path.insert(0, os.path.join(ROOT, "pushpull.jl_amd"))
 N = 256
 DEGS = (1, 2, 3)
 HBM_PEAK_GBS = 8000.0
-BYTES = {"pull": 20, "push": 20, "backward": 32}  # per voxel at C = 1
-TILED_NAMES = {"pull": "k_gather_tile<pull>", "push": "k_push_tile", "backward": "k_gather_tile<pullback>"}
-SAMPLE_NAMES = {"pull": "k_pull", "push": "k_push", "backward": "k_pull_bwd"}
+# per voxel at C = 1; the VJP (push and pullback from one launch) reads phi, f and delta
+# once and writes dphi and df: 12 + 4 + 4 + 12 + 4
+BYTES = {"pull": 20, "push": 20, "backward": 32, "vjp": 36}
+TILED_NAMES = {"pull": "k_gather_tile<pull>", "push": "k_push_tile", "backward": "k_gather_tile<pullback>",
+               "vjp": "k_gather_tile<vjp>"}
+SAMPLE_NAMES = {"pull": "k_pull", "push": "k_push", "backward": "k_pull_bwd", "vjp": "k_pull_vjp"}
 
 
 def kernel_name(op, deg, tiling):
@@ -559,9 +564,10 @@ def main():
                     help="skip the batched-uneven and matching-loop legs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run k_fill and k_axpy over 64 Mi floats (reference kernels for HBM counter scaling)")
-    ap.add_argument("--schedule", choices=["overlap", "serial"], default="overlap",
-                    help="overlap: the pullback runs on a second stream beside pull and push; "
-                         "serial: the nine launches in order on one stream")
+    ap.add_argument("--schedule", choices=["vjp", "overlap", "serial"], default="vjp",
+                    help="vjp: pull on the launch stream, push and pullback from one pp_pull_vjp launch "
+                         "on a second stream; overlap: the pullback runs on a second stream beside pull and "
+                         "push; serial: the nine separate launches in order on one stream")
     ap.add_argument("--leg", choices=["regulariser", "affine", "exp", "invert_def", "batched"], default=None,
                     help="run only this secondary leg and print its JSON (for profiling)")
     ap.add_argument("--reg-op", choices=["all", *REG_OPS], default="all",
@@ -632,6 +638,20 @@ def main():
         e2.record(side)
         main_s.wait_event(e2)
 
+    def step_vjp():
+        # pull on the launch stream; both cotangents (push(delta), pullback) from one VJP launch
+        # on a second stream; joined at the end of the step
+        e = torch.cuda.Event()
+        e.record(main_s)
+        side.wait_event(e)
+        for deg in DEGS:
+            d = ct.byref(descs[deg])
+            _lib.check(L.pp_pull(d, P(f1), P(phi), P(f), stream), "pull")
+            _lib.check(L.pp_pull_vjp(d, P(dphi), P(df), P(phi), P(f), P(delta), 0, side_h), "vjp")
+        e2 = torch.cuda.Event()
+        e2.record(side)
+        main_s.wait_event(e2)
+
     def step(events=None):
         for deg in DEGS:
             d = ct.byref(descs[deg])
@@ -661,7 +681,7 @@ def main():
         def add(self, deg, m):
             self.rows.append((deg, m))
 
-    timed_step = step_overlap if args.schedule == "overlap" else step
+    timed_step = {"vjp": step_vjp, "overlap": step_overlap, "serial": step}[args.schedule]
     _progress("warm-up")
     for _ in range(args.warmup):
         timed_step()
@@ -676,22 +696,29 @@ def main():
     t0 = time.perf_counter()
     start.record()
     for _ in range(args.steps):
-        if args.schedule == "overlap":
-            step_overlap()
-        else:
+        if args.schedule == "serial":
             step(ev)
+        else:
+            timed_step()
     end.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     gc.enable()
     ms_step = start.elapsed_time(end) / args.steps
-    if args.schedule == "overlap":  # per-op times come from separate serial steps
+    if args.schedule != "serial":  # per-op times come from separate serial steps
         _progress("serial steps for the per-op breakdown")
         for _ in range(args.steps):
             step(ev)
         torch.cuda.synchronize()
     ms_serial = float(np.sum([m[0].elapsed_time(m[3]) for _, m in ev.rows])) / args.steps
 
+    vjp_ms = {}
+    for deg in DEGS:  # the VJP per degree, zero fill of df included (as push's row)
+        dd = ct.byref(descs[deg])
+        vjp_ms[deg] = _time_ms_median(
+            torch, lambda: _lib.check(L.pp_pull_vjp(dd, P(dphi), P(df), P(phi), P(f), P(delta), 0, stream), "vjp"),
+            args.steps)
+    timed_ops = ("pull", "vjp") if args.schedule == "vjp" else ("pull", "push", "backward")
     per = {deg: {"pull": [], "push": [], "backward": []} for deg in DEGS}
     for deg, m in ev.rows:
         per[deg]["pull"].append(m[0].elapsed_time(m[1]))
@@ -702,27 +729,32 @@ def main():
     for deg in DEGS:
         row = {}
         tri = 0.0
-        for op in ("pull", "push", "backward"):
-            ms = float(np.mean(per[deg][op]))
-            tri += ms
+        for op in ("pull", "push", "backward", "vjp"):
+            ms = vjp_ms[deg] if op == "vjp" else float(np.mean(per[deg][op]))
+            if op != "vjp":
+                tri += ms
             gbs = BYTES[op] * n1 / (ms * 1e-3) / 1e9
             row[op] = {"ms": round(ms, 4), "GB_s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
-            if dominant is None or ms > dominant[2]:
+            if op in timed_ops and (dominant is None or ms > dominant[2]):
                 dominant = (deg, op, ms, gbs)
         row["triple_ms"] = round(tri, 4)
         row["Gvoxel_s"] = round(n1 / (tri * 1e-3) / 1e9, 2)
         row["triple_frac"] = round(72 * n1 / (tri * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        pv = float(np.mean(per[deg]["pull"])) + vjp_ms[deg]
+        row["pull_plus_vjp_ms"] = round(pv, 4)
+        row["pull_plus_vjp_Gvoxel_s"] = round(n1 / (pv * 1e-3) / 1e9, 2)
         detail[f"deg{deg}"] = row
 
     value = len(DEGS) * n1 / (ms_step * 1e-3) / 1e9
     deg, op, ms_op, _ = dominant
-    # The dominant op's kernel alone (push without its zero fill), timed with events on the
+    # The dominant op's kernel alone (push and the VJP without their zero fill), timed with events on the
     # stream it is launched on, as rocprofv3 --kernel-trace reports the same kernel.
     d = ct.byref(descs[deg])
     kcall = {
         "pull": lambda: L.pp_pull(d, P(f1), P(phi), P(f), stream),
         "push": lambda: L.pp_push(d, P(df), P(phi), P(delta), 1, stream),
         "backward": lambda: L.pp_pull_backward(d, P(dphi), P(phi), P(f), P(delta), stream),
+        "vjp": lambda: L.pp_pull_vjp(d, P(dphi), P(df), P(phi), P(f), P(delta), 1, stream),
     }[op]
     ms = _time_ms(torch, lambda: _lib.check(kcall(), op), 20, None)
     gbs = BYTES[op] * n1 / (ms * 1e-3) / 1e9
@@ -742,7 +774,9 @@ def main():
         "config": {"workload": "pull+push+backward, 256^3 x 1 channel, degrees 1,2,3 (one triple each per step), "
                                "reflect boundary, ext=1", "volume": [N, N, N], "channels": 1, "degrees": list(DEGS),
                    "kernels": {0: "per-sample", 1: "automatic", 2: "tiled"}[args.tiling], "tile_shape": args.shape,
-                   "schedule": {"overlap": "pull and push on the launch stream, pullback on a second stream, "
+                   "schedule": {"vjp": "pull on the launch stream; push and pullback from one pp_pull_vjp launch "
+                                       "on a second stream; joined per step",
+                                "overlap": "pull and push on the launch stream, pullback on a second stream, "
                                            "joined per step",
                                 "serial": "nine launches in order on one stream"}[args.schedule]},
         "roofline": {"bound": "hbm", "kernel": f"{kernel_name(op, deg, args.tiling)} deg{deg}", "achieved": round(gbs, 1),
