@@ -472,9 +472,10 @@ def cpu_regulariser_baseline(res, budget_s=8.0):
 def bench_matching(torch, pp, iters=50, alpha=0.02, seed=5, check=True):
     """8 subjects at 256^3, 4 channels, degree 3: `iters` iterations of objective and
     gradient (pull, residual, push, pullback) with phi <- phi - alpha * dL/dphi, all
-    subjects batched and each iteration replayed from a HIP graph. The FP64 oracle
-    re-evaluates dL/dphi of subject 0 on a 256x256x8 slab from the GPU's own inputs of
-    iterations 1 and `iters`."""
+    subjects batched and each iteration replayed from a HIP graph. Host copies of subject
+    0's inputs and dL/dphi on a 256x256x8 slab at iterations 1 and `iters` are returned
+    under "_fp64_pending" for the CPU leg, where the FP64 oracle re-evaluates dL/dphi from
+    the GPU's own inputs (fp64_checks)."""
     from pushpull import _lib
     from pushpull import batch as ppb
 
@@ -504,22 +505,13 @@ def bench_matching(torch, pp, iters=50, alpha=0.02, seed=5, check=True):
     graph = ppb.Graph(iteration)
     st = graph.stream
     _progress("matching loop: captured")
-    errs = {}
+    pending = []
     z0 = 124
 
-    def oracle_check(tag, phi_in):
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle
-
-        class _S:
-            deg, bnd, ext = (3, 3, 3), [(1, 1, 1)], 1
-
-        f0 = f[0].cpu().numpy()
-        ph = phi_in[:, z0:z0 + 8].cpu().numpy()
-        r = oracle.pull(f0[None], ph[None], _S, prec="f64")[0] - g[0, :, z0:z0 + 8].cpu().double().numpy()
-        ref = oracle.backward_phi(f0[None], ph[None], r[None], _S)[0]
-        got = dphi[0, :, z0:z0 + 8].cpu().double().numpy()
-        errs[tag] = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    def stash(tag, phi_in):
+        pending.append(dict(kind="objective_dphi", tag=tag, deg=3, f0=f[0].cpu().numpy(),
+                            phi=phi_in[:, z0:z0 + 8].cpu().numpy(), g=g[0, :, z0:z0 + 8].cpu().numpy(),
+                            got=dphi[0, :, z0:z0 + 8].cpu().numpy()))
 
     losses = []
     with torch.cuda.stream(st):
@@ -528,7 +520,7 @@ def bench_matching(torch, pp, iters=50, alpha=0.02, seed=5, check=True):
         st.synchronize()
         losses.append(float(loss.sum()))
         if check:
-            oracle_check("iteration_1", phi_first)
+            stash("iteration_1", phi_first)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(st)
         for _ in range(iters - 2):
@@ -541,14 +533,77 @@ def bench_matching(torch, pp, iters=50, alpha=0.02, seed=5, check=True):
         st.synchronize()
         losses.append(float(loss.sum()))
         if check:
-            oracle_check(f"iteration_{iters}", phi_last)
+            stash(f"iteration_{iters}", phi_last)
     graph.close()
     plan.close()
     chan_vox = B * C * N ** 3
     return {"subjects": B, "channels": C, "iterations": iters, "ms_per_iteration": round(t_ms, 3),
             "Gvoxel_s": round(chan_vox / (t_ms * 1e-3) / 1e9, 3),
             "hbm_frac": round((24 + 12 * C) * B * N ** 3 / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "loss_first_last": [losses[0], losses[-1]], "rel_err_vs_fp64_dphi_slab": errs}
+            "loss_first_last": [losses[0], losses[-1]], "_fp64_pending": pending}
+
+
+def gpu_slab_outputs(torch, pp, L, f, phi, delta, z0=124, nz=8):
+    """GPU results of the headline ops on a 256x256x`nz` slab of the deformation (pull,
+    push, pullback and the VJP, degrees 1-3), as host copies for the FP64 check of the CPU
+    leg (fp64_checks)."""
+    from pushpull import _lib, ops
+
+    P = lambda t: ct.c_void_p(t.data_ptr())  # noqa: E731
+    st = ct.c_void_p(torch.cuda.current_stream().cuda_stream)
+    ph = phi[:, z0:z0 + nz].contiguous()
+    dl = delta[z0:z0 + nz].contiguous()
+    out = []
+    for deg in DEGS:
+        d = ct.byref(ops._desc(pp.Settings(deg, 1, 1), (N, N, N), (nz, N, N), 1, 1))
+        f1, dphi, df = torch.empty_like(dl), torch.empty_like(ph), torch.empty_like(f)
+        vdphi, vdf = torch.empty_like(ph), torch.empty_like(f)
+        _lib.check(L.pp_pull(d, P(f1), P(ph), P(f), st), "pull")
+        _lib.check(L.pp_push(d, P(df), P(ph), P(dl), 0, st), "push")
+        _lib.check(L.pp_pull_backward(d, P(dphi), P(ph), P(f), P(dl), st), "backward")
+        _lib.check(L.pp_pull_vjp(d, P(vdphi), P(vdf), P(ph), P(f), P(dl), 0, st), "vjp")
+        torch.cuda.synchronize()
+        out.append(dict(kind="headline", deg=deg, pull=f1.cpu().numpy(), push=df.cpu().numpy(),
+                        backward=dphi.cpu().numpy(), vjp_df=vdf.cpu().numpy(), vjp_dphi=vdphi.cpu().numpy()))
+    return dict(f=f.cpu().numpy(), phi=ph.cpu().numpy(), delta=dl.cpu().numpy(), rows=out,
+                sample=f"256x256x{nz} slab of the deformation (z {z0}..{z0 + nz - 1}), all 256^3 of the image")
+
+
+def fp64_checks(headline, pending):
+    """CPU leg: norm-wise relative error of the GPU results against the FP64 oracle, on the
+    slabs the GPU legs kept (the oracle is the checker here, never the thing measured)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    def rel(got, ref):
+        ref = np.asarray(ref, np.float64)
+        got = np.asarray(got, np.float64).reshape(ref.shape)  # (B, C) leading dims of the oracle
+        m = np.isfinite(ref)
+        return float(np.linalg.norm(got[m] - ref[m]) / np.linalg.norm(ref[m]))
+
+    class _S:
+        def __init__(self, deg):
+            self.deg, self.bnd, self.ext = (deg,) * 3, [(1, 1, 1)], 1
+
+    out = {}
+    if headline is not None:
+        f, ph, dl = headline["f"], headline["phi"], headline["delta"]
+        errs = {}
+        for r in headline["rows"]:
+            s = _S(r["deg"])
+            ref_pull = oracle.pull(f, ph, s, prec="f64")
+            ref_push = oracle.push(dl, ph, f.shape, s, prec="f64")
+            ref_bwd = oracle.backward_phi(f, ph, dl, s)
+            errs[f"deg{r['deg']}"] = {"pull": rel(r["pull"], ref_pull), "push": rel(r["push"], ref_push),
+                                      "backward": rel(r["backward"], ref_bwd),
+                                      "vjp_df": rel(r["vjp_df"], ref_push), "vjp_dphi": rel(r["vjp_dphi"], ref_bwd)}
+        out["headline"] = {"sample": headline["sample"], "rel_err": errs}
+    for p in pending:
+        s = _S(p["deg"])
+        r = oracle.pull(p["f0"][None], p["phi"][None], s, prec="f64")[0] - p["g"].astype(np.float64)
+        ref = oracle.backward_phi(p["f0"][None], p["phi"][None], r[None], s)[0]
+        out.setdefault("matching_loop_dphi", {})[p["tag"]] = rel(p["got"], ref)
+    return out
 
 
 def main():
@@ -787,6 +842,9 @@ def main():
         "per_degree": detail,
         "wall_s": round(wall, 3),
     }
+    # GPU results on a slab, checked against the FP64 oracle in the CPU leg below
+    headline_slabs = None if args.no_cpu else gpu_slab_outputs(torch, pp, L, f, phi, delta)
+    pending = []
     if not args.no_extras:
         del f1, df, dphi
         _progress("batched uneven volumes")
@@ -794,6 +852,7 @@ def main():
         torch.cuda.empty_cache()
         _progress("matching loop")
         res["matching_loop"] = bench_matching(torch, pp)
+        pending = res["matching_loop"].pop("_fp64_pending", [])
         torch.cuda.empty_cache()
         _progress("Exp (scaling and squaring)")
         res["exp_256"] = bench_exp(torch, pp)
@@ -810,6 +869,11 @@ def main():
     if not args.no_cpu:
         _progress("cpu baseline")
         res["cpu_baseline"] = cpu_baseline(f, phi, delta)
+        _progress("FP64 oracle checks")
+        chk = fp64_checks(headline_slabs, pending)
+        res["err_vs_fp64"] = chk["headline"]
+        if "matching_loop_dphi" in chk:
+            res["matching_loop"]["rel_err_vs_fp64_dphi_slab"] = chk["matching_loop_dphi"]
         if not args.no_extras:
             _progress("cpu reference of the regulariser")
             res["regulariser_cpu_reference"] = cpu_regulariser_baseline(res)
