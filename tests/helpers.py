@@ -84,3 +84,15 @@ def check_close(got, ref, what, tol=1e-5, scale=None):
     den = float(np.linalg.norm(ref[m].astype(np.float64)))
     floor = 1e-6 * np.sqrt(max(1, int(m.sum()))) * scale if scale is not None else 0.0
     assert num <= tol * den + floor, f"{what}: rel err {num / den if den else num:.3g}"
+    return num / den if den else num
+
+
+def report(name, value):
+    """Append `name value` to the file named by PP_PARITY_LOG (when set): the measured
+    errors of the parity tests, for the summary of errors per workload."""
+    import os
+
+    path = os.environ.get("PP_PARITY_LOG")
+    if path:
+        with open(path, "a") as fh:
+            fh.write(f"{name} {float(value):.6g}\n")

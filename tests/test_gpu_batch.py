@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import S, identity, rel_err, same_nan_mask, smooth_field
+from helpers import S, identity, rel_err, report, same_nan_mask, smooth_field
 
 torch = pytest.importorskip("torch")
 pp = pytest.importorskip("pushpull")
@@ -52,6 +52,7 @@ def _dev(v, **extra):
 def _check(got, ref, what):
     assert same_nan_mask(got, ref), f"{what}: NaN mask differs"
     e = rel_err(got, ref)
+    report(f"batch {what}", e)
     assert e < TOL, f"{what}: rel err {e:.3g}"
 
 

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import S, case_settings, identity, rel_err, same_nan_mask, smooth_field
+from helpers import S, case_settings, identity, rel_err, report, same_nan_mask, smooth_field
 
 torch = pytest.importorskip("torch")
 pp = pytest.importorskip("pushpull")
@@ -64,6 +64,7 @@ def _check(got, ref, what, tol=TOL, scale=None):
     den = float(np.linalg.norm(ref[m].astype(np.float64)))
     floor = 1e-6 * np.sqrt(max(1, int(m.sum()))) * scale if scale is not None else 0.0
     assert num <= tol * den + floor, f"{what}: rel err {num / den if den else num:.3g}"
+    return num / den if den else num
 
 
 def _scale(a):
@@ -170,13 +171,15 @@ def test_config2_pull_push_adjoint(deg, bnd, path):
     f1 = H(pp.pull(tf, tphi, s))
     r32 = oracle.pull(f, phi, so)[0, 0]
     r64 = oracle.pull(f, phi, so, prec="f64")[0, 0]
-    _check(f1, r32, "pull vs f32 ref")
-    _check(f1, r64, "pull vs f64")
+    tag = f"config2 deg{deg} bnd{bnd} {path}"
+    report(f"{tag} pull vs f32 reference", _check(f1, r32, "pull vs f32 ref"))
+    report(f"{tag} pull vs f64", _check(f1, r64, "pull vs f64"))
     f0 = H(pp.push(tg, tphi, d, s))
     p64 = oracle.push(g, phi, d, so, prec="f64")[0, 0]
-    _check(f0, p64, "push vs f64")
+    report(f"{tag} push vs f64", _check(f0, p64, "push vs f64"))
     lhs = float(torch.sum(T(f1).double() * tg.double()))
     rhs = float(torch.sum(tf.double() * T(f0).double()))
+    report(f"{tag} adjoint <pull f, g> vs <f, push g>", abs(lhs - rhs) / (abs(lhs) + abs(rhs)))
     assert abs(lhs - rhs) / (abs(lhs) + abs(rhs)) < TOL
     # FP64 oracle adjointness (pins push as the exact adjoint of pull)
     lhs64 = float(np.sum(r64 * g.astype(np.float64)))
@@ -194,7 +197,8 @@ def test_config1_nan_mask_ext0(path):
     got = H(pp.pull(T(f), T(phi), s))
     ref = oracle.pull(f, phi, S(1, 1, 0))[0, 0]
     assert np.isnan(ref).sum() > 0
-    _check(got, ref, "config 1 pull")
+    report(f"config1 pull ext0 vs f32 reference {path}", _check(got, ref, "config 1 pull"))
+    report(f"config1 pull ext0 vs f64 {path}", _check(got, oracle.pull(f, phi, S(1, 1, 0), prec="f64")[0, 0], "config 1 f64"))
 
 
 @pytest.mark.parametrize("deg", [1, 2, 3])

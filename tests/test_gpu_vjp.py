@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import S, check_close, identity, smooth_field
+from helpers import S, check_close, identity, report, smooth_field
 
 torch = pytest.importorskip("torch")
 pp = pytest.importorskip("pushpull")
@@ -56,8 +56,10 @@ def test_pull_vjp_matches_oracle(deg, bnd, path):
     s, so = pp.Settings(deg, bnd, 1), S(deg, bnd, 1)
     df0, dphi = pp.pull_vjp(T(f), T(phi), T(delta), s)
     sc = float(np.abs(delta).max())
-    check_close(H(df0), oracle.push(delta, phi, d, so, prec="f64"), f"vjp df0 deg {deg} bnd {bnd}", scale=sc)
-    check_close(H(dphi), oracle.backward_phi(f, phi, delta, so), f"vjp dphi deg {deg} bnd {bnd}")
+    e0 = check_close(H(df0), oracle.push(delta, phi, d, so, prec="f64"), f"vjp df0 deg {deg} bnd {bnd}", scale=sc)
+    e1 = check_close(H(dphi), oracle.backward_phi(f, phi, delta, so), f"vjp dphi deg {deg} bnd {bnd}")
+    report(f"vjp deg{deg} bnd{bnd} {path} df0 vs f64", e0)
+    report(f"vjp deg{deg} bnd{bnd} {path} dphi vs f64", e1)
 
 
 @pytest.mark.parametrize("deg", [1, 2, 3])
