@@ -239,6 +239,7 @@ int pp_relax(const pp_sparse_kernel *kernel, float *v, const float *g, const flo
  * it can be captured into a HIP graph. */
 enum {
   PP_OP_PULL = 0,          /* out = pull(f, phi)                                   */
+  PP_OP_PULL_GRAD = 1,     /* out = pull_grad(f, phi)                              */
   PP_OP_PUSH = 3,          /* out = push(f, phi) (zeroed first)                    */
   PP_OP_PUSH_GRAD = 4,     /* out = push_grad(f, phi) (zeroed first)               */
   PP_OP_PULL_BACKWARD = 5, /* out = dphi = sum_c aux_c grad pull(f)_c              */
@@ -299,11 +300,12 @@ int pp_set_option(int key, int value);
 int pp_get_option(int key);
 
 /* ---- autotuner (no counterpart in the reference) ----------------------------
- * Times every kernel variant of `op` (PP_OP_PULL, _PUSH, _PUSH_GRAD, _PULL_BACKWARD,
- * _OBJECTIVE) on the caller's buffers -- one warm-up and `reps` timed calls each, HIP
- * events on `stream` -- and records the fastest for (op, degree, log2 of the phi-grid
- * size); later calls with that key use it. The buffers are those of the op's Tier B
- * function (out2/loss only for the objective, aux for the pullback and the objective).
+ * Times every kernel variant of `op` (PP_OP_PULL, _PULL_GRAD, _PUSH, _PUSH_GRAD,
+ * _PULL_BACKWARD, _OBJECTIVE, _PULL_VJP) on the caller's buffers -- one warm-up and `reps`
+ * timed calls each, HIP events on `stream` -- and records the fastest for (op, degree,
+ * log2 of the phi-grid size); later calls with that key use it. The buffers are those of
+ * the op's Tier B function (out2 for the objective and the VJP, loss for the objective,
+ * aux for the pullback, the objective and the VJP).
  * *variant receives 0 (one work-item per sample) or 1+k (tiled, tile shape k). */
 int pp_tune(int op, const pp_desc *d, float *out, float *out2, double *loss, const float *phi, const float *f,
             const float *aux, void *stream, int reps, int *variant);

@@ -88,7 +88,8 @@ hipError_t launch(pp::Op op, const pp::KArgs& a, int nbatch, hipStream_t s) {
   // Built-in choice: trilinear gathers (8 taps, served by L1/L2) run per sample, which
   // beats staging a box at that degree on gfx950; everything else runs tiled.
   const bool trilinear = a.dpx == 2 && a.dpy == 2 && a.dpz == 2;
-  if (trilinear && (op == pp::OP_PULL || op == pp::OP_PULL_BWD)) return launch_variant(op, a, nbatch, s, 0);
+  if (trilinear && (op == pp::OP_PULL || op == pp::OP_PULLG || op == pp::OP_PULL_BWD))
+    return launch_variant(op, a, nbatch, s, 0);
   return launch_variant(op, a, nbatch, s, tiled);
 }
 
@@ -980,6 +981,7 @@ int pp_tune(int op, const pp_desc* d, float* out, float* out2, double* loss, con
   auto call = [&]() -> int {
     switch (op) {
       case PP_OP_PULL: return pp_pull(d, out, phi, f, s);
+      case PP_OP_PULL_GRAD: return pp_pull_grad(d, out, phi, f, s);
       case PP_OP_PUSH: return pp_push(d, out, phi, f, 0, s);
       case PP_OP_PUSH_GRAD: return pp_push_grad(d, out, phi, f, 0, s);
       case PP_OP_PULL_BACKWARD: return pp_pull_backward(d, out, phi, f, aux, s);

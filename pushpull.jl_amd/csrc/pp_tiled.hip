@@ -538,8 +538,9 @@ __global__ __launch_bounds__(kTB) void k_push_tile(KArgs a) {
   push_tile<DP, TX, TY, TZ, CAP, GRAD>(a, xcd_block(blockIdx.x, gridDim.x), blockIdx.y, box, wsum, S);
 }
 
-// ---- gathers: pull (KIND 0), phi pullback of pull (1), fused objective (2) and the --
-// ---- vector-Jacobian product of pull (3): push(delta) and the phi pullback together --
+// ---- gathers: pull (KIND 0), phi pullback of pull (1), fused objective (2), the ------
+// ---- vector-Jacobian product of pull (3: push(delta) and the phi pullback together) --
+// ---- and pull_grad (4) ---------------------------------------------------------------
 template <int DP>
 __device__ __forceinline__ float lds_val(const float* box, const LSample<DP>& L, int BX, int BXY) {
   float acc = 0.f;
@@ -595,7 +596,7 @@ __device__ __forceinline__ float outlier_grad(const KArgs& a, const float* __res
   if (KIND == 2) sample_val_grad<DP, true>(src, ax, ay, az, val, gx, gy, gz);
   else sample_grad<DP, true>(src, ax, ay, az, gx, gy, gz);
   const float w = (KIND == 2) ? val - aux : aux;
-  if (KIND >= 2 && isfinite(w)) scatter_val<DP>(dst2, ax, ay, az, w);
+  if ((KIND == 2 || KIND == 3) && isfinite(w)) scatter_val<DP>(dst2, ax, ay, az, w);
   return w;
 }
 
@@ -603,11 +604,12 @@ __device__ __forceinline__ float outlier_grad(const KArgs& a, const float* __res
 // first holds the staged image (float) and then, reused, the int64 scatter accumulator.
 template <int CAP, int KIND>
 constexpr int box_floats() {
-  return KIND >= 2 ? 2 * CAP : CAP;
+  return (KIND == 2 || KIND == 3) ? 2 * CAP : CAP;
 }
 
-// One tile of pull (KIND 0), the pull pullback (1), the fused objective (2) or the VJP
-// of pull (3: dphi = sum_c delta_c grad f_c and df0 += push(delta) from one staging).
+// One tile of pull (KIND 0), the pull pullback (1), the fused objective (2), the VJP
+// of pull (3: dphi = sum_c delta_c grad f_c and df0 += push(delta) from one staging) or
+// pull_grad (4: the three components of grad f_c per channel).
 // box: box_floats<CAP, KIND>() floats (8-byte aligned); wsum: kTB/64 floats.
 // The pullback of every sample is summed over channels in registers (gsum), and the
 // objective keeps each sample's residual in registers between its two passes.
@@ -615,12 +617,13 @@ template <int DP, int TX, int TY, int TZ, int CAP, int KIND>
 __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, float* box, float* wsum,
                                             TileState& S) {
   constexpr int VPT = Shape<TX, TY, TZ>::VPT;
+  constexpr bool AUX = KIND >= 1 && KIND <= 3;  // reads an upstream gradient or target
   const float* phi = a.phi + bb * a.phi_bs;
   float auxv[VPT];  // upstream gradient / target; channel 0's is loaded before phi is read
   auto prefetch = [&](const Vox<VPT>& W) {
 #pragma unroll
     for (int q = 0; q < VPT; ++q)
-      auxv[q] = (KIND != 0 && ((W.inside >> q) & 1u)) ? a.aux[bb * a.aux_bs + W.v[q]] : 0.f;
+      auxv[q] = (AUX && ((W.inside >> q) & 1u)) ? a.aux[bb * a.aux_bs + W.v[q]] : 0.f;
   };
   Vox<VPT> V;
   tile_setup<DP, TX, TY, TZ, CAP>(a, phi, tile, V, S, prefetch);
@@ -638,7 +641,7 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
 #pragma unroll
     for (int q = 0; q < VPT; ++q) {
       if (c > 0)  // loaded before staging so both overlap
-        auxv[q] = (KIND != 0 && ((V.valid >> q) & 1u)) ? a.aux[bb * a.aux_bs + c * a.aux_cs + V.v[q]] : 0.f;
+        auxv[q] = (AUX && ((V.valid >> q) & 1u)) ? a.aux[bb * a.aux_bs + c * a.aux_cs + V.v[q]] : 0.f;
       res[q] = 0.f;
     }
     if (lds) {
@@ -651,6 +654,8 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
       const int v = V.v[0];
       if (!((V.valid >> q) & 1u)) {
         if (KIND == 0) a.out[bb * a.out_bs + c * a.out_cs + v] = __builtin_nanf("");
+        if (KIND == 4) put3(a.out + bb * a.out_bs + c * a.out_cs + v, n1, __builtin_nanf(""), __builtin_nanf(""),
+                            __builtin_nanf(""), 0);
         continue;
       }
       const float x = V.x[0], y = V.y[0], z = V.z[0];
@@ -675,6 +680,23 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
         a.out[bb * a.out_bs + c * a.out_cs + v] = val;
         continue;
       }
+      if (KIND == 4) {
+        float gx, gy, gz;
+        if (use_lds) {
+          LSample<DP> L;
+          lsample<DP, true>(x, y, z, S, L);
+          float val;
+          lds_grad<DP, false>(box, L, BX, BXY, val, gx, gy, gz);
+        } else {
+          Axis<DP> ax, ay, az;
+          make_axis<DP, kW | kDW>(x, DP, a.d0x, 1, a.bx, ax);
+          make_axis<DP, kW | kDW>(y, DP, a.d0y, a.d0x, a.by, ay);
+          make_axis<DP, kW | kDW>(z, DP, a.d0z, a.d0x * a.d0y, a.bz, az);
+          sample_grad<DP, true>(src, ax, ay, az, gx, gy, gz);
+        }
+        put3(a.out + bb * a.out_bs + c * a.out_cs + v, n1, gx, gy, gz, 0);
+        continue;
+      }
       const float aux = auxv[0];
       float val = 0.f, gx, gy, gz, w;
       if (use_lds) {
@@ -682,7 +704,7 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
         lsample<DP, true>(x, y, z, S, L);
         lds_grad<DP, KIND == 2>(box, L, BX, BXY, val, gx, gy, gz);
         w = (KIND == 2) ? val - aux : aux;
-        if (KIND >= 2 && isfinite(w)) {
+        if ((KIND == 2 || KIND == 3) && isfinite(w)) {
           res[0] = w;
           rfin |= 1u << q;
         }
@@ -696,7 +718,7 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
     }
     if (lds) {
       __syncthreads();
-      if (KIND >= 2) {  // scatter the residuals (gradients) of the in-window samples via the int64 box
+      if (KIND == 2 || KIND == 3) {  // scatter the residuals (gradients) of the in-window samples via the int64 box
         float vmax = 0.f;
 #pragma unroll
         for (int q = 0; q < VPT; ++q)
@@ -727,7 +749,7 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
       }
     }
   }
-  if (KIND != 0) {
+  if (AUX) {
     const float nan = __builtin_nanf("");
 #pragma unroll 1
     for (int q = 0; q < VPT; ++q, V.rotate(), rot_rows(gsum)) {
@@ -780,6 +802,9 @@ static hipError_t launch_shape(Op op, const KArgs& a, int nbatch, hipStream_t s)
   switch (op) {
     case OP_PULL:
       hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, cap_cells<DP, 0>(), 0>), grid, block, 0, s, a);
+      break;
+    case OP_PULLG:
+      hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, cap_cells<DP, 0>(), 4>), grid, block, 0, s, a);
       break;
     case OP_PULL_BWD:
       hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, cap_cells<DP, 1>(), 1>), grid, block, 0, s, a);
@@ -899,8 +924,8 @@ hipError_t launch_batch_group(Op op, int dp, const BatchVol* dvols, int nvol, co
 }
 
 hipError_t launch_tiled(Op op, const KArgs& a, int nbatch, hipStream_t s, TileCfg cfg) {
-  if (op != OP_PULL && op != OP_PULL_BWD && op != OP_OBJGRAD && op != OP_PULL_VJP && op != OP_PUSH &&
-      op != OP_PUSHG)
+  if (op != OP_PULL && op != OP_PULLG && op != OP_PULL_BWD && op != OP_OBJGRAD && op != OP_PULL_VJP &&
+      op != OP_PUSH && op != OP_PUSHG)
     return hipErrorNotSupported;
   if (a.n1 <= 0 || nbatch <= 0) return hipSuccess;
   if (a.dpx != a.dpy || a.dpx != a.dpz) return hipErrorNotSupported;

@@ -14,7 +14,7 @@ import torch
 from . import _lib, ops
 from .settings import Settings
 
-OPS = {"pull": 0, "push": 3, "push_grad": 4, "pull_backward": 5, "objective": 7}
+OPS = {"pull": 0, "pull_grad": 1, "push": 3, "push_grad": 4, "pull_backward": 5, "objective": 7, "pull_vjp": 8}
 VARIANTS = ("per-sample", "tiled 16x8x8", "tiled 32x8x4", "tiled 16x16x4")
 
 
@@ -40,12 +40,15 @@ def tune(op: str, f, phi, sett: Settings | None = None, aux=None, dim=None, reps
     out2 = loss = None
     if op == "pull":
         out = torch.empty((nb, C) + d1, device=dev)
+    elif op == "pull_grad":
+        out = torch.empty((nb, C, 3) + d1, device=dev)
     elif op in ("push", "push_grad"):
         out = torch.empty((nb, C) + d0, device=dev)
     else:
         out = torch.empty((nb, 3) + d1, device=dev)
-        if op == "objective":
+        if op in ("objective", "pull_vjp"):
             out2 = torch.empty((nb, C) + d0, device=dev)
+        if op == "objective":
             loss = torch.empty((), dtype=torch.float64, device=dev)
     v = ct.c_int(-1)
     s = ct.c_void_p(torch.cuda.current_stream(dev).cuda_stream)

@@ -13,7 +13,7 @@ from pushpull import tune as pt  # noqa: E402
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("op", ["pull", "push", "push_grad", "pull_backward", "objective"])
+@pytest.mark.parametrize("op", ["pull", "pull_grad", "push", "push_grad", "pull_backward", "objective", "pull_vjp"])
 def test_tune_picks_a_variant_and_results_stay_exact(op, tmp_path):
     rng = np.random.default_rng(3)
     d = (48, 40, 56)
@@ -38,6 +38,14 @@ def test_tune_picks_a_variant_and_results_stay_exact(op, tmp_path):
     if op == "pull":
         got = pp.pull(T(f), T(phi), s).cpu().numpy()
         ref = oracle.pull(f, phi, so, prec="f64")[0, 0]
+    elif op == "pull_grad":
+        got = pp.pull_grad(T(f), T(phi), s).cpu().numpy()
+        ref = oracle.pull_grad(f, phi, so, prec="f64")[0, 0]
+    elif op == "pull_vjp":
+        df, got = pp.pull_vjp(T(f), T(phi), T(g), s)
+        got = got.cpu().numpy()
+        ref = oracle.backward_phi(f, phi, g, so)[0]
+        assert rel_err(df.cpu().numpy(), oracle.push(g, phi, d, so, prec="f64")[0, 0]) < 1e-5
     elif op == "push":
         got = pp.push(T(g), T(phi), d, s).cpu().numpy()
         ref = oracle.push(g, phi, d, so, prec="f64")[0, 0]
