@@ -10,8 +10,9 @@ in HBM before timing. value = 3 * 256^3 voxels / step time, in Gvoxel/s.
 
 push(delta) and the pullback are the two cotangents of pull's rrule (src/pushpull_pb.jl:11-17):
 by default (--schedule vjp) they come from one launch (pp_pull_vjp, which stages each tile of
-f once and reads phi and delta once for both), on a second HIP stream beside pull (the two
-are independent), joined at the end of each step. --schedule overlap runs the three
+f once and reads phi and delta once for both); the six independent launches of a step (pull
+and the VJP per degree) run on three HIP streams balanced by their measured times, joined at
+the end of each step. --schedule overlap runs the three
 separate entry points (pull and push on the launch stream, the pullback beside them);
 --schedule serial the nine separate launches in order. The per-op breakdown (separate
 entry points, and the VJP) comes from serial steps run after the timed ones.
@@ -654,6 +655,9 @@ def main():
     f1 = torch.empty_like(f)
     df = torch.empty_like(f)
     dphi = torch.empty_like(phi)
+    # outputs of the launches that run beside those above in the default schedule
+    f12, f13, df2, df3 = (torch.empty_like(f) for _ in range(4))
+    dphi2, dphi3 = torch.empty_like(phi), torch.empty_like(phi)
     L = _lib.lib()
     _lib.set_option(_lib.PP_OPT_TILING, args.tiling)
     _lib.set_option(_lib.PP_OPT_TILE_SHAPE, args.shape)
@@ -675,6 +679,8 @@ def main():
 
     side = torch.cuda.Stream()  # default priority
     side_h = ct.c_void_p(side.cuda_stream)
+    side2 = torch.cuda.Stream()  # default priority
+    side2_h = ct.c_void_p(side2.cuda_stream)
     main_s = torch.cuda.current_stream()
 
     def step_overlap():
@@ -694,18 +700,24 @@ def main():
         main_s.wait_event(e2)
 
     def step_vjp():
-        # pull on the launch stream; both cotangents (push(delta), pullback) from one VJP launch
-        # on a second stream; joined at the end of the step
+        # the six independent launches of a step (pull and the VJP -- push(delta) and the
+        # pullback from one launch -- per degree) on three streams balanced by their measured
+        # times: VJP 3 | VJP 2, pull 1 | VJP 1, pull 2, pull 3; joined at the end of the step
         e = torch.cuda.Event()
         e.record(main_s)
         side.wait_event(e)
-        for deg in DEGS:
-            d = ct.byref(descs[deg])
-            _lib.check(L.pp_pull(d, P(f1), P(phi), P(f), stream), "pull")
-            _lib.check(L.pp_pull_vjp(d, P(dphi), P(df), P(phi), P(f), P(delta), 0, side_h), "vjp")
-        e2 = torch.cuda.Event()
-        e2.record(side)
-        main_s.wait_event(e2)
+        side2.wait_event(e)
+        d1, d2, d3 = (ct.byref(descs[deg]) for deg in DEGS)
+        _lib.check(L.pp_pull_vjp(d3, P(dphi), P(df), P(phi), P(f), P(delta), 0, stream), "vjp")
+        _lib.check(L.pp_pull_vjp(d2, P(dphi2), P(df2), P(phi), P(f), P(delta), 0, side_h), "vjp")
+        _lib.check(L.pp_pull_vjp(d1, P(dphi3), P(df3), P(phi), P(f), P(delta), 0, side2_h), "vjp")
+        _lib.check(L.pp_pull(d1, P(f1), P(phi), P(f), side_h), "pull")
+        _lib.check(L.pp_pull(d2, P(f12), P(phi), P(f), side2_h), "pull")
+        _lib.check(L.pp_pull(d3, P(f13), P(phi), P(f), side2_h), "pull")
+        for st in (side, side2):
+            e2 = torch.cuda.Event()
+            e2.record(st)
+            main_s.wait_event(e2)
 
     def step(events=None):
         for deg in DEGS:
@@ -829,8 +841,9 @@ def main():
         "config": {"workload": "pull+push+backward, 256^3 x 1 channel, degrees 1,2,3 (one triple each per step), "
                                "reflect boundary, ext=1", "volume": [N, N, N], "channels": 1, "degrees": list(DEGS),
                    "kernels": {0: "per-sample", 1: "automatic", 2: "tiled"}[args.tiling], "tile_shape": args.shape,
-                   "schedule": {"vjp": "pull on the launch stream; push and pullback from one pp_pull_vjp launch "
-                                       "on a second stream; joined per step",
+                   "schedule": {"vjp": "per degree, pull and one pp_pull_vjp launch (push and pullback); the six "
+                                       "launches on three streams: VJP 3 | VJP 2, pull 1 | VJP 1, pull 2, pull 3; "
+                                       "joined per step",
                                 "overlap": "pull and push on the launch stream, pullback on a second stream, "
                                            "joined per step",
                                 "serial": "nine launches in order on one stream"}[args.schedule]},
@@ -846,7 +859,7 @@ def main():
     headline_slabs = None if args.no_cpu else gpu_slab_outputs(torch, pp, L, f, phi, delta)
     pending = []
     if not args.no_extras:
-        del f1, df, dphi
+        del f1, df, dphi, f12, f13, df2, df3, dphi2, dphi3
         _progress("batched uneven volumes")
         res["batched_uneven"] = bench_batched(torch, pp)
         torch.cuda.empty_cache()
