@@ -544,6 +544,21 @@ def bench_matching(torch, pp, iters=50, alpha=0.02, seed=5, check=True):
             "loss_first_last": [losses[0], losses[-1]], "_fp64_pending": pending}
 
 
+def lds_atomic_peak(torch, L, blocks=4096, iters=64, reps=5):
+    """Calibrated rate of 64-bit integer LDS atomics (adds/s over the whole GPU) under the
+    scatters' own tap pattern (pp_lds_atomic_calibrate), the on-chip bound of push and the
+    VJP; HIP events on the current stream."""
+    from pushpull import _lib
+
+    out = torch.empty(blocks * 256, dtype=torch.int64, device="cuda:0")
+    st = ct.c_void_p(torch.cuda.current_stream().cuda_stream)
+    run = lambda: _lib.check(L.pp_lds_atomic_calibrate(ct.c_void_p(out.data_ptr()), blocks, iters, st),  # noqa: E731
+                             "lds calibration")
+    run()
+    ms = _time_ms(torch, run, reps, None)
+    return blocks * 256 * iters * 64 / (ms * 1e-3)
+
+
 def gpu_slab_outputs(torch, pp, L, f, phi, delta, z0=124, nz=8):
     """GPU results of the headline ops on a 256x256x`nz` slab of the deformation (pull,
     push, pullback and the VJP, degrees 1-3), as host copies for the FP64 check of the CPU
@@ -852,9 +867,19 @@ def main():
                      "traffic": _traffic(kernel_name(op, deg, args.tiling), deg), "bytes_per_launch": BYTES[op] * n1,
                      "avg_ms": round(ms, 4), "op_ms_with_fill": round(ms_op, 4)},
         "serial_ms_per_step": round(ms_serial, 4),
+        "roofline_on_chip": None,
         "per_degree": detail,
         "wall_s": round(wall, 3),
     }
+    if op in ("push", "vjp"):
+        # the scatters' on-chip bound: one 64-bit LDS atomic per tap of every sample
+        peak = lds_atomic_peak(torch, L)
+        adds = (deg + 1) ** 3 * n1
+        res["roofline_on_chip"] = {
+            "bound": "lds_atomics", "kernel": res["roofline"]["kernel"], "adds_per_launch": adds,
+            "achieved": round(adds / (ms * 1e-3) / 1e12, 4), "peak": round(peak / 1e12, 4), "unit": "Tadd/s",
+            "frac": round(adds / (ms * 1e-3) / peak, 4),
+            "calibration": "pp_lds_atomic_calibrate: 4096 x 256 lanes x 64 cubic samples, ds_add_u64 per tap"}
     # GPU results on a slab, checked against the FP64 oracle in the CPU leg below
     headline_slabs = None if args.no_cpu else gpu_slab_outputs(torch, pp, L, f, phi, delta)
     pending = []

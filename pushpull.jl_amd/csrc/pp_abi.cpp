@@ -936,6 +936,12 @@ int pp_axpy(float* y, const float* x, float a, size_t count, void* s) {
   hipError_t e = pp::launch_axpy(y, x, a, (long long)count, S(s));
   return e == hipSuccess ? ok() : hip_fail(e, "axpy");
 }
+int pp_lds_atomic_calibrate(unsigned long long* out, int blocks, int iters, void* s) {
+  if (!out) return fail(PP_EINVAL, "null array");
+  if (blocks < 1 || blocks > 65535 || iters < 1) return fail(PP_EINVAL, "blocks must be 1..65535, iters >= 1");
+  hipError_t e = pp::launch_lds_add_rate(out, blocks, iters, S(s));
+  return e == hipSuccess ? ok() : hip_fail(e, "lds atomic calibration");
+}
 int pp_stream_sync(void* s) {
   hipError_t e = hipStreamSynchronize(S(s));
   return e == hipSuccess ? ok() : hip_fail(e, "hipStreamSynchronize");

@@ -129,5 +129,6 @@ hipError_t launch_batch_group(Op op, int dp, const BatchVol* dvols, int nvol, co
 hipError_t launch_fill(float* p, long long n, float v, hipStream_t stream);
 hipError_t launch_fill_d(double* p, long long n, double v, hipStream_t stream);
 hipError_t launch_axpy(float* y, const float* x, float a, long long n, hipStream_t stream);
+hipError_t launch_lds_add_rate(unsigned long long* out, int blocks, int iters, hipStream_t stream);
 
 }  // namespace pp

@@ -872,6 +872,43 @@ __global__ __launch_bounds__(kTB) void k_batch(const BatchVol* __restrict__ vols
 
 int batch_tiles(const KArgs& a) { return tiles_of<kBTX, kBTY, kBTZ>(a); }
 
+// ---- LDS atomic calibration --------------------------------------------------------
+// The rate of ds_add_u64 under the scatter's own access pattern, the on-chip bound of
+// push and the VJP: every lane adds the 4x4x4 taps of cubic samples into a box shaped like
+// a 16x8x8 tile's at degree 3 (20 x 12 x 12 cells), its knots following the lane's x and y
+// (z: 2 per repetition, as a thread's samples), a third of the lanes shifted by a cell
+// along x and a sixth along y as under a smooth displacement. `iters` samples per lane, one
+// ds_add_u64 per tap; out (gridDim.x * 256 words) receives checksums of the box.
+__global__ __launch_bounds__(kTB) void k_lds_add_rate(acc_t* __restrict__ out, int iters) {
+  constexpr int BXc = 20, BYc = 12, BZc = 12, CELLS = BXc * BYc * BZc;
+  __shared__ acc_t cal[CELLS];
+  for (int i = threadIdx.x; i < CELLS; i += kTB) cal[i] = 0ull;
+  __syncthreads();
+  const int l = threadIdx.x;
+  const unsigned h = (blockIdx.x * kTB + l) * 2654435761u;
+  const int jx = ((h >> 24) & 255u) < 85u ? 1 : 0, jy = ((h >> 16) & 255u) < 43u ? 1 : 0;
+  // largest cell: 16 + 20 * 8 + 240 * (2 + 6) + 3 + 20 * 3 + 240 * 3 = 2879 < CELLS
+  const int base = (l % 16 + jx) + BXc * ((l / 16) % 8 + jy) + BXc * BYc * (2 * (l / 128));
+  for (int it = 0; it < iters; ++it) {
+    const int b = base + BXc * BYc * 2 * (it & 3);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) atomicAdd(cal + b + i + BXc * j + BXc * BYc * k, (acc_t)(it + i + 1));
+  }
+  __syncthreads();
+  acc_t sum = 0ull;
+  for (int i = l; i < CELLS; i += kTB) sum += cal[i];
+  out[(size_t)blockIdx.x * kTB + l] = sum;
+}
+
+hipError_t launch_lds_add_rate(unsigned long long* out, int blocks, int iters, hipStream_t s) {
+  hipLaunchKernelGGL(k_lds_add_rate, dim3((unsigned)blocks), dim3(kTB), 0, s, out, iters);
+  return hipGetLastError();
+}
+
 // Zero what a batched push / objective accumulates into (graph-capturable).
 __global__ void k_batch_zero(const BatchVol* __restrict__ vols, int kind) {
   const BatchVol& b = vols[blockIdx.y];

@@ -76,6 +76,7 @@ _SIGS = {
     "pp_memcpy_d2h": ([_P, _P, ct.c_size_t, _P], ct.c_int),
     "pp_memset_zero": ([_P, ct.c_size_t, _P], ct.c_int),
     "pp_axpy": ([_P, _P, ct.c_float, ct.c_size_t, _P], ct.c_int),
+    "pp_lds_atomic_calibrate": ([_P, ct.c_int, ct.c_int, _P], ct.c_int),
     "pp_stream_sync": ([_P], ct.c_int),
     "pp_device_sync": ([], ct.c_int),
     "pp_set_option": ([ct.c_int, ct.c_int], ct.c_int),
