@@ -98,7 +98,10 @@ __device__ __forceinline__ void row_taps(const float* __restrict__ row, const Ax
   for (int i = 0; i < NP; ++i) PP_TAP(DP, ax, i) v[i] = row[ax.off[i]];
 }
 
-// Value: sum_ijk wx_i wy_j wz_k f[...]
+// Value: sum_ijk wx_i wy_j wz_k f[...]. Contractions run one axis at a time (x within a
+// row, y within a plane, then z), in the same order as the LDS-tiled gathers, so both
+// kernel families give the same bits for a sample (boundary signs, folded here into the
+// weights and there into the staged data, factor out exactly).
 template <int DP, bool LEAN = false>
 __device__ __forceinline__ float gather_val(const float* __restrict__ f, const Axis<DP>& ax, const Axis<DP>& ay,
                                             const Axis<DP>& az) {
@@ -107,6 +110,7 @@ __device__ __forceinline__ float gather_val(const float* __restrict__ f, const A
   float acc = 0.f;
 #pragma unroll KU
   for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
+      float p = 0.f;
 #pragma unroll
       for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
           float v[NP];
@@ -114,8 +118,9 @@ __device__ __forceinline__ float gather_val(const float* __restrict__ f, const A
           float r = 0.f;
 #pragma unroll
           for (int i = 0; i < NP; ++i) PP_TAP(DP, ax, i) r += ax.w[i] * v[i];
-          acc += (az.w[k] * ay.w[j]) * r;
+          p += ay.w[j] * r;
         }
+      acc += az.w[k] * p;
     }
   return acc;
 }
@@ -129,6 +134,7 @@ __device__ __forceinline__ void gather_grad(const float* __restrict__ f, const A
   gx = gy = gz = 0.f;
 #pragma unroll KU
   for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
+      float pv = 0.f, px = 0.f, py = 0.f;
 #pragma unroll
       for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
           float v[NP];
@@ -139,10 +145,13 @@ __device__ __forceinline__ void gather_grad(const float* __restrict__ f, const A
               r += ax.w[i] * v[i];
               rd += ax.dw[i] * v[i];
             }
-          gx += (az.w[k] * ay.w[j]) * rd;
-          gy += (az.w[k] * ay.dw[j]) * r;
-          gz += (az.dw[k] * ay.w[j]) * r;
+          pv += ay.w[j] * r;
+          px += ay.w[j] * rd;
+          py += ay.dw[j] * r;
         }
+      gx += az.w[k] * px;
+      gy += az.w[k] * py;
+      gz += az.dw[k] * pv;
     }
 }
 
@@ -155,6 +164,7 @@ __device__ __forceinline__ void gather_val_grad(const float* __restrict__ f, con
   val = gx = gy = gz = 0.f;
 #pragma unroll KU
   for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
+      float pv = 0.f, px = 0.f, py = 0.f;
 #pragma unroll
       for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
           float v[NP];
@@ -165,12 +175,14 @@ __device__ __forceinline__ void gather_val_grad(const float* __restrict__ f, con
               r += ax.w[i] * v[i];
               rd += ax.dw[i] * v[i];
             }
-          const float wyz = az.w[k] * ay.w[j];
-          val += wyz * r;
-          gx += wyz * rd;
-          gy += (az.w[k] * ay.dw[j]) * r;
-          gz += (az.dw[k] * ay.w[j]) * r;
+          pv += ay.w[j] * r;
+          px += ay.w[j] * rd;
+          py += ay.dw[j] * r;
         }
+      val += az.w[k] * pv;
+      gx += az.w[k] * px;
+      gy += az.w[k] * py;
+      gz += az.dw[k] * pv;
     }
 }
 

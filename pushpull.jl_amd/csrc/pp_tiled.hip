@@ -530,8 +530,19 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, acc_
   __syncthreads();  // the tile state may be reused by the next tile
 }
 
+// Waves per SIMD each 16x8x8 tile kernel is compiled for (one wave per SIMD per
+// workgroup): as many workgroups as its LDS box lets share a CU, where the register
+// budget that implies costs no spills; 0 leaves the compiler's choice.
+template <int DP, int KIND>  // KIND: gather kinds 0-4, push 5, push_grad 6
+constexpr int tile_waves() {
+  return DP == 2 ? (KIND == 3 ? 5 : KIND == 5 ? 6 : 0)
+       : DP == 3 ? (KIND == 0 || KIND == 4 ? 6 : KIND == 1 ? 5 : KIND == 2 || KIND == 3 ? 4 : 0)
+       : DP == 4 ? (KIND == 2 ? 3 : 0)
+                 : 0;
+}
+
 template <int DP, int TX, int TY, int TZ, int CAP, bool GRAD>
-__global__ __launch_bounds__(kTB) void k_push_tile(KArgs a) {
+__global__ __launch_bounds__(kTB, (tile_waves<DP, GRAD ? 6 : 5>())) void k_push_tile(KArgs a) {
   __shared__ acc_t box[CAP];
   __shared__ TileState S;
   __shared__ float wsum[kTB / 64];
@@ -541,19 +552,25 @@ __global__ __launch_bounds__(kTB) void k_push_tile(KArgs a) {
 // ---- gathers: pull (KIND 0), phi pullback of pull (1), fused objective (2), the ------
 // ---- vector-Jacobian product of pull (3: push(delta) and the phi pullback together) --
 // ---- and pull_grad (4) ---------------------------------------------------------------
+// The contractions run one axis at a time -- x within each row, then y within each
+// plane, then z -- which takes fewer FMAs than weighting every row by its yz product
+// (a cubic gradient: 188 instead of 224 per sample).
 template <int DP>
 __device__ __forceinline__ float lds_val(const float* box, const LSample<DP>& L, int BX, int BXY) {
   float acc = 0.f;
 #pragma unroll
-  for (int k = 0; k < DP; ++k)
+  for (int k = 0; k < DP; ++k) {
+    float p = 0.f;
 #pragma unroll
     for (int j = 0; j < DP; ++j) {
       const float* row = box + L.base + BX * j + BXY * k;
       float r = 0.f;
 #pragma unroll
       for (int i = 0; i < DP; ++i) r += L.wx[i] * row[i];
-      acc += (L.wz[k] * L.wy[j]) * r;
+      p += L.wy[j] * r;
     }
+    acc += L.wz[k] * p;
+  }
   return acc;
 }
 
@@ -562,7 +579,8 @@ __device__ __forceinline__ void lds_grad(const float* box, const LSample<DP>& L,
                                          float& gx, float& gy, float& gz) {
   val = gx = gy = gz = 0.f;
 #pragma unroll
-  for (int k = 0; k < DP; ++k)
+  for (int k = 0; k < DP; ++k) {
+    float pv = 0.f, px = 0.f, py = 0.f;  // plane k: sum_j wy r, sum_j wy rd, sum_j dy r
 #pragma unroll
     for (int j = 0; j < DP; ++j) {
       const float* row = box + L.base + BX * j + BXY * k;
@@ -573,12 +591,15 @@ __device__ __forceinline__ void lds_grad(const float* box, const LSample<DP>& L,
         r += L.wx[i] * v;
         rd += L.dx[i] * v;
       }
-      const float wyz = L.wz[k] * L.wy[j];
-      if (VAL) val += wyz * r;
-      gx += wyz * rd;
-      gy += (L.wz[k] * L.dy[j]) * r;
-      gz += (L.dz[k] * L.wy[j]) * r;
+      pv += L.wy[j] * r;
+      px += L.wy[j] * rd;
+      py += L.dy[j] * r;
     }
+    if (VAL) val += L.wz[k] * pv;
+    gx += L.wz[k] * px;
+    gy += L.wz[k] * py;
+    gz += L.dz[k] * pv;
+  }
 }
 
 // Gradient gather of one sample outside the tile's box, through global memory; for the
@@ -773,7 +794,7 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
 }
 
 template <int DP, int TX, int TY, int TZ, int CAP, int KIND>
-__global__ __launch_bounds__(kTB) void k_gather_tile(KArgs a) {
+__global__ __launch_bounds__(kTB, (tile_waves<DP, KIND>())) void k_gather_tile(KArgs a) {
   __shared__ __align__(8) float box[box_floats<CAP, KIND>()];
   __shared__ TileState S;
   __shared__ float wsum[kTB / 64];
