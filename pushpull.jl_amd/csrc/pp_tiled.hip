@@ -867,8 +867,9 @@ static hipError_t launch_deg(Op op, const KArgs& a, int nbatch, hipStream_t s, T
 // bodies with an empty box budget, i.e. per sample through global memory.
 constexpr int kBTX = 16, kBTY = 8, kBTZ = 8;
 
-template <int DP, int KIND>
-__global__ __launch_bounds__(kTB) void k_batch(const BatchVol* __restrict__ vols, int nvol, XcdSplit split) {
+template <int DP, int KIND>  // KIND: gather kinds 0-2, push 3, push_grad 4
+__global__ __launch_bounds__(kTB, (tile_waves<DP, KIND <= 2 ? KIND : KIND + 2>())) void k_batch(
+    const BatchVol* __restrict__ vols, int nvol, XcdSplit split) {
   constexpr int KG = KIND <= 2 ? KIND : 0;
   constexpr int CAPE = (DP >= 2 && DP <= 4) ? cap_cells<DP, KIND>() : 0;
   constexpr int BOXF = KIND <= 2 ? box_floats<CAPE, KG>() : 2 * CAPE;  // push: CAPE int64 cells
