@@ -194,7 +194,9 @@ int run(pp::Op op, const pp_desc* d, const Geo& g, pp::KArgs a, hipStream_t s) {
     a.csign_on = 1;
     a.csign = mask;
     a.nchan = d->nchan;
-    hipError_t e = launch_variant(op, a, d->nbatch, s, 1 + g_shape.load(std::memory_order_relaxed));
+    // the per-sample pull (trilinear: faster there) and the tiled kernels take the signs
+    const bool per_sample = op == pp::OP_PULL && a.dpx == 2;
+    hipError_t e = launch_variant(op, a, d->nbatch, s, per_sample ? 0 : 1 + g_shape.load(std::memory_order_relaxed));
     return e == hipSuccess ? ok() : hip_fail(e, "kernel launch");
   }
   for (int c = 0; c < d->nchan; ++c) {

@@ -17,6 +17,52 @@ namespace pp {
   if (v >= a.n1) return;                                           \
   const float* phi = a.phi + bb * a.phi_bs;
 
+// r = c ? p : q, field by field (selects in registers; a reference chosen at run time
+// would put both axes in scratch memory).
+template <int DP>
+__device__ __forceinline__ void pick_axis(bool c, const Axis<DP>& p, const Axis<DP>& q, Axis<DP>& r) {
+#pragma unroll
+  for (int i = 0; i < Axis<DP>::NP; ++i) {
+    r.w[i] = c ? p.w[i] : q.w[i];
+    r.off[i] = c ? p.off[i] : q.off[i];
+  }
+  r.n = p.n;
+  r.fast = p.fast;
+}
+
+// Pull of channels whose boundary codes differ only between reflect (1) and antisymmetric
+// (2) per axis (csign): the two codes fold to the same indices, so each axis is built with
+// and without the sign once and every channel takes its own combination.
+template <int DP>
+__global__ __launch_bounds__(256) void k_pull_signed(KArgs a) {
+  PP_KERNEL_PROLOGUE
+  float* out = a.out + bb * a.out_bs + v;
+  float x, y, z;
+  sample_coords(a, phi, v, x, y, z);
+  if (!voxok(x, y, z, a.ext, a.lx, a.ly, a.lz)) {
+    for (int c = 0; c < a.nchan; ++c) out[c * a.out_cs] = __builtin_nanf("");
+    return;
+  }
+  // per axis: with the launch's code (s*), and with code 1 where that is 2 (u*)
+  Axis<DP> sx, sy, sz, ux, uy, uz;
+  make_axis<DP, kW>(x, a.dpx, a.d0x, 1, a.bx, sx);
+  make_axis<DP, kW>(y, a.dpy, a.d0y, a.d0x, a.by, sy);
+  make_axis<DP, kW>(z, a.dpz, a.d0z, a.d0x * a.d0y, a.bz, sz);
+  make_axis<DP, kW>(x, a.dpx, a.d0x, 1, a.bx == 2 ? 1 : a.bx, ux);
+  make_axis<DP, kW>(y, a.dpy, a.d0y, a.d0x, a.by == 2 ? 1 : a.by, uy);
+  make_axis<DP, kW>(z, a.dpz, a.d0z, a.d0x * a.d0y, a.bz == 2 ? 1 : a.bz, uz);
+  const float* f = a.f + bb * a.f_bs;
+  const int jk = v / a.d1x, i = v - jk * a.d1x;
+  for (int c = 0; c < a.nchan; ++c) {
+    const unsigned sg = chan_signs(a, c);
+    Axis<DP> ax, ay, az;
+    pick_axis((sg & 1u) != 0, sx, ux, ax);
+    pick_axis((sg & 2u) != 0, sy, uy, ay);
+    pick_axis((sg & 4u) != 0, sz, uz, az);
+    out[c * a.out_cs] = pull_out(a, sample_val<DP>(f + c * a.f_cs, ax, ay, az), c, i, jk % a.d1y, jk / a.d1y);
+  }
+}
+
 template <int DP>
 __global__ __launch_bounds__(256) void k_pull(KArgs a) {
   PP_KERNEL_PROLOGUE
@@ -271,7 +317,10 @@ template <int DP>
 static hipError_t launch_dp(Op op, const KArgs& a, int nbatch, hipStream_t s, int block) {
   const dim3 grid((unsigned)((a.n1 + block - 1) / block), (unsigned)nbatch, 1);
   switch (op) {
-    case OP_PULL: hipLaunchKernelGGL(k_pull<DP>, grid, dim3(block), 0, s, a); break;
+    case OP_PULL:
+      if (a.csign_on) hipLaunchKernelGGL(k_pull_signed<DP>, grid, dim3(block), 0, s, a);
+      else hipLaunchKernelGGL(k_pull<DP>, grid, dim3(block), 0, s, a);
+      break;
     case OP_PULLG: hipLaunchKernelGGL(k_pullg<DP>, grid, dim3(block), 0, s, a); break;
     case OP_PULLH: hipLaunchKernelGGL(k_pullh<DP>, grid, dim3(block), 0, s, a); break;
     case OP_PUSH: hipLaunchKernelGGL(k_push<DP>, grid, dim3(block), 0, s, a); break;
