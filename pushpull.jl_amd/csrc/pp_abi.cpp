@@ -88,7 +88,8 @@ hipError_t launch(pp::Op op, const pp::KArgs& a, int nbatch, hipStream_t s) {
   // Built-in choice: trilinear gathers (8 taps, served by L1/L2) run per sample, which
   // beats staging a box at that degree on gfx950; everything else runs tiled.
   const bool trilinear = a.dpx == 2 && a.dpy == 2 && a.dpz == 2;
-  if (trilinear && (op == pp::OP_PULL || op == pp::OP_PULLG || op == pp::OP_PULL_BWD))
+  if (trilinear && (op == pp::OP_PULL || op == pp::OP_PULLG || op == pp::OP_PULLH || op == pp::OP_PULL_BWD ||
+                    op == pp::OP_PULLG_BWD))
     return launch_variant(op, a, nbatch, s, 0);
   return launch_variant(op, a, nbatch, s, tiled);
 }

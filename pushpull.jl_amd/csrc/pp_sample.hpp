@@ -195,6 +195,7 @@ __device__ __forceinline__ void gather_hess(const float* __restrict__ f, const A
   for (int q = 0; q < 6; ++q) h[q] = 0.f;
 #pragma unroll
   for (int k = 0; k < NP; ++k) PP_TAP(DP, az, k) {
+      float pw = 0.f, pwd = 0.f, pwh = 0.f, pd = 0.f, pdd = 0.f, ph = 0.f;
 #pragma unroll
       for (int j = 0; j < NP; ++j) PP_TAP(DP, ay, j) {
           const float* row = f + az.off[k] + ay.off[j];
@@ -206,13 +207,19 @@ __device__ __forceinline__ void gather_hess(const float* __restrict__ f, const A
               rd += ax.dw[i] * v;
               rh += ax.hw[i] * v;
             }
-          h[0] += (az.w[k] * ay.w[j]) * rh;
-          h[1] += (az.w[k] * ay.hw[j]) * r;
-          h[2] += (az.hw[k] * ay.w[j]) * r;
-          h[3] += (az.w[k] * ay.dw[j]) * rd;
-          h[4] += (az.dw[k] * ay.w[j]) * rd;
-          h[5] += (az.dw[k] * ay.dw[j]) * r;
+          pw += ay.w[j] * r;
+          pwd += ay.w[j] * rd;
+          pwh += ay.w[j] * rh;
+          pd += ay.dw[j] * r;
+          pdd += ay.dw[j] * rd;
+          ph += ay.hw[j] * r;
         }
+      h[0] += az.w[k] * pwh;
+      h[1] += az.w[k] * ph;
+      h[2] += az.hw[k] * pw;
+      h[3] += az.w[k] * pdd;
+      h[4] += az.dw[k] * pwd;
+      h[5] += az.dw[k] * pd;
     }
 }
 

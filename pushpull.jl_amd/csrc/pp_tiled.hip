@@ -402,6 +402,21 @@ __device__ __forceinline__ void lsample(float x, float y, float z, const TileSta
   L.base = (ox - S.lo[0]) + S.b[0] * ((oy - S.lo[1]) + S.b[1] * (oz - S.lo[2]));
 }
 
+// ... with the second-derivative weights too (pull_hess and its pullback)
+template <int DP>
+struct LSampleH : LSample<DP> {
+  float hx[DP], hy[DP], hz[DP];
+};
+
+template <int DP>
+__device__ __forceinline__ void lsample_h(float x, float y, float z, const TileState& S, LSampleH<DP>& L) {
+  const int ox = knot_fast<DP>(x), oy = knot_fast<DP>(y), oz = knot_fast<DP>(z);
+  weights_ct<DP>(x - (float)ox, L.wx, L.dx, L.hx);
+  weights_ct<DP>(y - (float)oy, L.wy, L.dy, L.hy);
+  weights_ct<DP>(z - (float)oz, L.wz, L.dz, L.hz);
+  L.base = (ox - S.lo[0]) + S.b[0] * ((oy - S.lo[1]) + S.b[1] * (oz - S.lo[2]));
+}
+
 // ---- push / push_grad -------------------------------------------------------------
 // One tile of push (GRAD=false) or push_grad (GRAD=true) for batch index bb. The caller
 // provides the int64 LDS box (CAP cells), wsum (kTB/64 floats) and the tile state.
@@ -533,16 +548,16 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, acc_
 // Waves per SIMD each 16x8x8 tile kernel is compiled for (one wave per SIMD per
 // workgroup): as many workgroups as its LDS box lets share a CU, where the register
 // budget that implies costs no spills; 0 leaves the compiler's choice.
-template <int DP, int KIND>  // KIND: gather kinds 0-4, push 5, push_grad 6
+template <int DP, int KIND>  // KIND: gather kinds 0-6, push 7, push_grad 8
 constexpr int tile_waves() {
-  return DP == 2 ? (KIND == 3 ? 5 : KIND == 5 ? 6 : 0)
+  return DP == 2 ? (KIND == 3 ? 5 : KIND == 7 ? 6 : 0)
        : DP == 3 ? (KIND == 0 || KIND == 4 ? 6 : KIND == 1 ? 5 : KIND == 2 || KIND == 3 ? 4 : 0)
        : DP == 4 ? (KIND == 2 ? 3 : 0)
                  : 0;
 }
 
 template <int DP, int TX, int TY, int TZ, int CAP, bool GRAD>
-__global__ __launch_bounds__(kTB, (tile_waves<DP, GRAD ? 6 : 5>())) void k_push_tile(KArgs a) {
+__global__ __launch_bounds__(kTB, (tile_waves<DP, GRAD ? 8 : 7>())) void k_push_tile(KArgs a) {
   __shared__ acc_t box[CAP];
   __shared__ TileState S;
   __shared__ float wsum[kTB / 64];
@@ -602,6 +617,41 @@ __device__ __forceinline__ void lds_grad(const float* box, const LSample<DP>& L,
   }
 }
 
+// Hessian (xx, yy, zz, xy, xz, yz) from the box, one axis at a time as lds_grad.
+template <int DP>
+__device__ __forceinline__ void lds_hess(const float* box, const LSampleH<DP>& L, int BX, int BXY, float (&h)[6]) {
+#pragma unroll
+  for (int q = 0; q < 6; ++q) h[q] = 0.f;
+#pragma unroll
+  for (int k = 0; k < DP; ++k) {
+    float pw = 0.f, pwd = 0.f, pwh = 0.f, pd = 0.f, pdd = 0.f, ph = 0.f;
+#pragma unroll
+    for (int j = 0; j < DP; ++j) {
+      const float* row = box + L.base + BX * j + BXY * k;
+      float r = 0.f, rd = 0.f, rh = 0.f;
+#pragma unroll
+      for (int i = 0; i < DP; ++i) {
+        const float v = row[i];
+        r += L.wx[i] * v;
+        rd += L.dx[i] * v;
+        rh += L.hx[i] * v;
+      }
+      pw += L.wy[j] * r;
+      pwd += L.wy[j] * rd;
+      pwh += L.wy[j] * rh;
+      pd += L.dy[j] * r;
+      pdd += L.dy[j] * rd;
+      ph += L.hy[j] * r;
+    }
+    h[0] += L.wz[k] * pwh;
+    h[1] += L.wz[k] * ph;
+    h[2] += L.hz[k] * pw;
+    h[3] += L.wz[k] * pdd;
+    h[4] += L.dz[k] * pwd;
+    h[5] += L.dz[k] * pd;
+  }
+}
+
 // Gradient gather of one sample outside the tile's box, through global memory; for the
 // objective (KIND 2) it also scatters the residual, for the VJP (3) the upstream gradient.
 // Returns the weight w of the pullback (the upstream gradient, or the residual). Rarely taken, so it gathers lean: it must not
@@ -629,8 +679,9 @@ constexpr int box_floats() {
 }
 
 // One tile of pull (KIND 0), the pull pullback (1), the fused objective (2), the VJP
-// of pull (3: dphi = sum_c delta_c grad f_c and df0 += push(delta) from one staging) or
-// pull_grad (4: the three components of grad f_c per channel).
+// of pull (3: dphi = sum_c delta_c grad f_c and df0 += push(delta) from one staging),
+// pull_grad (4: the three components of grad f_c per channel), pull_hess (5: the 3x3
+// Hessian per channel) or the pullback of pull_grad (6: dphi_b = sum_{a,c} delta_{a,c} H_{a,b,c}).
 // box: box_floats<CAP, KIND>() floats (8-byte aligned); wsum: kTB/64 floats.
 // The pullback of every sample is summed over channels in registers (gsum), and the
 // objective keeps each sample's residual in registers between its two passes.
@@ -638,7 +689,8 @@ template <int DP, int TX, int TY, int TZ, int CAP, int KIND>
 __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, float* box, float* wsum,
                                             TileState& S) {
   constexpr int VPT = Shape<TX, TY, TZ>::VPT;
-  constexpr bool AUX = KIND >= 1 && KIND <= 3;  // reads an upstream gradient or target
+  constexpr bool AUX = KIND >= 1 && KIND <= 3;  // reads an upstream gradient or target per sample
+  constexpr bool RED = AUX || KIND == 6;         // writes dphi, reduced over the channels
   const float* phi = a.phi + bb * a.phi_bs;
   float auxv[VPT];  // upstream gradient / target; channel 0's is loaded before phi is read
   auto prefetch = [&](const Vox<VPT>& W) {
@@ -677,6 +729,8 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
         if (KIND == 0) a.out[bb * a.out_bs + c * a.out_cs + v] = __builtin_nanf("");
         if (KIND == 4) put3(a.out + bb * a.out_bs + c * a.out_cs + v, n1, __builtin_nanf(""), __builtin_nanf(""),
                             __builtin_nanf(""), 0);
+        if (KIND == 5)
+          for (int e = 0; e < 9; ++e) a.out[bb * a.out_bs + c * a.out_cs + v + e * n1] = __builtin_nanf("");
         continue;
       }
       const float x = V.x[0], y = V.y[0], z = V.z[0];
@@ -699,6 +753,34 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
           val = pull_out(a, val, c, S.org[0] + l % TX, S.org[1] + (l / TX) % TY, S.org[2] + l / (TX * TY));
         }
         a.out[bb * a.out_bs + c * a.out_cs + v] = val;
+        continue;
+      }
+      if (KIND == 5 || KIND == 6) {
+        float h[6];
+        if (use_lds) {
+          LSampleH<DP> L;
+          lsample_h<DP>(x, y, z, S, L);
+          lds_hess<DP>(box, L, BX, BXY, h);
+        } else {
+          Axis<DP> ax, ay, az;
+          make_axis<DP, kW | kDW | kHW>(x, DP, a.d0x, 1, a.bx, ax);
+          make_axis<DP, kW | kDW | kHW>(y, DP, a.d0y, a.d0x, a.by, ay);
+          make_axis<DP, kW | kDW | kHW>(z, DP, a.d0z, a.d0x * a.d0y, a.bz, az);
+          sample_hess<DP>(src, ax, ay, az, h);
+        }
+        if (KIND == 5) {  // column-major 3x3 (pushpull.c:129-134)
+          float* o = a.out + bb * a.out_bs + c * a.out_cs + v;
+          o[0] = h[0]; o[4 * n1] = h[1]; o[8 * n1] = h[2];
+          o[1 * n1] = h[3]; o[3 * n1] = h[3];
+          o[2 * n1] = h[4]; o[6 * n1] = h[4];
+          o[5 * n1] = h[5]; o[7 * n1] = h[5];
+        } else {
+          const float* dl = a.aux + bb * a.aux_bs + c * a.aux_cs + v;
+          const float d0 = dl[0], d1 = dl[n1], d2 = dl[2 * n1];
+          gsum[0][0] += d0 * h[0] + d1 * h[3] + d2 * h[4];
+          gsum[1][0] += d0 * h[3] + d1 * h[1] + d2 * h[5];
+          gsum[2][0] += d0 * h[4] + d1 * h[5] + d2 * h[2];
+        }
         continue;
       }
       if (KIND == 4) {
@@ -770,7 +852,7 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
       }
     }
   }
-  if (AUX) {
+  if (RED) {
     const float nan = __builtin_nanf("");
 #pragma unroll 1
     for (int q = 0; q < VPT; ++q, V.rotate(), rot_rows(gsum)) {
@@ -827,6 +909,12 @@ static hipError_t launch_shape(Op op, const KArgs& a, int nbatch, hipStream_t s)
     case OP_PULLG:
       hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, cap_cells<DP, 0>(), 4>), grid, block, 0, s, a);
       break;
+    case OP_PULLH:
+      hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, cap_cells<DP, 0>(), 5>), grid, block, 0, s, a);
+      break;
+    case OP_PULLG_BWD:
+      hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, cap_cells<DP, 1>(), 6>), grid, block, 0, s, a);
+      break;
     case OP_PULL_BWD:
       hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, cap_cells<DP, 1>(), 1>), grid, block, 0, s, a);
       break;
@@ -868,7 +956,7 @@ static hipError_t launch_deg(Op op, const KArgs& a, int nbatch, hipStream_t s, T
 constexpr int kBTX = 16, kBTY = 8, kBTZ = 8;
 
 template <int DP, int KIND>  // KIND: gather kinds 0-2, push 3, push_grad 4
-__global__ __launch_bounds__(kTB, (tile_waves<DP, KIND <= 2 ? KIND : KIND + 2>())) void k_batch(
+__global__ __launch_bounds__(kTB, (tile_waves<DP, KIND <= 2 ? KIND : KIND + 4>())) void k_batch(
     const BatchVol* __restrict__ vols, int nvol, XcdSplit split) {
   constexpr int KG = KIND <= 2 ? KIND : 0;
   constexpr int CAPE = (DP >= 2 && DP <= 4) ? cap_cells<DP, KIND>() : 0;
@@ -983,8 +1071,8 @@ hipError_t launch_batch_group(Op op, int dp, const BatchVol* dvols, int nvol, co
 }
 
 hipError_t launch_tiled(Op op, const KArgs& a, int nbatch, hipStream_t s, TileCfg cfg) {
-  if (op != OP_PULL && op != OP_PULLG && op != OP_PULL_BWD && op != OP_OBJGRAD && op != OP_PULL_VJP &&
-      op != OP_PUSH && op != OP_PUSHG)
+  if (op != OP_PULL && op != OP_PULLG && op != OP_PULLH && op != OP_PULL_BWD && op != OP_PULLG_BWD &&
+      op != OP_OBJGRAD && op != OP_PULL_VJP && op != OP_PUSH && op != OP_PUSHG)
     return hipErrorNotSupported;
   if (a.n1 <= 0 || nbatch <= 0) return hipSuccess;
   if (a.dpx != a.dpy || a.dpx != a.dpz) return hipErrorNotSupported;
