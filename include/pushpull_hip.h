@@ -294,10 +294,11 @@ int pp_device_sync(void);
 /* ---- kernel selection (no counterpart in the reference) -------------------- */
 enum {
   PP_OPT_TILING = 1,     /* 1 (default): automatic -- the autotuner's choice where it has one,
-                            else one work-item per sample for trilinear pull and pullback and
-                            LDS-tiled kernels for the rest; 2: LDS-tiled kernels wherever they
-                            exist (pull, push, push_grad, the pull pullback, the fused
-                            objective; degrees 1-3); 0: one work-item per sample */
+                            else one work-item per sample for the trilinear gathers (pull,
+                            pull_grad, pull_hess and their pullbacks) and LDS-tiled kernels for
+                            the rest; 2: LDS-tiled kernels wherever they exist (pull, pull_grad,
+                            pull_hess, push, push_grad, the pullbacks, the pull VJP, the fused
+                            objective; isotropic degrees 1-3); 0: one work-item per sample */
   PP_OPT_TILE_SHAPE = 2, /* tile of the phi grid: 0 = 16x8x8 (default), 1 = 32x8x4, 2 = 16x16x4 */
   PP_OPT_USE_TUNED = 3   /* 1 (default): prefer the autotuner's choice when it has one */
 };
