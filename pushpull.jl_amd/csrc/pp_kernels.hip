@@ -294,10 +294,20 @@ __global__ __launch_bounds__(256) void k_pull_vjp(KArgs a) {
   put3(out, n1, sx, sy, sz, a.accum);
 }
 
+// p[0..n) = v with 16-byte stores: scalar head up to the first 16-byte boundary, float4
+// body, scalar tail.
 __global__ void k_fill(float* p, long long n, float v) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long stride = (long long)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) p[i] = v;
+  long long head = (long long)(((16u - ((unsigned)(uintptr_t)p & 15u)) & 15u) / 4u);
+  if (head > n) head = n;
+  if (t < head) p[t] = v;
+  float4* q = reinterpret_cast<float4*>(p + head);
+  const long long nq = (n - head) / 4;
+  const float4 vv = make_float4(v, v, v, v);
+  for (long long i = t; i < nq; i += stride) q[i] = vv;
+  const long long done = head + 4 * nq;
+  if (t < n - done) p[done + t] = v;
 }
 
 __global__ void k_axpy(float* __restrict__ y, const float* __restrict__ x, float a, long long n) {
@@ -350,7 +360,7 @@ hipError_t launch_op(Op op, const KArgs& a, int nbatch, hipStream_t s, int block
 
 hipError_t launch_fill(float* p, long long n, float v, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  long long nb = (n + 255) / 256;
+  long long nb = (n / 4 + 255) / 256 + 1;  // a float4 per thread, and the head / tail
   if (nb > 65536) nb = 65536;
   hipLaunchKernelGGL(k_fill, dim3((unsigned)nb), dim3(256), 0, s, p, n, v);
   return hipGetLastError();
