@@ -890,8 +890,8 @@ __global__ __launch_bounds__(kTB, (tile_waves<DP, KIND>())) void k_gather_tile(K
 // scatters and the objective accumulate int64 cells.
 template <int DP, int KIND>
 constexpr int cap_cells() {
-  return (KIND == 0 || KIND == 1) ? (DP <= 2 ? 4096 : DP == 3 ? 6144 : 9472)
-                                  : (DP <= 2 ? 3072 : DP == 3 ? 4800 : 6400);
+  return (KIND == 0 || KIND == 1) ? (DP <= 2 ? 4096 : DP == 3 ? 6400 : 9472)
+                                  : (DP <= 2 ? 3200 : DP == 3 ? 4800 : 6400);
 }
 
 template <int TX, int TY, int TZ>
