@@ -547,7 +547,8 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, acc_
 
 // Waves per SIMD each 16x8x8 tile kernel is compiled for (one wave per SIMD per
 // workgroup): as many workgroups as its LDS box lets share a CU, where the register
-// budget that implies costs no spills; 0 leaves the compiler's choice.
+// budget that implies spills at most a few registers and measured faster; 0 leaves the
+// compiler's choice.
 template <int DP, int KIND>  // KIND: gather kinds 0-6, push 7, push_grad 8
 constexpr int tile_waves() {
   return DP == 2 ? (KIND == 3 ? 5 : KIND == 7 ? 6 : 0)
@@ -565,8 +566,8 @@ __global__ __launch_bounds__(kTB, (tile_waves<DP, GRAD ? 8 : 7>())) void k_push_
 }
 
 // ---- gathers: pull (KIND 0), phi pullback of pull (1), fused objective (2), the ------
-// ---- vector-Jacobian product of pull (3: push(delta) and the phi pullback together) --
-// ---- and pull_grad (4) ---------------------------------------------------------------
+// ---- vector-Jacobian product of pull (3: push(delta) and the phi pullback together), -
+// ---- pull_grad (4), pull_hess (5) and the pullback of pull_grad (6) ------------------
 // The contractions run one axis at a time -- x within each row, then y within each
 // plane, then z -- which takes fewer FMAs than weighting every row by its yz product
 // (a cubic gradient: 188 instead of 224 per sample).
@@ -654,8 +655,8 @@ __device__ __forceinline__ void lds_hess(const float* box, const LSampleH<DP>& L
 
 // Gradient gather of one sample outside the tile's box, through global memory; for the
 // objective (KIND 2) it also scatters the residual, for the VJP (3) the upstream gradient.
-// Returns the weight w of the pullback (the upstream gradient, or the residual). Rarely taken, so it gathers lean: it must not
-// set the register budget of the whole kernel.
+// Returns the weight w of the pullback (the upstream gradient, or the residual). Rarely
+// taken, so it gathers lean: it must not set the register budget of the whole kernel.
 template <int DP, int KIND>
 __device__ __forceinline__ float outlier_grad(const KArgs& a, const float* __restrict__ src, float* __restrict__ dst2,
                                            float x, float y, float z, float aux, float& gx, float& gy, float& gz) {
