@@ -112,7 +112,7 @@ def make_inputs(torch, dev, seed=1):
     return f.contiguous(), phi.contiguous(), delta.contiguous()
 
 
-def cpu_baseline(f, phi, delta, budget_s=20.0):
+def cpu_baseline(f, phi, delta, budget_s=40.0):
     """Reference CPU path on a z-slab of the workload, single thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
