@@ -107,3 +107,24 @@ def test_autograd_pull_uses_vjp_and_matches_oracle():
     pp.pull(tf, tp, s).backward(T(delta))
     check_close(H(tf.grad), oracle.push(delta, phi, d, so, prec="f64"), "autograd df0", scale=1.0)
     check_close(H(tp.grad), oracle.backward_phi(f, phi, delta, so), "autograd dphi")
+
+
+@pytest.mark.parametrize("tiling", [1, 2])
+@pytest.mark.parametrize("deg", [1, 2])
+def test_pull_channels_differing_only_in_sign(tiling, deg):
+    """Channels whose boundary codes differ only between reflect (1) and antisymmetric (2)
+    run as one launch with per-channel signs: the per-sample signed pull (trilinear, automatic
+    choice) and the tiled kernels, against the FP64 oracle, with out-of-FOV samples folding."""
+    old = pp._lib.get_option(pp._lib.PP_OPT_TILING)
+    pp._lib.set_option(pp._lib.PP_OPT_TILING, tiling)
+    try:
+        rng = np.random.default_rng(40 + deg + 10 * tiling)
+        d = (21, 26, 19)
+        f, phi, _ = _case(rng, d, 3, 6.0, out_frac=0.05)
+        bnd = [(2, 1, 1), (1, 2, 1), (1, 1, 2)]
+        s, so = pp.Settings(deg, bnd, 1), S(deg, bnd, 1)
+        got = H(pp.pull(T(f), T(phi), s))
+        e = check_close(got, oracle.pull(f, phi, so, prec="f64"), f"signed pull deg {deg} tiling {tiling}")
+        report(f"signed pull deg{deg} tiling{tiling} vs f64", e)
+    finally:
+        pp._lib.set_option(pp._lib.PP_OPT_TILING, old)
