@@ -298,7 +298,7 @@ enum {
                             pull_grad, pull_hess and their pullbacks) and LDS-tiled kernels for
                             the rest; 2: LDS-tiled kernels wherever they exist (pull, pull_grad,
                             pull_hess, push, push_grad, the pullbacks, the pull VJP, the fused
-                            objective; isotropic degrees 1-3); 0: one work-item per sample */
+                            objective; isotropic degrees 1-4); 0: one work-item per sample */
   PP_OPT_TILE_SHAPE = 2, /* tile of the phi grid: 0 = 16x8x8 (default), 1 = 32x8x4, 2 = 16x16x4 */
   PP_OPT_USE_TUNED = 3   /* 1 (default): prefer the autotuner's choice when it has one */
 };

@@ -186,7 +186,7 @@ int run(pp::Op op, const pp_desc* d, const Geo& g, pp::KArgs a, hipStream_t s) {
   }
   int base[3];
   unsigned mask;
-  const bool iso_tiled = a.dpx == a.dpy && a.dpx == a.dpz && a.dpx >= 2 && a.dpx <= 4;
+  const bool iso_tiled = a.dpx == a.dpy && a.dpx == a.dpz && a.dpx >= 2 && a.dpx <= 5;
   if ((op == pp::OP_PULL || op == pp::OP_PUSH) && iso_tiled && g_tiling.load(std::memory_order_relaxed) != 0 &&
       sign_split(d, base, mask)) {
     a.bx = base[0];
@@ -626,7 +626,7 @@ int push_count_impl(const pp_desc* d, float* f0, float* c0, const float* phi, co
   }
   int base[3];
   unsigned mask;
-  const bool iso_tiled = a.dpx == a.dpy && a.dpx == a.dpz && a.dpx >= 2 && a.dpx <= 4;
+  const bool iso_tiled = a.dpx == a.dpy && a.dpx == a.dpz && a.dpx >= 2 && a.dpx <= 5;
   if (fits && iso_tiled && g_tiling.load(std::memory_order_relaxed) != 0 && sign_split(&dd, base, mask)) {
     a.bx = base[0];
     a.by = base[1];

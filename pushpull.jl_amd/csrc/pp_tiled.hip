@@ -892,7 +892,7 @@ __global__ __launch_bounds__(kTB, (tile_waves<DP, KIND>())) void k_gather_tile(K
 template <int DP, int KIND>
 constexpr int cap_cells() {
   return (KIND == 0 || KIND == 1) ? (DP <= 2 ? 4096 : DP == 3 ? 6400 : 9472)
-                                  : (DP <= 2 ? 3200 : DP == 3 ? 4800 : 6400);
+                                  : (DP <= 2 ? 3200 : DP == 3 ? 4800 : 6400);  // DP 5 as DP 4
 }
 
 template <int TX, int TY, int TZ>
@@ -952,7 +952,7 @@ static hipError_t launch_deg(Op op, const KArgs& a, int nbatch, hipStream_t s, T
 // are numbered consecutively (BatchVol::tile0, relative to the group) and cut into eight
 // contiguous ranges of equal work, one per XCD (workgroup b runs on XCD b % 8), so that
 // neighbouring tiles share an L2 and no XCD idles while another still holds the heavy
-// (many-channel) volumes. Degrees 1-3 use the LDS tile bodies; degrees 0 and 4 run the same
+// (many-channel) volumes. Degrees 1-4 use the LDS tile bodies; degree 0 runs the same
 // bodies with an empty box budget, i.e. per sample through global memory.
 constexpr int kBTX = 16, kBTY = 8, kBTZ = 8;
 
@@ -960,7 +960,7 @@ template <int DP, int KIND>  // KIND: gather kinds 0-2, push 3, push_grad 4
 __global__ __launch_bounds__(kTB, (tile_waves<DP, KIND <= 2 ? KIND : KIND + 4>())) void k_batch(
     const BatchVol* __restrict__ vols, int nvol, XcdSplit split) {
   constexpr int KG = KIND <= 2 ? KIND : 0;
-  constexpr int CAPE = (DP >= 2 && DP <= 4) ? cap_cells<DP, KIND>() : 0;
+  constexpr int CAPE = (DP >= 2 && DP <= 5) ? cap_cells<DP, KIND>() : 0;
   constexpr int BOXF = KIND <= 2 ? box_floats<CAPE, KG>() : 2 * CAPE;  // push: CAPE int64 cells
   __shared__ __align__(8) float box[BOXF > 0 ? BOXF : 2];
   __shared__ TileState S;
@@ -1082,6 +1082,7 @@ hipError_t launch_tiled(Op op, const KArgs& a, int nbatch, hipStream_t s, TileCf
     case 2: return launch_deg<2>(op, a, nbatch, s, cfg);
     case 3: return launch_deg<3>(op, a, nbatch, s, cfg);
     case 4: return launch_deg<4>(op, a, nbatch, s, cfg);
+    case 5: return launch_deg<5>(op, a, nbatch, s, cfg);
     default: return hipErrorNotSupported;
   }
 }
