@@ -250,9 +250,9 @@ enum {
 typedef struct pp_volume {
   const float *f;    /* image (d0,C) for the pull family; pushed values (d1,C) or (d1,3,C) for push */
   const float *phi;  /* (d1,3), 1-based */
-  const float *aux;  /* PULL_BACKWARD: upstream gradient (d1,C); OBJECTIVE: target g (d1,C) */
-  float *out;        /* PULL: (d1,C); PUSH/PUSH_GRAD: (d0,C); PULL_BACKWARD/OBJECTIVE: dphi (d1,3) */
-  float *out2;       /* OBJECTIVE: dL/df (d0,C) */
+  const float *aux;  /* PULL_BACKWARD, PULL_VJP: upstream gradient (d1,C); OBJECTIVE: target g (d1,C) */
+  float *out;        /* PULL: (d1,C); PUSH/PUSH_GRAD: (d0,C); PULL_BACKWARD/OBJECTIVE/PULL_VJP: dphi (d1,3) */
+  float *out2;       /* OBJECTIVE: dL/df (d0,C); PULL_VJP: push(aux) (d0,C) (both zeroed by the run) */
   double *loss;      /* OBJECTIVE: 0.5*sum r^2 of this volume (device, one double) */
   int64_t d0[3];     /* image grid (X,Y,Z) */
   int64_t d1[3];     /* phi grid (X,Y,Z) */

@@ -757,7 +757,7 @@ int pp_batch_plan_create(int op, const pp_volume* vols, int nvol, pp_batch_plan*
   *plan = nullptr;
   if (!vols || nvol < 1) return fail(PP_EINVAL, "need at least one volume");
   if (op != PP_OP_PULL && op != PP_OP_PUSH && op != PP_OP_PUSH_GRAD && op != PP_OP_PULL_BACKWARD &&
-      op != PP_OP_OBJECTIVE)
+      op != PP_OP_OBJECTIVE && op != PP_OP_PULL_VJP)
     return fail(PP_EINVAL, "unsupported batch op %d", op);
   std::vector<pp::BatchVol> hv((size_t)nvol);
   long long total = 0;
@@ -775,8 +775,10 @@ int pp_batch_plan_create(int op, const pp_volume* vols, int nvol, pp_batch_plan*
     if (v.nchan < 1) return fail(PP_EINVAL, "volume %d: nchan must be >= 1", i);
     if (v.deg < 0 || v.deg > 4) return fail(PP_EINVAL, "volume %d: degree %d outside 0..4", i, v.deg);
     if (!v.f || !v.phi || !v.out) return fail(PP_EINVAL, "volume %d: null array", i);
-    if ((op == PP_OP_PULL_BACKWARD || op == PP_OP_OBJECTIVE) && !v.aux) return fail(PP_EINVAL, "volume %d: null aux", i);
+    if ((op == PP_OP_PULL_BACKWARD || op == PP_OP_OBJECTIVE || op == PP_OP_PULL_VJP) && !v.aux)
+      return fail(PP_EINVAL, "volume %d: null aux", i);
     if (op == PP_OP_OBJECTIVE && (!v.out2 || !v.loss)) return fail(PP_EINVAL, "volume %d: null out2/loss", i);
+    if (op == PP_OP_PULL_VJP && !v.out2) return fail(PP_EINVAL, "volume %d: null out2", i);
     pp::KArgs& a = hv[i].a;
     std::memset(&a, 0, sizeof a);
     a.n1 = (int)n1;
@@ -812,7 +814,7 @@ int pp_batch_plan_create(int op, const pp_volume* vols, int nvol, pp_batch_plan*
         a.aux_cs = n1;
         a.out2 = v.out2;
         a.out2_cs = n0;
-        a.red = v.loss;
+        a.red = op == PP_OP_OBJECTIVE ? v.loss : nullptr;
         hv[i].zero_n = n0 * v.nchan;
         break;
     }

@@ -956,12 +956,17 @@ static hipError_t launch_deg(Op op, const KArgs& a, int nbatch, hipStream_t s, T
 // bodies with an empty box budget, i.e. per sample through global memory.
 constexpr int kBTX = 16, kBTY = 8, kBTZ = 8;
 
-template <int DP, int KIND>  // KIND: gather kinds 0-2, push 3, push_grad 4
-__global__ __launch_bounds__(kTB, (tile_waves<DP, KIND <= 2 ? KIND : KIND + 4>())) void k_batch(
+// k_batch kinds: gather kinds 0-2, push 3, push_grad 4, the pull VJP 5 (gather kind 3)
+constexpr int batch_gather_kind(int kind) { return kind <= 2 ? kind : kind == 5 ? 3 : -1; }
+constexpr int batch_waves_kind(int kind) { return kind <= 2 ? kind : kind == 5 ? 3 : kind + 4; }
+
+template <int DP, int KIND>
+__global__ __launch_bounds__(kTB, (tile_waves<DP, batch_waves_kind(KIND)>())) void k_batch(
     const BatchVol* __restrict__ vols, int nvol, XcdSplit split) {
-  constexpr int KG = KIND <= 2 ? KIND : 0;
+  constexpr int KG = batch_gather_kind(KIND) >= 0 ? batch_gather_kind(KIND) : 0;
+  constexpr bool GATHER = batch_gather_kind(KIND) >= 0;
   constexpr int CAPE = (DP >= 2 && DP <= 5) ? cap_cells<DP, KIND>() : 0;
-  constexpr int BOXF = KIND <= 2 ? box_floats<CAPE, KG>() : 2 * CAPE;  // push: CAPE int64 cells
+  constexpr int BOXF = GATHER ? box_floats<CAPE, KG>() : 2 * CAPE;  // push: CAPE int64 cells
   __shared__ __align__(8) float box[BOXF > 0 ? BOXF : 2];
   __shared__ TileState S;
   __shared__ float wsum[kTB / 64];
@@ -977,7 +982,7 @@ __global__ __launch_bounds__(kTB, (tile_waves<DP, KIND <= 2 ? KIND : KIND + 4>()
   }
   const KArgs a = vols[lo].a;
   const int tile = t - vols[lo].tile0;
-  if (KIND <= 2) gather_tile<DP, kBTX, kBTY, kBTZ, CAPE, KG>(a, tile, 0, box, wsum, S);
+  if (GATHER) gather_tile<DP, kBTX, kBTY, kBTZ, CAPE, KG>(a, tile, 0, box, wsum, S);
   else push_tile<DP, kBTX, kBTY, kBTZ, CAPE, KIND == 4>(a, tile, 0, reinterpret_cast<acc_t*>(box), wsum, S);
 }
 
@@ -1023,7 +1028,7 @@ hipError_t launch_lds_add_rate(unsigned long long* out, int blocks, int iters, h
 // Zero what a batched push / objective accumulates into (graph-capturable).
 __global__ void k_batch_zero(const BatchVol* __restrict__ vols, int kind) {
   const BatchVol& b = vols[blockIdx.y];
-  PP_GLOBAL float* p = (kind == OP_OBJGRAD) ? b.a.out2 : b.a.out;
+  PP_GLOBAL float* p = (kind == OP_OBJGRAD || kind == OP_PULL_VJP) ? b.a.out2 : b.a.out;
   const long long n = b.zero_n;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     p[i] = 0.0f;
@@ -1046,6 +1051,7 @@ static const void* batch_kernel_for(Op op, int dp) {
     case OP_PULL: return batch_kernel<0>(dp);
     case OP_PULL_BWD: return batch_kernel<1>(dp);
     case OP_OBJGRAD: return batch_kernel<2>(dp);
+    case OP_PULL_VJP: return batch_kernel<5>(dp);
     case OP_PUSH: return batch_kernel<3>(dp);
     case OP_PUSHG: return batch_kernel<4>(dp);
     default: return nullptr;
@@ -1053,7 +1059,7 @@ static const void* batch_kernel_for(Op op, int dp) {
 }
 
 hipError_t launch_batch_zero(Op op, const BatchVol* dvols, int nvol, hipStream_t s) {
-  if (nvol <= 0 || !(op == OP_PUSH || op == OP_PUSHG || op == OP_OBJGRAD)) return hipSuccess;
+  if (nvol <= 0 || !(op == OP_PUSH || op == OP_PUSHG || op == OP_OBJGRAD || op == OP_PULL_VJP)) return hipSuccess;
   hipLaunchKernelGGL(k_batch_zero, dim3(512, (unsigned)nvol), dim3(256), 0, s, dvols, (int)op);
   return hipGetLastError();
 }

@@ -21,7 +21,7 @@ import torch
 
 from . import _lib
 
-OPS = {"pull": 0, "push": 3, "push_grad": 4, "pull_backward": 5, "objective": 7}
+OPS = {"pull": 0, "push": 3, "push_grad": 4, "pull_backward": 5, "objective": 7, "pull_vjp": 8}
 
 
 class PPVolume(ct.Structure):
@@ -72,7 +72,7 @@ class BatchPlan:
             C = f.shape[0]
             aux = v.get("aux")
             aux = aux.contiguous() if aux is not None else None
-            if op in ("pull", "pull_backward", "objective"):
+            if op in ("pull", "pull_backward", "objective", "pull_vjp"):
                 d0 = tuple(f.shape[-3:])
             else:
                 d0 = tuple(v["dim"])
@@ -88,9 +88,10 @@ class BatchPlan:
             elif tuple(out.shape) != shape or not out.is_contiguous():
                 raise AssertionError(f"`out` must be a contiguous {shape} tensor")
             out2 = loss = None
-            if op == "objective":
+            if op in ("objective", "pull_vjp"):
                 out2 = v.get("out2")
                 out2 = torch.empty_like(f) if out2 is None else out2
+            if op == "objective":
                 loss = v.get("loss")
                 loss = torch.zeros((), dtype=torch.float64, device=f.device) if loss is None else loss
             rec = dict(f=f, phi=phi, aux=aux, out=out, out2=out2, loss=loss)

@@ -168,3 +168,28 @@ def test_graph_of_tier_b_calls():
         for o, r in zip(outs, ref):
             _check(H(o), r, f"graph replay {rep}")
     g.close()
+
+
+def test_batch_pull_vjp_matches_per_volume_calls():
+    """One persistent launch per degree of the pull VJP over uneven volumes (degrees 0-4,
+    mixed boundaries, some samples far outside) against per-volume pp_pull_vjp calls, and
+    graph replays against the eager run."""
+    vols = _volumes(5)
+    plan = ppb.BatchPlan("pull_vjp", [_dev(v) for v in vols])
+    outs = plan.run()
+    torch.cuda.synchronize()
+    for i, (v, o) in enumerate(zip(vols, outs)):
+        s = pp.Settings(v["deg"], v["bnd"], v["ext"])
+        df, dphi = pp.pull_vjp(T(v["f"][None]), T(v["phi"][None]), T(v["aux"][None]), s)
+        _check(H(o["out"]), H(dphi[0]), f"vjp dphi vol {i}")
+        _check(H(o["out2"]), H(df[0]), f"vjp df vol {i}")
+    first = [(H(o["out"]).copy(), H(o["out2"]).copy()) for o in outs]
+    graph = ppb.Graph(lambda st: plan.run(st))
+    for _ in range(3):
+        graph.launch()
+        graph.stream.synchronize()
+        for i, ((a, b), o) in enumerate(zip(first, outs)):
+            _check(H(o["out"]), a, f"vjp replay dphi vol {i}")
+            _check(H(o["out2"]), b, f"vjp replay df vol {i}")
+    graph.close()
+    plan.close()
