@@ -6,10 +6,11 @@
 // space of the image grid; the box (<= CAP cells) lives in LDS:
 //   gathers  stage the box once (coalesced row loads, boundary fold and sign applied
 //            while staging), then every sample reads its dp^3 taps from LDS;
-//   scatters accumulate every tap with an LDS float atomic (ds_add_f32), then flush each
-//            non-zero cell with one folded global atomic -- a few global atomics per
-//            sample instead of dp^3, and none of the same-address contention of
-//            neighbouring samples hitting the same HBM lines.
+//   scatters accumulate every tap in fixed point with an LDS integer atomic (32-bit
+//            cells for values, 64-bit for push_grad), then flush each non-zero cell
+//            with one folded global atomic -- a few global atomics per sample instead
+//            of dp^3, and none of the same-address contention of neighbouring samples
+//            hitting the same HBM lines.
 // Samples whose knot lies outside a window around the tile's centre sample (displaced
 // far away, sent outside the field of view, or on the large-coordinate path) are
 // handled individually through global memory, so a few outliers never inflate the box.
@@ -113,9 +114,12 @@ struct NoBound {
 // Hooks, so that callers share its two barriers instead of adding their own:
 //   pre(V)    runs once the samples' grid indices are known, before the deformation is
 //             read: loads issued there overlap the deformation's;
-//   bound()   a per-thread value >= 0 whose workgroup maximum lands in *block_bound;
-//   zero_box  when given, the box's int64 cells are zeroed before the final barrier.
-template <int DP, int TX, int TY, int TZ, int CAP, class Pre = NoPrefetch, class Bound = NoBound>
+//   bound()   a per-thread value >= 0 whose workgroup maximum (NARROW: sum) lands in
+//             *block_bound;
+//   zero_box  when given, the box's cells (NARROW: 32-bit, else 64-bit) are zeroed
+//             before the final barrier.
+template <int DP, int TX, int TY, int TZ, int CAP, bool NARROW = false, class Pre = NoPrefetch,
+          class Bound = NoBound>
 __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restrict__ phi, int tile,
                                            Vox<Shape<TX, TY, TZ>::VPT>& V, TileState& S, Pre pre = Pre(),
                                            Bound bound = Bound(), acc_t* zero_box = nullptr,
@@ -184,7 +188,7 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
   mx0 = wave_max(mx0); mx1 = wave_max(mx1); mx2 = wave_max(mx2);
   float bnd = bound();
   if (block_bound)
-    bnd = __ockl_wfred_max_f32(bnd);
+    bnd = NARROW ? __ockl_wfred_add_f32(bnd) : __ockl_wfred_max_f32(bnd);
   const int wid = tid >> 6;
   if ((tid & 63) == 0) {
     S.red[0][wid] = mn0; S.red[1][wid] = mn1; S.red[2][wid] = mn2;
@@ -195,7 +199,7 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
   if (block_bound) {
     float m = S.redf[0];
 #pragma unroll
-    for (int w = 1; w < kTB / 64; ++w) m = fmaxf(m, S.redf[w]);
+    for (int w = 1; w < kTB / 64; ++w) m = NARROW ? m + S.redf[w] : fmaxf(m, S.redf[w]);
     *block_bound = m;
   }
   // every thread sizes the box (a broadcast read of the partials)
@@ -220,8 +224,8 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
     S.cells = cells;
     S.lds = use;
   }
-  if (use && zero_box)
-    for (int i = tid; i < cells; i += kTB) zero_box[i] = 0ull;
+  if (use && zero_box)  // (32-bit cells: as many 64-bit words as cover them)
+    for (int i = tid; i < (NARROW ? (cells + 1) >> 1 : cells); i += kTB) zero_box[i] = 0ull;
   if (use) {  // fold tables of the box planes
     bool neg;
     if (tid < by) {
@@ -317,18 +321,66 @@ __device__ __forceinline__ void stage_box(float* box, const TileState& S, const 
 
 // ---- fixed-point scatter accumulation ---------------------------------------------
 // gfx950 executes float atomics (LDS ds_add_f32 and global alike) at a small fraction
-// of the rate of integer LDS atomics, so scatters accumulate into an int64 LDS box
-// with ds_add_u64: every contribution c is stored as round(c * 2^scale). Per channel
-// and tile, scale = 50 - e where 2^e bounds the largest contribution: every scaled
-// contribution is below 2^50 and no cell can overflow (at most 2^10 samples per tile
-// touch a cell), integer sums are exact and order-independent, and each cell is
-// rounded to float once, when it is flushed.
+// of the rate of integer LDS atomics, so scatters accumulate into a fixed-point LDS
+// box with integer adds: every contribution c is stored as round(c * 2^scale). In
+// 64-bit cells (ds_add_u64), per channel and tile, scale = 50 - e where 2^e bounds the
+// largest contribution: every scaled contribution is below 2^50 and no cell can
+// overflow (at most 2^10 samples per tile touch a cell). Integer sums are exact and
+// order-independent, and each cell is rounded to float once, when it is flushed.
 __device__ __forceinline__ int fixed_scale(float vmax) {
   if (!(vmax > 0.0f)) return 0;
   int e;
   (void)frexpf(vmax, &e);  // vmax < 2^e
   return 50 - e;
 }
+
+// Scatters of values (push, and the residual scatter of the objective and of the VJP)
+// use 32-bit cells and ds_add_u32, which retires ~1.4x as many lanes per clock as
+// ds_add_u64. Wrapping two's-complement sums are exact in any order, so only a cell's
+// final value has to fit: no cell exceeds wmax^3 * sum|v| over the tile's samples (wmax
+// the largest 1-D B-spline weight, every tap weight lies in [0, wmax^3]), so with scale
+// = 30 - e, 2^e bounding that, a cell stays below 2^30 plus the rounding of its at most
+// 2^10 contributions. The quantum is at most 2^-29 * wmax^3 * 1024 times the tile's
+// largest value (2^-21 of it for cubic splines) and finer where values vary. push_grad
+// keeps 64-bit cells.
+template <int DP, bool GRAD>
+constexpr bool narrow_cells() {
+  return !GRAD;
+}
+
+template <int DP>
+constexpr float wmax3() {  // (largest weight of the degree DP-1 B-spline)^3
+  return DP <= 2 ? 1.0f : DP == 3 ? 0.421875f : DP == 4 ? 0.2962963f : 0.2148771f;
+}
+
+template <int DP>
+__device__ __forceinline__ int fixed_scale32(float vsum) {
+  const float bound = vsum * (wmax3<DP>() * 1.0001f);
+  if (!(bound > 0.0f)) return 0;
+  int e = 139;  // a sum beyond the float range is below 2^139 (2^10 finite values)
+  if (bound <= __FLT_MAX__) (void)frexpf(bound, &e);
+  return 30 - e;
+}
+
+template <bool NARROW>
+struct CellOf {
+  typedef acc_t T;
+};
+template <>
+struct CellOf<true> {
+  typedef unsigned T;
+};
+
+template <class T>
+struct Cell;
+template <>
+struct Cell<acc_t> {
+  static __device__ __forceinline__ long long q(acc_t v) { return (long long)v; }
+};
+template <>
+struct Cell<unsigned> {
+  static __device__ __forceinline__ long long q(unsigned v) { return (long long)(int)v; }
+};
 
 // round(a * b) as int64 for |a * b| < 2^51: the FMA with 2^52 + 2^51 rounds the product
 // to an integer held in the low mantissa bits (one FMA instead of a float -> int64
@@ -352,30 +404,33 @@ __device__ __forceinline__ float fixed_to_float(long long q, int scale) {
   return ldexpf((float)q, -scale);
 }
 
-__device__ __forceinline__ void zero_acc(acc_t* box, int cells) {
-  for (int i = threadIdx.x; i < cells; i += kTB) box[i] = 0ull;
+template <class T>
+__device__ __forceinline__ void zero_acc(T* box, int cells) {
+  for (int i = threadIdx.x; i < cells; i += kTB) box[i] = 0;
 }
 
-// Largest value over the workgroup (v >= 0); uses and releases wsum.
+// Largest value (SUM: the sum) over the workgroup (v >= 0); uses and releases wsum.
+template <bool SUM = false>
 __device__ __forceinline__ float block_max(float v, float* wsum) {
-  v = __ockl_wfred_max_f32(v);
+  v = SUM ? __ockl_wfred_add_f32(v) : __ockl_wfred_max_f32(v);
   if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
   __syncthreads();
   float m = wsum[0];
-  for (int w = 1; w < kTB / 64; ++w) m = fmaxf(m, wsum[w]);
+  for (int w = 1; w < kTB / 64; ++w) m = SUM ? m + wsum[w] : fmaxf(m, wsum[w]);
   __syncthreads();
   return m;
 }
 
-// Flush the int64 box into image channel `dst`: one folded global float atomic per
+// Flush the fixed-point box into image channel `dst`: one folded global float atomic per
 // non-zero cell.
-__device__ __forceinline__ void flush_acc(const acc_t* box, const TileState& S, int scale, float* __restrict__ dst,
+template <class T>
+__device__ __forceinline__ void flush_acc(const T* box, const TileState& S, int scale, float* __restrict__ dst,
                                           const KArgs& a, unsigned signs) {
   BoxWalk w = box_walk(S, a);
   if (!w.active) return;
   const int BX = S.b[0], BY = S.b[1];
   for (; w.r < w.rows; walk_next(w, BY)) {
-    const long long q = (long long)box[w.r * BX + w.c];
+    const long long q = Cell<T>::q(box[w.r * BX + w.c]);
     if (q != 0) {
       bool neg;
       const int off = walk_offset(S, w, signs, neg);
@@ -419,11 +474,15 @@ __device__ __forceinline__ void lsample_h(float x, float y, float z, const TileS
 
 // ---- push / push_grad -------------------------------------------------------------
 // One tile of push (GRAD=false) or push_grad (GRAD=true) for batch index bb. The caller
-// provides the int64 LDS box (CAP cells), wsum (kTB/64 floats) and the tile state.
+// provides the LDS box (CAP int64 cells; value scatters use it as 32-bit cells), wsum
+// (kTB/64 floats) and the tile state.
 template <int DP, int TX, int TY, int TZ, int CAP, bool GRAD>
 __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, acc_t* box, float* wsum, TileState& S) {
   constexpr int VPT = Shape<TX, TY, TZ>::VPT;
   constexpr int NG = GRAD ? 3 : 1;
+  constexpr bool NW = narrow_cells<DP, GRAD>();
+  typedef typename CellOf<NW>::T C;
+  C* cbox = reinterpret_cast<C*>(box);
   const float* phi = a.phi + bb * a.phi_bs;
   const long long n1 = a.n1;
   // the first channel's values are loaded before the deformation is read
@@ -436,21 +495,22 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, acc_
 #pragma unroll
       for (int e = 0; e < NG; ++e) g[e][q] = (read0 && ((W.inside >> q) & 1u)) ? src0[W.v[q] + e * n1] : 0.f;
   };
-  // channel 0's scale bound: the largest finite per-sample bound over all samples in the
-  // grid (a superset of the in-window ones, so a valid bound), reduced with the box
+  // channel 0's scale bound: the largest (32-bit cells: the sum of the) finite per-sample
+  // bound over all samples in the grid (a superset of the in-window ones, so a valid
+  // bound), reduced with the box
   auto bound0 = [&]() {
     float m = 0.f;
 #pragma unroll
     for (int q = 0; q < VPT; ++q) {
       float b = fabsf(g[0][q]);
       if (GRAD) b += fabsf(g[NG > 1 ? 1 : 0][q]) + fabsf(g[NG > 2 ? 2 : 0][q]);
-      if (isfinite(b)) m = fmaxf(m, b);
+      if (isfinite(b)) m = NW ? m + b : fmaxf(m, b);
     }
     return m;
   };
   Vox<VPT> V;
   float bound_c0 = 0.f;
-  tile_setup<DP, TX, TY, TZ, CAP>(a, phi, tile, V, S, prefetch, bound0, box, &bound_c0);
+  tile_setup<DP, TX, TY, TZ, CAP, NW>(a, phi, tile, V, S, prefetch, bound0, box, &bound_c0);
   const bool lds = S.lds != 0;
   const int BX = S.b[0], BXY = S.b[0] * S.b[1];
   // push with a count image (out2): one more channel of ones after the nchan channels
@@ -485,14 +545,15 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, acc_
         float b = fabsf(g[0][q]);
         if (GRAD) b += fabsf(g[NG > 1 ? 1 : 0][q]) + fabsf(g[NG > 2 ? 2 : 0][q]);
         if (!((use >> q) & 1u)) continue;
-        if (isfinite(b)) vmax = fmaxf(vmax, b);
+        if (isfinite(b)) vmax = NW ? vmax + b : fmaxf(vmax, b);
         else use &= ~(1u << q);
       }
       if (c == 0 && read0) {  // bound reduced and box zeroed within tile_setup's barriers
-        scale = fixed_scale(bound_c0);
+        scale = NW ? fixed_scale32<DP>(bound_c0) : fixed_scale(bound_c0);
       } else {
-        scale = fixed_scale(block_max(vmax, wsum));
-        zero_acc(box, S.cells);
+        const float m = block_max<NW>(vmax, wsum);
+        scale = NW ? fixed_scale32<DP>(m) : fixed_scale(m);
+        zero_acc(cbox, S.cells);
         __syncthreads();
       }
     }
@@ -514,16 +575,16 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, acc_
         for (int k = 0; k < DP; ++k)
 #pragma unroll
           for (int j = 0; j < DP; ++j) {
-            acc_t* row = box + L.base + BX * j + BXY * k;
+            C* row = cbox + L.base + BX * j + BXY * k;
             if (GRAD) {
               const double cx = (double)(L.wz[k] * L.wy[j]) * s0;
               const double cyz = (double)(L.wz[k] * L.dy[j]) * s1 + (double)(L.dz[k] * L.wy[j]) * s2;
 #pragma unroll
-              for (int i = 0; i < DP; ++i) atomicAdd(row + i, fix2(L.dx[i], cx, L.wx[i], cyz));
+              for (int i = 0; i < DP; ++i) atomicAdd(row + i, (C)fix2(L.dx[i], cx, L.wx[i], cyz));
             } else {
               const double wv = (double)(L.wz[k] * L.wy[j]) * s0;
 #pragma unroll
-              for (int i = 0; i < DP; ++i) atomicAdd(row + i, fix1(L.wx[i], wv));
+              for (int i = 0; i < DP; ++i) atomicAdd(row + i, (C)fix1(L.wx[i], wv));
             }
           }
       } else {
@@ -538,7 +599,7 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, acc_
     }
     if (lds) {
       __syncthreads();
-      flush_acc(box, S, scale, dst, a, chan_signs(a, c));
+      flush_acc(cbox, S, scale, dst, a, chan_signs(a, c));
       __syncthreads();
     }
   }
@@ -822,14 +883,17 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
     }
     if (lds) {
       __syncthreads();
-      if (KIND == 2 || KIND == 3) {  // scatter the residuals (gradients) of the in-window samples via the int64 box
-        float vmax = 0.f;
+      if (KIND == 2 || KIND == 3) {  // scatter the residuals (gradients) of the in-window samples via the fixed-point box
+        constexpr bool NW = narrow_cells<DP, false>();
+        typedef typename CellOf<NW>::T C;
+        float vb = 0.f;
 #pragma unroll
         for (int q = 0; q < VPT; ++q)
-          if ((rfin >> q) & 1u) vmax = fmaxf(vmax, fabsf(res[q]));
-        const int scale = fixed_scale(block_max(vmax, wsum));  // also orders the staging reads
-        acc_t* box64 = reinterpret_cast<acc_t*>(box);
-        zero_acc(box64, S.cells);
+          if ((rfin >> q) & 1u) vb = NW ? vb + fabsf(res[q]) : fmaxf(vb, fabsf(res[q]));
+        const float m = block_max<NW>(vb, wsum);  // also orders the staging reads
+        const int scale = NW ? fixed_scale32<DP>(m) : fixed_scale(m);
+        C* cbox = reinterpret_cast<C*>(box);
+        zero_acc(cbox, S.cells);
         __syncthreads();
 #pragma unroll 1
         for (int q = 0; q < VPT; ++q, V.rotate(), rot(res)) {
@@ -841,14 +905,14 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
           for (int k = 0; k < DP; ++k)
 #pragma unroll
             for (int j = 0; j < DP; ++j) {
-              acc_t* row = box64 + L.base + BX * j + BXY * k;
+              C* row = cbox + L.base + BX * j + BXY * k;
               const double wv = (double)(L.wz[k] * L.wy[j]) * ws;
 #pragma unroll
-              for (int i = 0; i < DP; ++i) atomicAdd(row + i, fix1(L.wx[i], wv));
+              for (int i = 0; i < DP; ++i) atomicAdd(row + i, (C)fix1(L.wx[i], wv));
             }
         }
         __syncthreads();
-        flush_acc(box64, S, scale, a.out2 + bb * a.out2_bs + c * a.out2_cs, a, chan_signs(a, c));
+        flush_acc(cbox, S, scale, a.out2 + bb * a.out2_bs + c * a.out2_cs, a, chan_signs(a, c));
         __syncthreads();
       }
     }
@@ -888,7 +952,7 @@ __global__ __launch_bounds__(kTB, (tile_waves<DP, KIND>())) void k_gather_tile(K
 // Box budgets in cells, per degree: small boxes let more workgroups share a CU, whose
 // latency chains (load phi, size the box, stage, compute) then overlap. A tile whose box
 // exceeds the budget runs per sample through global memory. Gathers stage floats;
-// scatters and the objective accumulate int64 cells.
+// scatters and the objective accumulate fixed-point cells (budgets in int64 cells).
 template <int DP, int KIND>
 constexpr int cap_cells() {
   return (KIND == 0 || KIND == 1) ? (DP <= 2 ? 4096 : DP == 3 ? 6400 : 9472)
@@ -989,16 +1053,17 @@ __global__ __launch_bounds__(kTB, (tile_waves<DP, batch_waves_kind(KIND)>())) vo
 int batch_tiles(const KArgs& a) { return tiles_of<kBTX, kBTY, kBTZ>(a); }
 
 // ---- LDS atomic calibration --------------------------------------------------------
-// The rate of ds_add_u64 under the scatter's own access pattern, the on-chip bound of
-// push and the VJP: every lane adds the 4x4x4 taps of cubic samples into a box shaped like
+// The rate of LDS integer adds on cells of type T (the cubic value scatter's: 32-bit)
+// under the scatter's own access pattern, the on-chip bound of push and the VJP: every lane adds the 4x4x4 taps of cubic samples into a box shaped like
 // a 16x8x8 tile's at degree 3 (20 x 12 x 12 cells), its knots following the lane's x and y
 // (z: 2 per repetition, as a thread's samples), a third of the lanes shifted by a cell
 // along x and a sixth along y as under a smooth displacement. `iters` samples per lane, one
-// ds_add_u64 per tap; out (gridDim.x * 256 words) receives checksums of the box.
+// atomic add per tap; out (gridDim.x * 256 words) receives checksums of the box.
+template <class T>
 __global__ __launch_bounds__(kTB) void k_lds_add_rate(acc_t* __restrict__ out, int iters) {
   constexpr int BXc = 20, BYc = 12, BZc = 12, CELLS = BXc * BYc * BZc;
-  __shared__ acc_t cal[CELLS];
-  for (int i = threadIdx.x; i < CELLS; i += kTB) cal[i] = 0ull;
+  __shared__ T cal[CELLS];
+  for (int i = threadIdx.x; i < CELLS; i += kTB) cal[i] = 0;
   __syncthreads();
   const int l = threadIdx.x;
   const unsigned h = (blockIdx.x * kTB + l) * 2654435761u;
@@ -1012,16 +1077,17 @@ __global__ __launch_bounds__(kTB) void k_lds_add_rate(acc_t* __restrict__ out, i
 #pragma unroll
       for (int j = 0; j < 4; ++j)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) atomicAdd(cal + b + i + BXc * j + BXc * BYc * k, (acc_t)(it + i + 1));
+        for (int i = 0; i < 4; ++i) atomicAdd(cal + b + i + BXc * j + BXc * BYc * k, (T)(it + i + 1));
   }
   __syncthreads();
-  acc_t sum = 0ull;
+  T sum = 0;
   for (int i = l; i < CELLS; i += kTB) sum += cal[i];
-  out[(size_t)blockIdx.x * kTB + l] = sum;
+  out[(size_t)blockIdx.x * kTB + l] = (acc_t)sum;
 }
 
 hipError_t launch_lds_add_rate(unsigned long long* out, int blocks, int iters, hipStream_t s) {
-  hipLaunchKernelGGL(k_lds_add_rate, dim3((unsigned)blocks), dim3(kTB), 0, s, out, iters);
+  hipLaunchKernelGGL((k_lds_add_rate<typename CellOf<narrow_cells<4, false>()>::T>), dim3((unsigned)blocks), dim3(kTB),
+                     0, s, out, iters);
   return hipGetLastError();
 }
 
