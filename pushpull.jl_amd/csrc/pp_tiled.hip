@@ -327,23 +327,25 @@ __device__ __forceinline__ void stage_box(float* box, const TileState& S, const 
 // largest contribution: every scaled contribution is below 2^50 and no cell can
 // overflow (at most 2^10 samples per tile touch a cell). Integer sums are exact and
 // order-independent, and each cell is rounded to float once, when it is flushed.
-__device__ __forceinline__ int fixed_scale(float vmax) {
-  if (!(vmax > 0.0f)) return 0;
-  int e;
-  (void)frexpf(vmax, &e);  // vmax < 2^e
-  return 50 - e;
+
+// A channel's conversion factor mul: contribution c -> round(c * mul); cell q -> q / mul.
+__device__ __forceinline__ double fixed_scale(float vmax) {
+  int e = 50;                               // (vmax = 0: scale 0)
+  if (vmax > 0.0f) (void)frexpf(vmax, &e);  // vmax < 2^e
+  return ldexp(1.0, 50 - e);                // a power of two: products stay exact
 }
 
 // Scatters of values (push, and the residual scatter of the objective and of the VJP)
-// use 32-bit cells and ds_add_u32, which retires ~1.4x as many lanes per clock as
+// use 32-bit cells and ds_add_u32, which retires ~1.3x as many lanes per clock as
 // ds_add_u64. Wrapping two's-complement sums are exact in any order, so only a cell's
-// final value has to fit: no cell exceeds wmax^3 * sum|v| over the tile's samples (wmax
-// the largest 1-D B-spline weight, every tap weight lies in [0, wmax^3]), so with scale
-// = 30 - e, 2^e bounding that, a cell stays below 2^30 plus the rounding of its at most
-// 2^10 contributions. The quantum is at most 2^-29 * wmax^3 * 1024 times the tile's
-// largest value (2^-21 of it for cubic splines) and finer where values vary. push_grad
-// keeps 64-bit cells.
-template <int DP, bool GRAD>
+// final value has to fit: no cell exceeds B = wmax^3 * sum|v| over the tile's samples
+// (wmax the largest 1-D B-spline weight, every tap weight lies in [0, wmax^3]). With
+// mul = (2^31 - 2^10) / B a cell stays below 2^31 even with the rounding of its at most
+// 2^10 contributions added. mul is no power of two: the scaled products round once more
+// in double (2^-53 relative, far below the quantum). The quantum 1/mul is at most
+// 2^-31 * wmax^3 * 1024 times the tile's largest value (1.4e-7 of it for cubic splines)
+// and finer where values vary. push_grad keeps 64-bit cells.
+template <bool GRAD>
 constexpr bool narrow_cells() {
   return !GRAD;
 }
@@ -354,12 +356,12 @@ constexpr float wmax3() {  // (largest weight of the degree DP-1 B-spline)^3
 }
 
 template <int DP>
-__device__ __forceinline__ int fixed_scale32(float vsum) {
-  const float bound = vsum * (wmax3<DP>() * 1.0001f);
-  if (!(bound > 0.0f)) return 0;
-  int e = 139;  // a sum beyond the float range is below 2^139 (2^10 finite values)
-  if (bound <= __FLT_MAX__) (void)frexpf(bound, &e);
-  return 30 - e;
+__device__ __forceinline__ double fixed_scale32(float vsum) {
+  // (1.0001: the float sum and the float weights may fall short of the exact bound by
+  // a few parts in 10^5; a sum beyond the float range is below 2^10 * FLT_MAX)
+  const double s = vsum <= __FLT_MAX__ ? (double)vsum : 1024.0 * (double)__FLT_MAX__;
+  const double bound = s * (double)wmax3<DP>() * 1.0001;
+  return bound > 0.0 ? (2147483648.0 - 1024.0) / bound : 1.0;
 }
 
 template <bool NARROW>
@@ -394,14 +396,14 @@ __device__ __forceinline__ acc_t fix2(double a, double b, double c, double d) {
   return (acc_t)(__double_as_longlong(fma(a, b, fma(c, d, kMagic))) - __double_as_longlong(kMagic));
 }
 
-// q * 2^-scale rounded to float: below 2^51 in magnitude q converts exactly through the
-// same magic constant (as fix1, in reverse); larger sums take the general conversion.
-__device__ __forceinline__ float fixed_to_float(long long q, int scale) {
+// q * inv rounded to float: below 2^51 in magnitude q converts exactly through the same
+// magic constant (as fix1, in reverse); larger sums take the general conversion.
+__device__ __forceinline__ float fixed_to_float(long long q, double inv) {
   if (q > -(1LL << 51) && q < (1LL << 51)) {
     const double d = __longlong_as_double(q + __double_as_longlong(kMagic)) - kMagic;
-    return (float)ldexp(d, -scale);
+    return (float)(d * inv);
   }
-  return ldexpf((float)q, -scale);
+  return (float)((double)q * inv);
 }
 
 template <class T>
@@ -424,17 +426,18 @@ __device__ __forceinline__ float block_max(float v, float* wsum) {
 // Flush the fixed-point box into image channel `dst`: one folded global float atomic per
 // non-zero cell.
 template <class T>
-__device__ __forceinline__ void flush_acc(const T* box, const TileState& S, int scale, float* __restrict__ dst,
+__device__ __forceinline__ void flush_acc(const T* box, const TileState& S, double mul, float* __restrict__ dst,
                                           const KArgs& a, unsigned signs) {
   BoxWalk w = box_walk(S, a);
   if (!w.active) return;
   const int BX = S.b[0], BY = S.b[1];
+  const double inv = 1.0 / mul;  // (exact for the powers of two of 64-bit cells)
   for (; w.r < w.rows; walk_next(w, BY)) {
     const long long q = Cell<T>::q(box[w.r * BX + w.c]);
     if (q != 0) {
       bool neg;
       const int off = walk_offset(S, w, signs, neg);
-      const float v = fixed_to_float(q, scale);
+      const float v = fixed_to_float(q, inv);
       unsafeAtomicAdd(dst + off, neg ? -v : v);
     }
   }
@@ -480,7 +483,7 @@ template <int DP, int TX, int TY, int TZ, int CAP, bool GRAD>
 __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, acc_t* box, float* wsum, TileState& S) {
   constexpr int VPT = Shape<TX, TY, TZ>::VPT;
   constexpr int NG = GRAD ? 3 : 1;
-  constexpr bool NW = narrow_cells<DP, GRAD>();
+  constexpr bool NW = narrow_cells<GRAD>();
   typedef typename CellOf<NW>::T C;
   C* cbox = reinterpret_cast<C*>(box);
   const float* phi = a.phi + bb * a.phi_bs;
@@ -536,7 +539,7 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, acc_
       }
     }
     unsigned use = lds ? V.inl : 0u;  // samples accumulated in LDS for this channel
-    int scale = 0;
+    double scale = 1.0;
     if (lds) {
       // bound on |contribution| (|w|, |dw| <= 1); non-finite values take the global path
       float vmax = 0.f;
@@ -568,9 +571,9 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, acc_
       if ((use >> q) & 1u) {
         LSample<DP> L;
         lsample<DP, GRAD>(x, y, z, S, L);
-        const double s0 = ldexp((double)g0, scale);  // exact
-        const double s1 = GRAD ? ldexp((double)g1, scale) : 0.0;
-        const double s2 = GRAD ? ldexp((double)g2, scale) : 0.0;
+        const double s0 = (double)g0 * scale;
+        const double s1 = GRAD ? (double)g1 * scale : 0.0;
+        const double s2 = GRAD ? (double)g2 * scale : 0.0;
 #pragma unroll
         for (int k = 0; k < DP; ++k)
 #pragma unroll
@@ -620,7 +623,7 @@ constexpr int tile_waves() {
 
 template <int DP, int TX, int TY, int TZ, int CAP, bool GRAD>
 __global__ __launch_bounds__(kTB, (tile_waves<DP, GRAD ? 8 : 7>())) void k_push_tile(KArgs a) {
-  __shared__ acc_t box[CAP];
+  __shared__ acc_t box[narrow_cells<GRAD>() ? (CAP + 1) / 2 : CAP];
   __shared__ TileState S;
   __shared__ float wsum[kTB / 64];
   push_tile<DP, TX, TY, TZ, CAP, GRAD>(a, xcd_block(blockIdx.x, gridDim.x), blockIdx.y, box, wsum, S);
@@ -733,11 +736,11 @@ __device__ __forceinline__ float outlier_grad(const KArgs& a, const float* __res
   return w;
 }
 
-// LDS box of a gather kind, in floats. The scattering kinds' box is twice as large: it
-// first holds the staged image (float) and then, reused, the int64 scatter accumulator.
+// LDS box of a gather kind, in floats. The scattering kinds' box first holds the staged
+// image (float) and then, reused, the scatter accumulator (twice as large in 64-bit cells).
 template <int CAP, int KIND>
 constexpr int box_floats() {
-  return (KIND == 2 || KIND == 3) ? 2 * CAP : CAP;
+  return (KIND == 2 || KIND == 3) && !narrow_cells<false>() ? 2 * CAP : CAP;
 }
 
 // One tile of pull (KIND 0), the pull pullback (1), the fused objective (2), the VJP
@@ -884,14 +887,14 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
     if (lds) {
       __syncthreads();
       if (KIND == 2 || KIND == 3) {  // scatter the residuals (gradients) of the in-window samples via the fixed-point box
-        constexpr bool NW = narrow_cells<DP, false>();
+        constexpr bool NW = narrow_cells<false>();
         typedef typename CellOf<NW>::T C;
         float vb = 0.f;
 #pragma unroll
         for (int q = 0; q < VPT; ++q)
           if ((rfin >> q) & 1u) vb = NW ? vb + fabsf(res[q]) : fmaxf(vb, fabsf(res[q]));
         const float m = block_max<NW>(vb, wsum);  // also orders the staging reads
-        const int scale = NW ? fixed_scale32<DP>(m) : fixed_scale(m);
+        const double scale = NW ? fixed_scale32<DP>(m) : fixed_scale(m);
         C* cbox = reinterpret_cast<C*>(box);
         zero_acc(cbox, S.cells);
         __syncthreads();
@@ -900,7 +903,7 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
           if (!((rfin >> q) & 1u)) continue;
           LSample<DP> L;
           lsample<DP, false>(V.x[0], V.y[0], V.z[0], S, L);
-          const double ws = ldexp((double)res[0], scale);  // exact
+          const double ws = (double)res[0] * scale;
 #pragma unroll
           for (int k = 0; k < DP; ++k)
 #pragma unroll
@@ -1030,7 +1033,7 @@ __global__ __launch_bounds__(kTB, (tile_waves<DP, batch_waves_kind(KIND)>())) vo
   constexpr int KG = batch_gather_kind(KIND) >= 0 ? batch_gather_kind(KIND) : 0;
   constexpr bool GATHER = batch_gather_kind(KIND) >= 0;
   constexpr int CAPE = (DP >= 2 && DP <= 5) ? cap_cells<DP, KIND>() : 0;
-  constexpr int BOXF = GATHER ? box_floats<CAPE, KG>() : 2 * CAPE;  // push: CAPE int64 cells
+  constexpr int BOXF = GATHER ? box_floats<CAPE, KG>() : narrow_cells<KIND == 4>() ? CAPE : 2 * CAPE;  // push cells
   __shared__ __align__(8) float box[BOXF > 0 ? BOXF : 2];
   __shared__ TileState S;
   __shared__ float wsum[kTB / 64];
@@ -1086,7 +1089,7 @@ __global__ __launch_bounds__(kTB) void k_lds_add_rate(acc_t* __restrict__ out, i
 }
 
 hipError_t launch_lds_add_rate(unsigned long long* out, int blocks, int iters, hipStream_t s) {
-  hipLaunchKernelGGL((k_lds_add_rate<typename CellOf<narrow_cells<4, false>()>::T>), dim3((unsigned)blocks), dim3(kTB),
+  hipLaunchKernelGGL((k_lds_add_rate<typename CellOf<narrow_cells<false>()>::T>), dim3((unsigned)blocks), dim3(kTB),
                      0, s, out, iters);
   return hipGetLastError();
 }
