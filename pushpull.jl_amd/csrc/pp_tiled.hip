@@ -616,8 +616,8 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, acc_
 template <int DP, int KIND>  // KIND: gather kinds 0-6, push 7, push_grad 8
 constexpr int tile_waves() {
   return DP == 2 ? (KIND == 3 ? 5 : KIND == 7 ? 6 : 0)
-       : DP == 3 ? (KIND == 0 || KIND == 4 ? 6 : KIND == 1 || KIND == 5 ? 5 : KIND == 2 || KIND == 3 ? 4 : 0)
-       : DP == 4 ? (KIND == 2 ? 3 : KIND == 6 ? 4 : 0)
+       : DP == 3 ? (KIND == 0 || KIND == 4 ? 6 : KIND == 1 || KIND == 5 || KIND == 7 ? 5 : KIND == 2 || KIND == 3 ? 4 : 0)
+       : DP == 4 ? (KIND == 2 ? 3 : KIND == 6 ? 4 : KIND == 7 ? 5 : 0)
                  : 0;
 }
 
