@@ -477,8 +477,8 @@ __device__ __forceinline__ void lsample_h(float x, float y, float z, const TileS
 
 // ---- push / push_grad -------------------------------------------------------------
 // One tile of push (GRAD=false) or push_grad (GRAD=true) for batch index bb. The caller
-// provides the LDS box (CAP int64 cells; value scatters use it as 32-bit cells), wsum
-// (kTB/64 floats) and the tile state.
+// provides the LDS box (CAP cells: 32-bit for push, 64-bit for push_grad, 8-byte aligned),
+// wsum (kTB/64 floats) and the tile state.
 template <int DP, int TX, int TY, int TZ, int CAP, bool GRAD>
 __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, acc_t* box, float* wsum, TileState& S) {
   constexpr int VPT = Shape<TX, TY, TZ>::VPT;
@@ -775,7 +775,7 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
   for (int c = 0; c < a.nchan; ++c) {
     const float* src = a.f + bb * a.f_bs + c * a.f_cs;
     unsigned rfin = 0u;  // KIND 2/3: in-window samples with a finite weight (scattered via LDS)
-    float res[VPT];      // KIND 2/3: residual / upstream gradient of each sample
+    float res[VPT];      // KIND 2: residual of each sample (KIND 3 scatters auxv itself)
 #pragma unroll
     for (int q = 0; q < VPT; ++q) {
       if (c > 0)  // loaded before staging so both overlap
@@ -873,7 +873,7 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
         lds_grad<DP, KIND == 2>(box, L, BX, BXY, val, gx, gy, gz);
         w = (KIND == 2) ? val - aux : aux;
         if ((KIND == 2 || KIND == 3) && isfinite(w)) {
-          res[0] = w;
+          if (KIND == 2) res[0] = w;
           rfin |= 1u << q;
         }
       } else {
@@ -892,18 +892,21 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
         float vb = 0.f;
 #pragma unroll
         for (int q = 0; q < VPT; ++q)
-          if ((rfin >> q) & 1u) vb = NW ? vb + fabsf(res[q]) : fmaxf(vb, fabsf(res[q]));
+          if ((rfin >> q) & 1u) {
+            const float r = fabsf(KIND == 3 ? auxv[q] : res[q]);
+            vb = NW ? vb + r : fmaxf(vb, r);
+          }
         const float m = block_max<NW>(vb, wsum);  // also orders the staging reads
         const double scale = NW ? fixed_scale32<DP>(m) : fixed_scale(m);
         C* cbox = reinterpret_cast<C*>(box);
         zero_acc(cbox, S.cells);
         __syncthreads();
 #pragma unroll 1
-        for (int q = 0; q < VPT; ++q, V.rotate(), rot(res)) {
+        for (int q = 0; q < VPT; ++q, V.rotate(), rot(res), rot(auxv)) {
           if (!((rfin >> q) & 1u)) continue;
           LSample<DP> L;
           lsample<DP, false>(V.x[0], V.y[0], V.z[0], S, L);
-          const double ws = (double)res[0] * scale;
+          const double ws = (double)(KIND == 3 ? auxv[0] : res[0]) * scale;
 #pragma unroll
           for (int k = 0; k < DP; ++k)
 #pragma unroll
@@ -955,7 +958,7 @@ __global__ __launch_bounds__(kTB, (tile_waves<DP, KIND>())) void k_gather_tile(K
 // Box budgets in cells, per degree: small boxes let more workgroups share a CU, whose
 // latency chains (load phi, size the box, stage, compute) then overlap. A tile whose box
 // exceeds the budget runs per sample through global memory. Gathers stage floats;
-// scatters and the objective accumulate fixed-point cells (budgets in int64 cells).
+// scatters and the objective accumulate fixed-point cells (32-bit, push_grad 64-bit).
 template <int DP, int KIND>
 constexpr int cap_cells() {
   return (KIND == 0 || KIND == 1) ? (DP <= 2 ? 4096 : DP == 3 ? 6400 : 9472)
