@@ -774,20 +774,19 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
   float lsum = 0.f;
   for (int c = 0; c < a.nchan; ++c) {
     const float* src = a.f + bb * a.f_bs + c * a.f_cs;
-    unsigned rfin = 0u;  // KIND 2/3: in-window samples with a finite weight (scattered via LDS)
-    float res[VPT];      // KIND 2: residual of each sample (KIND 3 scatters auxv itself)
+    // KIND 2/3: in-window samples with a finite weight, scattered via LDS; their weight
+    // (the residual, which replaces the target in auxv, or the upstream gradient) is auxv
+    unsigned rfin = 0u;
 #pragma unroll
-    for (int q = 0; q < VPT; ++q) {
+    for (int q = 0; q < VPT; ++q)
       if (c > 0)  // loaded before staging so both overlap
         auxv[q] = (AUX && ((V.valid >> q) & 1u)) ? a.aux[bb * a.aux_bs + c * a.aux_cs + V.v[q]] : 0.f;
-      res[q] = 0.f;
-    }
     if (lds) {
       stage_box(box, S, src, a, chan_signs(a, c));
       __syncthreads();
     }
 #pragma unroll 1
-    for (int q = 0; q < VPT; ++q, V.rotate(), rot(auxv), rot(res), rot_rows(gsum)) {
+    for (int q = 0; q < VPT; ++q, V.rotate(), rot(auxv), rot_rows(gsum)) {
       if (!((V.inside >> q) & 1u)) continue;
       const int v = V.v[0];
       if (!((V.valid >> q) & 1u)) {
@@ -873,7 +872,7 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
         lds_grad<DP, KIND == 2>(box, L, BX, BXY, val, gx, gy, gz);
         w = (KIND == 2) ? val - aux : aux;
         if ((KIND == 2 || KIND == 3) && isfinite(w)) {
-          if (KIND == 2) res[0] = w;
+          if (KIND == 2) auxv[0] = w;
           rfin |= 1u << q;
         }
       } else {
@@ -893,7 +892,7 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
 #pragma unroll
         for (int q = 0; q < VPT; ++q)
           if ((rfin >> q) & 1u) {
-            const float r = fabsf(KIND == 3 ? auxv[q] : res[q]);
+            const float r = fabsf(auxv[q]);
             vb = NW ? vb + r : fmaxf(vb, r);
           }
         const float m = block_max<NW>(vb, wsum);  // also orders the staging reads
@@ -902,11 +901,11 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
         zero_acc(cbox, S.cells);
         __syncthreads();
 #pragma unroll 1
-        for (int q = 0; q < VPT; ++q, V.rotate(), rot(res), rot(auxv)) {
+        for (int q = 0; q < VPT; ++q, V.rotate(), rot(auxv)) {
           if (!((rfin >> q) & 1u)) continue;
           LSample<DP> L;
           lsample<DP, false>(V.x[0], V.y[0], V.z[0], S, L);
-          const double ws = (double)(KIND == 3 ? auxv[0] : res[0]) * scale;
+          const double ws = (double)auxv[0] * scale;
 #pragma unroll
           for (int k = 0; k < DP; ++k)
 #pragma unroll
