@@ -617,7 +617,7 @@ template <int DP, int KIND>  // KIND: gather kinds 0-6, push 7, push_grad 8
 constexpr int tile_waves() {
   return DP == 2 ? (KIND == 3 ? 5 : KIND == 7 ? 6 : 0)
        : DP == 3 ? (KIND == 0 || KIND == 4 ? 6 : KIND == 1 || KIND == 5 || KIND == 7 ? 5 : KIND == 2 || KIND == 3 ? 4 : 0)
-       : DP == 4 ? (KIND == 2 ? 3 : KIND == 6 ? 4 : KIND == 7 ? 5 : 0)
+       : DP == 4 ? (KIND == 2 ? 3 : KIND == 6 ? 4 : KIND == 7 || KIND == 0 ? 5 : 0)
                  : 0;
 }
 
@@ -960,7 +960,8 @@ __global__ __launch_bounds__(kTB, (tile_waves<DP, KIND>())) void k_gather_tile(K
 // scatters and the objective accumulate fixed-point cells (32-bit, push_grad 64-bit).
 template <int DP, int KIND>
 constexpr int cap_cells() {
-  return (KIND == 0 || KIND == 1) ? (DP <= 2 ? 4096 : DP == 3 ? 6400 : 9472)
+  // (the cubic pull: 7680 floats, so that five workgroups of 91 registers share a CU)
+  return KIND == 0 && DP == 4 ? 7680 : (KIND == 0 || KIND == 1) ? (DP <= 2 ? 4096 : DP == 3 ? 6400 : 9472)
                                   : (DP <= 2 ? 3200 : DP == 3 ? 4800 : 6400);  // DP 5 as DP 4
 }
 
