@@ -301,7 +301,11 @@ enum {
                             pull_hess, push, push_grad, the pullbacks, the pull VJP, the fused
                             objective; isotropic degrees 1-4); 0: one work-item per sample */
   PP_OPT_TILE_SHAPE = 2, /* tile of the phi grid: 0 = 16x8x8 (default), 1 = 32x8x4, 2 = 16x16x4 */
-  PP_OPT_USE_TUNED = 3   /* 1 (default): prefer the autotuner's choice when it has one */
+  PP_OPT_USE_TUNED = 3,  /* 1 (default): prefer the autotuner's choice when it has one */
+  PP_OPT_BOX_CAP = 4     /* LDS box budget of the tiled kernels in floats (256..16384); 0
+                            (default): a per-op, per-degree default. A tile whose box exceeds
+                            the budget runs per sample through global memory; a larger budget
+                            leaves fewer workgroups per CU */
 };
 int pp_set_option(int key, int value);
 int pp_get_option(int key);

@@ -46,6 +46,7 @@ int hip_fail(hipError_t e, const char* where) {
 constexpr long long kMaxVox = (1LL << 31) - 1;  // per-volume voxel count (32-bit in-volume offsets)
 constexpr long long kMaxDim = 1LL << 29;        // per-axis size (2*d must fit an int)
 constexpr int kBlock = 256;
+constexpr int kMaxCap = 16384;  // LDS box budget limit in floats (64 KiB of dynamic LDS)
 
 // ---- kernel selection --------------------------------------------------------------
 // Variants: 0 = one work-item per sample; 1 + k = tiled with tile shape k.
@@ -53,6 +54,7 @@ constexpr int kVariants = 4;
 std::atomic<int> g_tiling{1};  // PP_OPT_TILING: 0 per sample, 1 automatic, 2 tiled
 std::atomic<int> g_shape{0};   // PP_OPT_TILE_SHAPE (default when nothing is tuned)
 std::atomic<int> g_tuned{1};   // PP_OPT_USE_TUNED
+std::atomic<int> g_cap{0};     // PP_OPT_BOX_CAP: LDS box budget in floats (0: per-op default)
 thread_local int t_force = -1; // variant forced while pp_tune times candidates
 
 // Autotuner table: (op, dp, log2 of the phi-grid size) -> variant.
@@ -67,7 +69,7 @@ int size_class(long long n) {
 
 hipError_t launch_variant(pp::Op op, const pp::KArgs& a, int nbatch, hipStream_t s, int variant) {
   if (variant > 0) {
-    hipError_t e = pp::launch_tiled(op, a, nbatch, s, pp::TileCfg{variant - 1});
+    hipError_t e = pp::launch_tiled(op, a, nbatch, s, pp::TileCfg{variant - 1, g_cap.load(std::memory_order_relaxed)});
     if (e != hipErrorNotSupported) return e;
   }
   return pp::launch_op(op, a, nbatch, s, kBlock);
@@ -969,6 +971,11 @@ int pp_set_option(int key, int value) {
     case PP_OPT_USE_TUNED:
       g_tuned.store(value != 0);
       return ok();
+    case PP_OPT_BOX_CAP:
+      if (value != 0 && (value < 256 || value > kMaxCap))
+        return fail(PP_EINVAL, "box budget %d floats not 0 or in 256..%d", value, kMaxCap);
+      g_cap.store(value);
+      return ok();
     default:
       return fail(PP_EINVAL, "unknown option %d", key);
   }
@@ -979,6 +986,7 @@ int pp_get_option(int key) {
     case PP_OPT_TILING: return g_tiling.load();
     case PP_OPT_TILE_SHAPE: return g_shape.load();
     case PP_OPT_USE_TUNED: return g_tuned.load();
+    case PP_OPT_BOX_CAP: return g_cap.load();
     default: return fail(PP_EINVAL, "unknown option %d", key);
   }
 }

@@ -102,7 +102,10 @@ hipError_t launch_op(Op op, const KArgs& a, int nbatch, hipStream_t stream, int 
 // degree they do not cover (the caller then uses launch_op).
 struct TileCfg {
   int shape;  // index into the compiled tile shapes (0: 16x8x8, 1: 32x8x4, 2: 16x16x4)
+  int cap;    // LDS box budget in floats (0: the default for the op and degree)
 };
+// Default LDS box budget (floats) of the tiled kernel of op at isotropic dp.
+int default_cap(Op op, int dp);
 hipError_t launch_tiled(Op op, const KArgs& a, int nbatch, hipStream_t stream, TileCfg cfg);
 
 // One volume of a batched launch (device-resident array, built by the batch plan).
@@ -123,7 +126,7 @@ struct XcdSplit {
 hipError_t launch_batch_zero(Op op, const BatchVol* dvols, int nvol, hipStream_t stream);
 // One degree group: dvols[0..nvol) all have this dp; tile0 counts from the group start.
 hipError_t launch_batch_group(Op op, int dp, const BatchVol* dvols, int nvol, const XcdSplit& split,
-                              hipStream_t stream);
+                              hipStream_t stream, int cap_override = 0);
 
 // Fill n floats with v (graph-capturable; no hipMemsetAsync nodes).
 hipError_t launch_fill(float* p, long long n, float v, hipStream_t stream);

@@ -25,22 +25,39 @@ namespace pp {
 
 constexpr int kTB = 256;
 
+// A tile kernel's LDS box is the launch's dynamic LDS: `cap` floats (8-byte aligned) that
+// hold the staged image or the fixed-point accumulator, and at their end the row table.
+extern __shared__ __align__(16) float pp_box[];
+
 // Largest box extent along y or z: 2 * (window half-width) + dp for every tile shape.
 constexpr int kMaxYZ = 64;
 
 struct TileState {
   int lo[3];      // unfolded image index of box cell (0,0,0), per axis
   int b[3];       // box extent per axis
-  int cells;      // b[0]*b[1]*b[2]
+  int Pf, Qf;     // LDS pitches of a box row and plane: staged image (floats)
+  int Pc, Qc;     // ... and fixed-point accumulator (64-bit cells); see tile_setup
+  int cells_f;    // floats of the staged image: Qf * b[2]
+  int cells_c;    // 64-bit cells of the accumulator: Qc * b[2]
   int lds;        // 1 when the box is used
   int ctr[3];     // window centre (knot of the tile's centre sample)
   int org[3];     // tile origin in the phi grid
   int red[6][4];  // per-wave partial min/max
   float redf[4];  // per-wave partial bound (tile_setup's `bound` hook)
-  // folded image offset of box plane y = i (d0x * fold(lo[1] + i)) and z = i
-  // (d0x * d0y * fold(lo[2] + i)); bit 31 set where the boundary flips the sign
-  int yoff[kMaxYZ], zoff[kMaxYZ];
+  int rows;       // box rows: b[1] * b[2]
+  int tab;        // float offset, in the launch's box memory, of the row table (RowEnt[rows])
 };
+
+// One box row (plane indices ry, rz; row r = ry + b[1] rz): the LDS index of its cell 0
+// (ry P + rz Q, in the layout's units) with the boundary's sign flips of the row's y and
+// z planes in bits 31 and 30, and the folded image offset of its column 0 before the x
+// fold (d0x fold(lo[1] + ry) + d0x d0y fold(lo[2] + rz)). Built once per tile, it makes
+// staging and flushing a cell a table read and a few instructions.
+struct RowEnt {
+  int cell;
+  int goff;
+};
+constexpr int kFlipY = INT_MIN, kFlipZ = 1 << 30, kCellMask = (1 << 30) - 1;
 
 template <int TX, int TY, int TZ>
 struct Shape {
@@ -52,9 +69,21 @@ struct Shape {
   // a box row (<= 2 * WX + dp cells) fits in one workgroup pass (box_walk relies on it)
   static_assert(2 * WX + 5 < kTB, "box rows must be shorter than the workgroup");
   static_assert(2 * WY + 5 <= kMaxYZ && 2 * WZ + 5 <= kMaxYZ, "box planes fit the fold tables");
-  // local index of the centre sample
-  static constexpr int LC = TX / 2 + TX * (TY / 2 + TY * (TZ / 2));
+  // local index of the centre sample (tile_pos order)
+  static constexpr int LC = TX / 2 + TX * (TZ / 2 + TZ * (TY / 2));
 };
+
+// Position in the tile of local sample l = tid + kTB * q (q: the thread's sample slot):
+// x fastest, then z, then y. A wave's 64 lanes then hold TX consecutive x positions on
+// 64 / TX consecutive z planes of one y row, and a thread's slots step along y. With a
+// box plane pitch Q = TX (mod 2 TX) the lanes of one LDS read fall on distinct banks (the
+// 64 banks of gfx950) for samples whose knots follow their positions.
+template <int TX, int TY, int TZ>
+__device__ __forceinline__ void tile_pos(int l, int& x, int& y, int& z) {
+  x = l % TX;
+  z = (l / TX) % TZ;
+  y = l / (TX * TZ);
+}
 
 // Cyclic rotation by one element. Loops over a thread's samples that are not unrolled
 // read element 0 and rotate once per iteration (VPT rotations restore the order), so the
@@ -114,13 +143,12 @@ struct NoBound {
 // Hooks, so that callers share its two barriers instead of adding their own:
 //   pre(V)    runs once the samples' grid indices are known, before the deformation is
 //             read: loads issued there overlap the deformation's;
-//   bound()   a per-thread value >= 0 whose workgroup maximum (NARROW: sum) lands in
-//             *block_bound;
-//   zero_box  when given, the box's cells (NARROW: 32-bit, else 64-bit) are zeroed
-//             before the final barrier.
-template <int DP, int TX, int TY, int TZ, int CAP, bool NARROW = false, class Pre = NoPrefetch,
-          class Bound = NoBound>
-__device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restrict__ phi, int tile,
+//   bound()   a per-thread value >= 0 whose workgroup maximum lands in *block_bound;
+//   zero_box  when given, the box's 64-bit cells are zeroed before the final barrier.
+// The box is used when the layouts the op needs (GF: the staged image, GC: the
+// accumulator) fit the cap floats of LDS the launch provides.
+template <int DP, int TX, int TY, int TZ, bool GF, bool GC, class Pre = NoPrefetch, class Bound = NoBound>
+__device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restrict__ phi, int tile, int cap,
                                            Vox<Shape<TX, TY, TZ>::VPT>& V, TileState& S, Pre pre = Pre(),
                                            Bound bound = Bound(), acc_t* zero_box = nullptr,
                                            float* block_bound = nullptr) {
@@ -136,7 +164,11 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
 #pragma unroll
   for (int q = 0; q < VPT; ++q) {
     const int l = tid + kTB * q;
-    const int gx = x0 + l % TX, gy = y0 + (l / TX) % TY, gz = z0 + l / (TX * TY);
+    int gx, gy, gz;
+    tile_pos<TX, TY, TZ>(l, gx, gy, gz);
+    gx += x0;
+    gy += y0;
+    gz += z0;
     const bool in = gx < a.d1x && gy < a.d1y && gz < a.d1z;
     V.v[q] = in ? gx + a.d1x * (gy + a.d1y * gz) : 0;
     if (in) V.inside |= 1u << q;
@@ -145,7 +177,11 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
 #pragma unroll
   for (int q = 0; q < VPT; ++q) {
     const int l = tid + kTB * q;
-    const int gx = x0 + l % TX, gy = y0 + (l / TX) % TY, gz = z0 + l / (TX * TY);
+    int gx, gy, gz;
+    tile_pos<TX, TY, TZ>(l, gx, gy, gz);
+    gx += x0;
+    gy += y0;
+    gz += z0;
     float x = 0.f, y = 0.f, z = 0.f;
     bool ok = false;
     if ((V.inside >> q) & 1u) {
@@ -188,7 +224,7 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
   mx0 = wave_max(mx0); mx1 = wave_max(mx1); mx2 = wave_max(mx2);
   float bnd = bound();
   if (block_bound)
-    bnd = NARROW ? __ockl_wfred_add_f32(bnd) : __ockl_wfred_max_f32(bnd);
+    bnd = __ockl_wfred_max_f32(bnd);
   const int wid = tid >> 6;
   if ((tid & 63) == 0) {
     S.red[0][wid] = mn0; S.red[1][wid] = mn1; S.red[2][wid] = mn2;
@@ -199,7 +235,7 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
   if (block_bound) {
     float m = S.redf[0];
 #pragma unroll
-    for (int w = 1; w < kTB / 64; ++w) m = NARROW ? m + S.redf[w] : fmaxf(m, S.redf[w]);
+    for (int w = 1; w < kTB / 64; ++w) m = fmaxf(m, S.redf[w]);
     *block_bound = m;
   }
   // every thread sizes the box (a broadcast read of the partials)
@@ -211,122 +247,159 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
       lo[q] = min(lo[q], S.red[q][w]);
       hi[q] = max(hi[q], S.red[3 + q][w]);
     }
-  const bool any = lo[0] <= hi[0];  // some in-window sample
-  const int by = hi[1] - lo[1] + DP, bz = hi[2] - lo[2] + DP;
-  const int cells = any ? (hi[0] - lo[0] + DP) * by * bz : 0;
-  const bool use = any && cells <= CAP;
+  bool any = lo[0] <= hi[0];  // some in-window sample
+  // LDS layouts: cell (i, j, k) at i + P j + Q k with P = bx. Plane pitches are padded to
+  // Q = TX (mod 2 TX), so that the TX x-lanes on 64 / TX planes of a wave (tile_pos) fall
+  // on distinct banks.
+  auto pitch = [](int P, int by) { return P * by + ((TX - P * by) & (2 * TX - 1)); };
+  auto fits = [&](int bx, int by, int bz) {  // box cells and the row table (two floats a row)
+    const int q = pitch(bx, by) * bz, t = 2 * by * bz;
+    return (!GF || q + t <= cap) && (!GC || 2 * q + t <= cap);
+  };
+  // A box beyond the budget is clipped, one knot plane at a time on the side farther from
+  // the window centre along its most spread axis, until it fits; the samples whose
+  // footprint then leaves the box take the global path. (Only a box that cannot shrink
+  // to the budget sends the whole tile there.)
+  bool clipped = false;
+  if (any && !fits(hi[0] - lo[0] + DP, hi[1] - lo[1] + DP, hi[2] - lo[2] + DP)) {
+    const int c0[3] = {S.ctr[0], S.ctr[1], S.ctr[2]};
+    const int span[3] = {TX, TY, TZ};
+    while (!fits(hi[0] - lo[0] + DP, hi[1] - lo[1] + DP, hi[2] - lo[2] + DP)) {
+      int ax = -1;
+      float worst = 0.f;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const float r = (float)(hi[q] - lo[q] + 1) / (float)span[q];
+        if (hi[q] > lo[q] && r > worst) {
+          worst = r;
+          ax = q;
+        }
+      }
+      if (ax < 0) break;
+      if (c0[ax] - lo[ax] > hi[ax] - c0[ax]) ++lo[ax];
+      else --hi[ax];
+      clipped = true;
+    }
+    any = fits(hi[0] - lo[0] + DP, hi[1] - lo[1] + DP, hi[2] - lo[2] + DP);
+  }
+  if (clipped) {  // (a uniform branch: every thread sized the same box)
+#pragma unroll
+    for (int q = 0; q < VPT; ++q) {
+      if (!((V.inl >> q) & 1u)) continue;
+      const int ox = knot_fast<DP>(V.x[q]), oy = knot_fast<DP>(V.y[q]), oz = knot_fast<DP>(V.z[q]);
+      if (ox < lo[0] || ox > hi[0] || oy < lo[1] || oy > hi[1] || oz < lo[2] || oz > hi[2]) V.inl &= ~(1u << q);
+    }
+  }
+  const int bx = hi[0] - lo[0] + DP, by = hi[1] - lo[1] + DP, bz = hi[2] - lo[2] + DP;
+  const int Pf = bx, Pc = bx, Qf = pitch(bx, by), Qc = Qf;
+  const int cells_f = any ? Qf * bz : 0, cells_c = any ? Qc * bz : 0;
+  const bool use = any;
+  const int rows = by * bz, tab = (cap & ~1) - 2 * rows;
   if (tid == 0) {
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       S.lo[q] = lo[q];
       S.b[q] = hi[q] - lo[q] + DP;
     }
-    S.cells = cells;
+    S.Pf = Pf;
+    S.Qf = Qf;
+    S.Pc = Pc;
+    S.Qc = Qc;
+    S.cells_f = cells_f;
+    S.cells_c = cells_c;
     S.lds = use;
+    S.rows = rows;
+    S.tab = tab;
   }
-  if (use && zero_box)  // (32-bit cells: as many 64-bit words as cover them)
-    for (int i = tid; i < (NARROW ? (cells + 1) >> 1 : cells); i += kTB) zero_box[i] = 0ull;
-  if (use) {  // fold tables of the box planes
-    bool neg;
-    if (tid < by) {
-      const int f = fold32(a.by, lo[1] + tid, a.d0y, neg);
-      S.yoff[tid] = (a.d0x * f) | (neg ? INT_MIN : 0);
-    } else if (tid >= 128 && tid - 128 < bz) {
-      const int f = fold32(a.bz, lo[2] + tid - 128, a.d0z, neg);
-      S.zoff[tid - 128] = (a.d0x * a.d0y * f) | (neg ? INT_MIN : 0);
+  if (use && zero_box)
+    for (int i = tid; i < cells_c; i += kTB) zero_box[i] = 0ull;
+  if (use) {  // the row table, at the end of the box memory
+    RowEnt* rt = reinterpret_cast<RowEnt*>(pp_box + tab);
+    const float inv_by = 1.0f / (float)by;
+    for (int r = tid; r < rows; r += kTB) {
+      const int rz = (int)(((float)r + 0.5f) * inv_by), ry = r - rz * by;  // (exact: r < 2^14)
+      bool ny, nz;
+      const int fy = fold32(a.by, lo[1] + ry, a.d0y, ny), fz = fold32(a.bz, lo[2] + rz, a.d0z, nz);
+      RowEnt e;
+      e.cell = (ry * Pf + rz * Qf) | (ny ? kFlipY : 0) | (nz ? kFlipZ : 0);
+      e.goff = a.d0x * (fy + a.d0y * fz);
+      rt[r] = e;
     }
   }
   __syncthreads();
 }
 
-// The box is walked by the whole workgroup: thread t owns column t % BX of rows
-// t / BX, t / BX + RP, ... (RP = kTB / BX rows per pass; BX <= 2 * WX + DP < kTB; rows
-// run over y, then z). Row coordinates advance incrementally and the folds of y and z
-// come from the tile's tables, so a cell costs a few instructions. kBatch rows' loads
-// are issued before any is used, so a wave has that many in flight.
-constexpr int kBatch = 16;
+// The box is walked by the whole workgroup: thread t owns column c = t % b[0] of rows
+// t / b[0], t / b[0] + RP, ... (RP = kTB / b[0] rows per pass; b[0] <= 2 WX + DP < kTB).
+// A cell's LDS index and image offset come from its row's table entry and the column's
+// x fold; kBatch cells' reads are issued before any is used.
+constexpr int kBatch = 8;
 
-struct BoxWalk {
-  int c, rp;       // column, rows per pass
-  int ry, rz;      // plane indices of the current row
-  int dy, dz;      // plane step per pass: RP = dz * BY + dy
-  int r, rows;     // current row, rows in the box
-  int fx;          // folded x offset of the column
-  bool nx;         // sign flip of the column
-  bool active;     // r0 < RP: this thread owns a column
+struct RowWalk {
+  int c, r, rp;  // column, first row, rows per pass
+  int fx;        // folded x offset of the column
+  int xflip;     // kFlipY when the column's x fold flips the sign (merged into the row parity)
+  bool active;   // this thread owns a column
 };
 
-__device__ __forceinline__ BoxWalk box_walk(const TileState& S, const KArgs& a) {
-  const int BX = S.b[0], BY = S.b[1];
-  BoxWalk w;
+__device__ __forceinline__ RowWalk row_walk(const TileState& S, const KArgs& a) {
+  const int BX = S.b[0];
+  RowWalk w;
   w.c = threadIdx.x % BX;
-  const int r0 = threadIdx.x / BX;
+  w.r = threadIdx.x / BX;
   w.rp = kTB / BX;
-  w.active = r0 < w.rp;
-  w.r = r0;
-  w.rows = BY * S.b[2];
-  w.ry = r0 % BY;
-  w.rz = r0 / BY;
-  w.dy = w.rp % BY;
-  w.dz = w.rp / BY;
-  w.fx = fold32(a.bx, S.lo[0] + w.c, a.d0x, w.nx);
+  w.active = w.r < w.rp;
+  bool nx;
+  w.fx = fold32(a.bx, S.lo[0] + w.c, a.d0x, nx);
+  w.xflip = nx ? kFlipY : 0;
   return w;
 }
 
-__device__ __forceinline__ void walk_next(BoxWalk& w, int BY) {
-  w.r += w.rp;
-  w.ry += w.dy;
-  w.rz += w.dz;
-  if (w.ry >= BY) {
-    w.ry -= BY;
-    ++w.rz;
-  }
-}
-
-// Folded image offset of the walk's current cell and whether its sign flips; `signs`
-// (bit a) selects the axes whose boundary sign applies to the channel.
-__device__ __forceinline__ int walk_offset(const TileState& S, const BoxWalk& w, unsigned signs, bool& neg) {
-  const int yo = S.yoff[w.ry], zo = S.zoff[w.rz];
-  neg = ((yo < 0) && (signs & 2u)) != ((zo < 0) && (signs & 4u));
-  neg = neg != (w.nx && (signs & 1u));
-  return w.fx + (yo & INT_MAX) + (zo & INT_MAX);
+// Sign of a cell: the parity of the flips of its row's y and z planes and of its column,
+// each counted only for the axes (bits of `signs`) whose boundary sign applies.
+__device__ __forceinline__ bool cell_neg(int cellword, const RowWalk& w, unsigned signs) {
+  const bool fy = (signs & 2u) && (cellword & kFlipY);
+  const bool fz = (signs & 4u) && (cellword & kFlipZ);
+  const bool fx = (signs & 1u) && w.xflip;
+  return fy != fz != fx;
 }
 
 // Stage the box from image channel `src` (fold + boundary sign applied here).
 __device__ __forceinline__ void stage_box(float* box, const TileState& S, const float* __restrict__ src,
                                           const KArgs& a, unsigned signs) {
-  BoxWalk w = box_walk(S, a);
+  RowWalk w = row_walk(S, a);
   if (!w.active) return;
-  const int BX = S.b[0], BY = S.b[1];
-  while (w.r < w.rows) {
-    const int r0 = w.r;
+  const RowEnt* rt = reinterpret_cast<const RowEnt*>(pp_box + S.tab);
+  const int rows = S.rows;
+  for (; w.r < rows; w.r += kBatch * w.rp) {
+    RowEnt e[kBatch];
     float v[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
-      v[u] = 0.f;
-      if (w.r < w.rows) {
-        bool neg;
-        const float t = src[walk_offset(S, w, signs, neg)];
-        v[u] = neg ? -t : t;
-      }
-      walk_next(w, BY);
+      const int r = w.r + u * w.rp;
+      e[u] = r < rows ? rt[r] : RowEnt{0, 0};
     }
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) {
-      const int r = r0 + u * w.rp;
-      if (r < w.rows) box[r * BX + w.c] = v[u];
-    }
+    for (int u = 0; u < kBatch; ++u) v[u] = w.r + u * w.rp < rows ? src[e[u].goff + w.fx] : 0.f;
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u)
+      if (w.r + u * w.rp < rows) box[(e[u].cell & kCellMask) + w.c] = cell_neg(e[u].cell, w, signs) ? -v[u] : v[u];
   }
 }
 
 // ---- fixed-point scatter accumulation ---------------------------------------------
 // gfx950 executes float atomics (LDS ds_add_f32 and global alike) at a small fraction
-// of the rate of integer LDS atomics, so scatters accumulate into a fixed-point LDS
-// box with integer adds: every contribution c is stored as round(c * 2^scale). In
-// 64-bit cells (ds_add_u64), per channel and tile, scale = 50 - e where 2^e bounds the
-// largest contribution: every scaled contribution is below 2^50 and no cell can
-// overflow (at most 2^10 samples per tile touch a cell). Integer sums are exact and
-// order-independent, and each cell is rounded to float once, when it is flushed.
+// of the rate of integer LDS atomics (ds_add_f32: ~190 cycles per wave instruction against
+// ~6 for ds_add_u64 on distinct cells), so scatters accumulate into a fixed-point LDS box
+// of 64-bit cells with integer adds: every contribution c is stored as round(c * 2^s).
+// Per channel and tile, s = 50 - e where 2^e bounds the largest |value|: every scaled
+// contribution is below 2^50 (tap weights are at most 1) and no cell can overflow (at
+// most 2^10 samples per tile touch a cell). Integer sums are exact and order-independent;
+// each cell is rounded to float once, when it is flushed. The quantum 2^(e-50) is below
+// 2^-49 of the tile's largest value, so a cell's error stays far below the float
+// rounding of the partial sums that the flush adds into the image -- relative to the
+// cell's own magnitude, as with the reference's float atomics, down to cells 2^-25 of
+// the tile's largest value.
 
 // A channel's conversion factor mul: contribution c -> round(c * mul); cell q -> q / mul.
 __device__ __forceinline__ double fixed_scale(float vmax) {
@@ -335,58 +408,10 @@ __device__ __forceinline__ double fixed_scale(float vmax) {
   return ldexp(1.0, 50 - e);                // a power of two: products stay exact
 }
 
-// Scatters of values (push, and the residual scatter of the objective and of the VJP)
-// use 32-bit cells and ds_add_u32, which retires ~1.3x as many lanes per clock as
-// ds_add_u64. Wrapping two's-complement sums are exact in any order, so only a cell's
-// final value has to fit: no cell exceeds B = wmax^3 * sum|v| over the tile's samples
-// (wmax the largest 1-D B-spline weight, every tap weight lies in [0, wmax^3]). With
-// mul = (2^31 - 2^10) / B a cell stays below 2^31 even with the rounding of its at most
-// 2^10 contributions added. mul is no power of two: the scaled products round once more
-// in double (2^-53 relative, far below the quantum). The quantum 1/mul is at most
-// 2^-31 * wmax^3 * 1024 times the tile's largest value (1.4e-7 of it for cubic splines)
-// and finer where values vary. push_grad keeps 64-bit cells.
-template <bool GRAD>
-constexpr bool narrow_cells() {
-  return !GRAD;
-}
-
-template <int DP>
-constexpr float wmax3() {  // (largest weight of the degree DP-1 B-spline)^3
-  return DP <= 2 ? 1.0f : DP == 3 ? 0.421875f : DP == 4 ? 0.2962963f : 0.2148771f;
-}
-
-template <int DP>
-__device__ __forceinline__ double fixed_scale32(float vsum) {
-  // (1.0001: the float sum and the float weights may fall short of the exact bound by
-  // a few parts in 10^5; a sum beyond the float range is below 2^10 * FLT_MAX)
-  const double s = vsum <= __FLT_MAX__ ? (double)vsum : 1024.0 * (double)__FLT_MAX__;
-  const double bound = s * (double)wmax3<DP>() * 1.0001;
-  return bound > 0.0 ? (2147483648.0 - 1024.0) / bound : 1.0;
-}
-
-template <bool NARROW>
-struct CellOf {
-  typedef acc_t T;
-};
-template <>
-struct CellOf<true> {
-  typedef unsigned T;
-};
-
-template <class T>
-struct Cell;
-template <>
-struct Cell<acc_t> {
-  static __device__ __forceinline__ long long q(acc_t v) { return (long long)v; }
-};
-template <>
-struct Cell<unsigned> {
-  static __device__ __forceinline__ long long q(unsigned v) { return (long long)(int)v; }
-};
-
 // round(a * b) as int64 for |a * b| < 2^51: the FMA with 2^52 + 2^51 rounds the product
 // to an integer held in the low mantissa bits (one FMA instead of a float -> int64
-// conversion sequence); fix2 adds two such products.
+// conversion sequence; the low word of the constant is zero, so removing it costs one
+// 32-bit subtraction); fix2 adds two such products.
 constexpr double kMagic = 6755399441055744.0;  // 2^52 + 2^51
 
 __device__ __forceinline__ acc_t fix1(double a, double b) {
@@ -411,35 +436,43 @@ __device__ __forceinline__ void zero_acc(T* box, int cells) {
   for (int i = threadIdx.x; i < cells; i += kTB) box[i] = 0;
 }
 
-// Largest value (SUM: the sum) over the workgroup (v >= 0); uses and releases wsum.
-template <bool SUM = false>
+// Largest value over the workgroup (v >= 0); uses and releases wsum.
 __device__ __forceinline__ float block_max(float v, float* wsum) {
-  v = SUM ? __ockl_wfred_add_f32(v) : __ockl_wfred_max_f32(v);
+  v = __ockl_wfred_max_f32(v);
   if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
   __syncthreads();
   float m = wsum[0];
-  for (int w = 1; w < kTB / 64; ++w) m = SUM ? m + wsum[w] : fmaxf(m, wsum[w]);
+  for (int w = 1; w < kTB / 64; ++w) m = fmaxf(m, wsum[w]);
   __syncthreads();
   return m;
 }
 
 // Flush the fixed-point box into image channel `dst`: one folded global float atomic per
 // non-zero cell.
-template <class T>
-__device__ __forceinline__ void flush_acc(const T* box, const TileState& S, double mul, float* __restrict__ dst,
+__device__ __forceinline__ void flush_acc(const acc_t* box, const TileState& S, double mul, float* __restrict__ dst,
                                           const KArgs& a, unsigned signs) {
-  BoxWalk w = box_walk(S, a);
+  RowWalk w = row_walk(S, a);
   if (!w.active) return;
-  const int BX = S.b[0], BY = S.b[1];
-  const double inv = 1.0 / mul;  // (exact for the powers of two of 64-bit cells)
-  for (; w.r < w.rows; walk_next(w, BY)) {
-    const long long q = Cell<T>::q(box[w.r * BX + w.c]);
-    if (q != 0) {
-      bool neg;
-      const int off = walk_offset(S, w, signs, neg);
-      const float v = fixed_to_float(q, inv);
-      unsafeAtomicAdd(dst + off, neg ? -v : v);
+  const RowEnt* rt = reinterpret_cast<const RowEnt*>(pp_box + S.tab);
+  const int rows = S.rows;
+  const double inv = 1.0 / mul;  // (exact: mul is a power of two)
+  constexpr int kFB = 4;  // cells read before the first is used (their LDS latencies overlap)
+  for (; w.r < rows; w.r += kFB * w.rp) {
+    RowEnt e[kFB];
+    long long q[kFB];
+#pragma unroll
+    for (int u = 0; u < kFB; ++u) {
+      const int r = w.r + u * w.rp;
+      e[u] = r < rows ? rt[r] : RowEnt{0, 0};
     }
+#pragma unroll
+    for (int u = 0; u < kFB; ++u) q[u] = w.r + u * w.rp < rows ? (long long)box[(e[u].cell & kCellMask) + w.c] : 0;
+#pragma unroll
+    for (int u = 0; u < kFB; ++u)
+      if (q[u] != 0) {
+        const float v = fixed_to_float(q[u], inv);
+        unsafeAtomicAdd(dst + e[u].goff + w.fx, cell_neg(e[u].cell, w, signs) ? -v : v);
+      }
   }
 }
 
@@ -451,13 +484,13 @@ struct LSample {
 };
 
 template <int DP, bool DERIV>
-__device__ __forceinline__ void lsample(float x, float y, float z, const TileState& S, LSample<DP>& L) {
+__device__ __forceinline__ void lsample(float x, float y, float z, const TileState& S, int P, int Q, LSample<DP>& L) {
   const int ox = knot_fast<DP>(x), oy = knot_fast<DP>(y), oz = knot_fast<DP>(z);
   float h[DP];
   weights_ct<DP>(x - (float)ox, L.wx, L.dx, h);
   weights_ct<DP>(y - (float)oy, L.wy, L.dy, h);
   weights_ct<DP>(z - (float)oz, L.wz, L.dz, h);
-  L.base = (ox - S.lo[0]) + S.b[0] * ((oy - S.lo[1]) + S.b[1] * (oz - S.lo[2]));
+  L.base = (ox - S.lo[0]) + P * (oy - S.lo[1]) + Q * (oz - S.lo[2]);
 }
 
 // ... with the second-derivative weights too (pull_hess and its pullback)
@@ -467,25 +500,87 @@ struct LSampleH : LSample<DP> {
 };
 
 template <int DP>
-__device__ __forceinline__ void lsample_h(float x, float y, float z, const TileState& S, LSampleH<DP>& L) {
+__device__ __forceinline__ void lsample_h(float x, float y, float z, const TileState& S, int P, int Q,
+                                          LSampleH<DP>& L) {
   const int ox = knot_fast<DP>(x), oy = knot_fast<DP>(y), oz = knot_fast<DP>(z);
   weights_ct<DP>(x - (float)ox, L.wx, L.dx, L.hx);
   weights_ct<DP>(y - (float)oy, L.wy, L.dy, L.hy);
   weights_ct<DP>(z - (float)oz, L.wz, L.dz, L.hz);
-  L.base = (ox - S.lo[0]) + S.b[0] * ((oy - S.lo[1]) + S.b[1] * (oz - S.lo[2]));
+  L.base = (ox - S.lo[0]) + P * (oy - S.lo[1]) + Q * (oz - S.lo[2]);
+}
+
+// ---- scatter slot order ------------------------------------------------------------
+// LDS atomics of one instruction that hit the same cell serialise: on gfx950 a single
+// such collision replays the instruction (two to three times the cost of one whose 64
+// lanes hit distinct cells; banks matter little for these atomics). Samples that
+// neighbour along x or z -- diagonals included -- often share a knot, and in tile_pos
+// order they sit in neighbouring lanes of one slot. The scatter passes therefore visit a
+// lane's slots rotated by the colour r = (x mod 2) + 2 (z mod 2) of its tile position: in
+// one pass, lanes whose samples neighbour along x or z (or both) hold samples on different
+// y rows, and lanes of one colour are two positions apart, where equal knots would need a
+// twofold compression of the deformation.
+template <int TX, int TY, int TZ>
+__device__ __forceinline__ int scatter_rot() {
+  constexpr int VPT = Shape<TX, TY, TZ>::VPT;
+  static_assert(VPT >= 4 && (VPT & (VPT - 1)) == 0, "slots per thread: a power of two >= 4");
+  const int t = threadIdx.x;
+  return ((t % TX) & 1) + 2 * (((t / TX) % TZ) & 1);
+}
+
+// Keeps the scheduler from interleaving the unrolled per-sample bodies (each body is
+// long enough to hide its own latencies among the resident waves; interleaving them
+// only multiplies the live registers).
+__device__ __forceinline__ void sample_fence() { __builtin_amdgcn_sched_barrier(0); }
+
+// a[q] <- a[(q + r) mod N] for a lane-dependent r (selects, no register indexing).
+template <int N, class T>
+__device__ __forceinline__ void rotate_by(T (&a)[N], int r) {
+#pragma unroll
+  for (int s = 1; s < N; s <<= 1) {
+    T b[N];
+#pragma unroll
+    for (int q = 0; q < N; ++q) b[q] = (r & s) ? a[(q + s) % N] : a[q];
+#pragma unroll
+    for (int q = 0; q < N; ++q) a[q] = b[q];
+  }
+}
+
+// Bit q of the result is bit (q + r) mod N of m.
+template <int N>
+__device__ __forceinline__ unsigned rotate_bits(unsigned m, int r) {
+  constexpr unsigned mask = (1u << N) - 1u;
+  m &= mask;
+  return ((m >> r) | (m << ((N - r) & (N - 1)))) & mask;
+}
+
+// Fixed-point contribution of tap weight w times the row factor c (scaled) to a 64-bit cell.
+__device__ __forceinline__ void lds_add_cell(acc_t* cell, float w, double c) { atomicAdd(cell, fix1(w, c)); }
+
+// Scatter one in-window sample's value (s0 = value * scale) into the fixed-point box.
+template <int DP>
+__device__ __forceinline__ void lds_scatter_val(acc_t* box, const LSample<DP>& L, int P, int Q, double s0) {
+#pragma unroll
+  for (int k = 0; k < DP; ++k) {
+    acc_t* plane = box + L.base + Q * k;
+    const double sk = (double)L.wz[k] * s0;
+#pragma unroll
+    for (int j = 0; j < DP; ++j) {
+      const double wv = (double)L.wy[j] * sk;
+#pragma unroll
+      for (int i = 0; i < DP; ++i) lds_add_cell(plane + P * j + i, L.wx[i], wv);
+    }
+  }
 }
 
 // ---- push / push_grad -------------------------------------------------------------
 // One tile of push (GRAD=false) or push_grad (GRAD=true) for batch index bb. The caller
-// provides the LDS box (CAP cells: 32-bit for push, 64-bit for push_grad, 8-byte aligned),
-// wsum (kTB/64 floats) and the tile state.
-template <int DP, int TX, int TY, int TZ, int CAP, bool GRAD>
-__device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, acc_t* box, float* wsum, TileState& S) {
+// provides the LDS box (cap floats, 8-byte aligned; used as 64-bit cells), wsum (kTB/64
+// floats) and the tile state.
+template <int DP, int TX, int TY, int TZ, bool GRAD>
+__device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, int cap, acc_t* box, float* wsum,
+                                          TileState& S) {
   constexpr int VPT = Shape<TX, TY, TZ>::VPT;
   constexpr int NG = GRAD ? 3 : 1;
-  constexpr bool NW = narrow_cells<GRAD>();
-  typedef typename CellOf<NW>::T C;
-  C* cbox = reinterpret_cast<C*>(box);
   const float* phi = a.phi + bb * a.phi_bs;
   const long long n1 = a.n1;
   // the first channel's values are loaded before the deformation is read
@@ -498,24 +593,24 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, acc_
 #pragma unroll
       for (int e = 0; e < NG; ++e) g[e][q] = (read0 && ((W.inside >> q) & 1u)) ? src0[W.v[q] + e * n1] : 0.f;
   };
-  // channel 0's scale bound: the largest (32-bit cells: the sum of the) finite per-sample
-  // bound over all samples in the grid (a superset of the in-window ones, so a valid
-  // bound), reduced with the box
+  // channel 0's scale bound: the largest finite per-sample bound over all samples in the
+  // grid (a superset of the in-window ones, so a valid bound), reduced with the box
   auto bound0 = [&]() {
     float m = 0.f;
 #pragma unroll
     for (int q = 0; q < VPT; ++q) {
       float b = fabsf(g[0][q]);
       if (GRAD) b += fabsf(g[NG > 1 ? 1 : 0][q]) + fabsf(g[NG > 2 ? 2 : 0][q]);
-      if (isfinite(b)) m = NW ? m + b : fmaxf(m, b);
+      if (isfinite(b)) m = fmaxf(m, b);
     }
     return m;
   };
   Vox<VPT> V;
   float bound_c0 = 0.f;
-  tile_setup<DP, TX, TY, TZ, CAP, NW>(a, phi, tile, V, S, prefetch, bound0, box, &bound_c0);
+  tile_setup<DP, TX, TY, TZ, false, true>(a, phi, tile, cap, V, S, prefetch, bound0, box, &bound_c0);
   const bool lds = S.lds != 0;
-  const int BX = S.b[0], BXY = S.b[0] * S.b[1];
+  const int P = S.Pc, Q = S.Qc;
+  const int rot = scatter_rot<TX, TY, TZ>();
   // push with a count image (out2): one more channel of ones after the nchan channels
   const int nsrc = a.nchan + ((!GRAD && a.out2) ? 1 : 0);
   for (int c = 0; c < nsrc; ++c) {
@@ -529,7 +624,9 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, acc_
       for (int q = 0; q < VPT; ++q) {
         const int l = threadIdx.x + kTB * q;
         const int ax = c + a.chan0;
-        const int id = ax == 0 ? S.org[0] + l % TX : ax == 1 ? S.org[1] + (l / TX) % TY : S.org[2] + l / (TX * TY);
+        int px, py, pz;
+        tile_pos<TX, TY, TZ>(l, px, py, pz);
+        const int id = ax == 0 ? S.org[0] + px : ax == 1 ? S.org[1] + py : S.org[2] + pz;
 #pragma unroll
         for (int e = 0; e < NG; ++e)
           g[e][q] = !((V.valid >> q) & 1u) ? 0.f
@@ -548,61 +645,81 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, acc_
         float b = fabsf(g[0][q]);
         if (GRAD) b += fabsf(g[NG > 1 ? 1 : 0][q]) + fabsf(g[NG > 2 ? 2 : 0][q]);
         if (!((use >> q) & 1u)) continue;
-        if (isfinite(b)) vmax = NW ? vmax + b : fmaxf(vmax, b);
+        if (isfinite(b)) vmax = fmaxf(vmax, b);
         else use &= ~(1u << q);
       }
       if (c == 0 && read0) {  // bound reduced and box zeroed within tile_setup's barriers
-        scale = NW ? fixed_scale32<DP>(bound_c0) : fixed_scale(bound_c0);
+        scale = fixed_scale(bound_c0);
       } else {
-        const float m = block_max<NW>(vmax, wsum);
-        scale = NW ? fixed_scale32<DP>(m) : fixed_scale(m);
-        zero_acc(cbox, S.cells);
+        const float m = block_max(vmax, wsum);
+        scale = fixed_scale(m);
+        zero_acc(box, S.cells_c);
         __syncthreads();
       }
     }
+    // samples outside the box (rare): through global memory, per sample
+    unsigned glob = V.valid & ~use;
+    if (!GRAD) {
+#pragma unroll
+      for (int q = 0; q < VPT; ++q)
+        if (!isfinite(g[0][q])) glob &= ~(1u << q);  // push skips non-finite values (pushpull.c:155)
+    }
+    if (__any(glob != 0u)) {  // one slot at a time, rotating the arrays (VPT rotations restore them)
 #pragma unroll 1
-    for (int q = 0; q < VPT; ++q, V.rotate(), rot_rows(g)) {
-      if (!((V.valid >> q) & 1u)) continue;
-      const float g0 = g[0][0];
-      const float g1 = GRAD ? g[NG > 1 ? 1 : 0][0] : 0.f;
-      const float g2 = GRAD ? g[NG > 2 ? 2 : 0][0] : 0.f;
-      if (!GRAD && !isfinite(g0)) continue;  // push skips non-finite values (pushpull.c:155)
-      const float x = V.x[0], y = V.y[0], z = V.z[0];
-      if ((use >> q) & 1u) {
-        LSample<DP> L;
-        lsample<DP, GRAD>(x, y, z, S, L);
-        const double s0 = (double)g0 * scale;
-        const double s1 = GRAD ? (double)g1 * scale : 0.0;
-        const double s2 = GRAD ? (double)g2 * scale : 0.0;
-#pragma unroll
-        for (int k = 0; k < DP; ++k)
-#pragma unroll
-          for (int j = 0; j < DP; ++j) {
-            C* row = cbox + L.base + BX * j + BXY * k;
-            if (GRAD) {
-              const double cx = (double)(L.wz[k] * L.wy[j]) * s0;
-              const double cyz = (double)(L.wz[k] * L.dy[j]) * s1 + (double)(L.dz[k] * L.wy[j]) * s2;
-#pragma unroll
-              for (int i = 0; i < DP; ++i) atomicAdd(row + i, (C)fix2(L.dx[i], cx, L.wx[i], cyz));
-            } else {
-              const double wv = (double)(L.wz[k] * L.wy[j]) * s0;
-#pragma unroll
-              for (int i = 0; i < DP; ++i) atomicAdd(row + i, (C)fix1(L.wx[i], wv));
-            }
-          }
-      } else {
+      for (int q = 0; q < VPT; ++q, V.rotate(), rot_rows(g)) {
+        if (!((glob >> q) & 1u)) continue;
         Axis<DP> ax, ay, az;
         constexpr int NEED = GRAD ? (kW | kDW) : kW;
-        make_axis<DP, NEED>(x, DP, a.d0x, 1, chan_code(a, c, 0, a.bx), ax);
-        make_axis<DP, NEED>(y, DP, a.d0y, a.d0x, chan_code(a, c, 1, a.by), ay);
-        make_axis<DP, NEED>(z, DP, a.d0z, a.d0x * a.d0y, chan_code(a, c, 2, a.bz), az);
-        if (GRAD) scatter_grad<DP>(dst, ax, ay, az, g0, g1, g2);
-        else scatter_val<DP>(dst, ax, ay, az, g0);
+        make_axis<DP, NEED>(V.x[0], DP, a.d0x, 1, chan_code(a, c, 0, a.bx), ax);
+        make_axis<DP, NEED>(V.y[0], DP, a.d0y, a.d0x, chan_code(a, c, 1, a.by), ay);
+        make_axis<DP, NEED>(V.z[0], DP, a.d0z, a.d0x * a.d0y, chan_code(a, c, 2, a.bz), az);
+        if (GRAD) scatter_grad<DP>(dst, ax, ay, az, g[0][0], g[NG > 1 ? 1 : 0][0], g[NG > 2 ? 2 : 0][0]);
+        else scatter_val<DP>(dst, ax, ay, az, g[0][0]);
       }
     }
     if (lds) {
+      // the box, one pass per slot in the lane's order (scatter_rot)
+      float sx[VPT], sy[VPT], sz[VPT], sg[NG][VPT];
+#pragma unroll
+      for (int q = 0; q < VPT; ++q) {
+        sx[q] = V.x[q];
+        sy[q] = V.y[q];
+        sz[q] = V.z[q];
+#pragma unroll
+        for (int e = 0; e < NG; ++e) sg[e][q] = g[e][q];
+      }
+      rotate_by(sx, rot);
+      rotate_by(sy, rot);
+      rotate_by(sz, rot);
+#pragma unroll
+      for (int e = 0; e < NG; ++e) rotate_by(sg[e], rot);
+      const unsigned suse = rotate_bits<VPT>(use, rot);
+#pragma unroll
+      for (int q = 0; q < VPT; ++q) {
+        if (!((suse >> q) & 1u)) continue;
+        LSample<DP> L;
+        lsample<DP, GRAD>(sx[q], sy[q], sz[q], S, P, Q, L);
+        if (GRAD) {
+          const double s0 = (double)sg[0][q] * scale;
+          const double s1 = (double)sg[NG > 1 ? 1 : 0][q] * scale;
+          const double s2 = (double)sg[NG > 2 ? 2 : 0][q] * scale;
+#pragma unroll
+          for (int k = 0; k < DP; ++k)
+#pragma unroll
+            for (int j = 0; j < DP; ++j) {
+              acc_t* row = box + L.base + P * j + Q * k;
+              const double cx = (double)(L.wz[k] * L.wy[j]) * s0;
+              const double cyz = (double)(L.wz[k] * L.dy[j]) * s1 + (double)(L.dz[k] * L.wy[j]) * s2;
+#pragma unroll
+              for (int i = 0; i < DP; ++i) atomicAdd(row + i, fix2(L.dx[i], cx, L.wx[i], cyz));
+            }
+        } else {
+          lds_scatter_val<DP>(box, L, P, Q, (double)sg[0][q] * scale);
+        }
+        sample_fence();
+      }
       __syncthreads();
-      flush_acc(cbox, S, scale, dst, a, chan_signs(a, c));
+      flush_acc(box, S, scale, dst, a, chan_signs(a, c));
       __syncthreads();
     }
   }
@@ -615,18 +732,18 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, acc_
 // compiler's choice.
 template <int DP, int KIND>  // KIND: gather kinds 0-6, push 7, push_grad 8
 constexpr int tile_waves() {
-  return DP == 2 ? (KIND == 3 ? 5 : KIND == 7 ? 6 : 0)
+  return DP == 2 ? (KIND == 7 ? 5 : 0)
        : DP == 3 ? (KIND == 0 || KIND == 4 ? 6 : KIND == 1 || KIND == 5 || KIND == 7 ? 5 : KIND == 2 || KIND == 3 ? 4 : 0)
-       : DP == 4 ? (KIND == 2 ? 3 : KIND == 6 ? 4 : KIND == 7 || KIND == 0 ? 5 : 0)
+       : DP == 4 ? (KIND == 2 || KIND == 3 || KIND == 6 ? 4 : KIND == 7 || KIND == 0 ? 5 : 0)
                  : 0;
 }
 
-template <int DP, int TX, int TY, int TZ, int CAP, bool GRAD>
-__global__ __launch_bounds__(kTB, (tile_waves<DP, GRAD ? 8 : 7>())) void k_push_tile(KArgs a) {
-  __shared__ acc_t box[narrow_cells<GRAD>() ? (CAP + 1) / 2 : CAP];
+template <int DP, int TX, int TY, int TZ, bool GRAD>
+__global__ __launch_bounds__(kTB, (tile_waves<DP, GRAD ? 8 : 7>())) void k_push_tile(KArgs a, int cap) {
   __shared__ TileState S;
   __shared__ float wsum[kTB / 64];
-  push_tile<DP, TX, TY, TZ, CAP, GRAD>(a, xcd_block(blockIdx.x, gridDim.x), blockIdx.y, box, wsum, S);
+  push_tile<DP, TX, TY, TZ, GRAD>(a, xcd_block(blockIdx.x, gridDim.x), blockIdx.y, cap,
+                                  reinterpret_cast<acc_t*>(pp_box), wsum, S);
 }
 
 // ---- gathers: pull (KIND 0), phi pullback of pull (1), fused objective (2), the ------
@@ -635,18 +752,33 @@ __global__ __launch_bounds__(kTB, (tile_waves<DP, GRAD ? 8 : 7>())) void k_push_
 // The contractions run one axis at a time -- x within each row, then y within each
 // plane, then z -- which takes fewer FMAs than weighting every row by its yz product
 // (a cubic gradient: 188 instead of 224 per sample).
+// The dp x dp taps of box plane k of a sample, loaded together: the reads of a plane are
+// all issued before the first is used (left to itself the scheduler issues one read and
+// waits for it before the next, which leaves every read's latency exposed).
 template <int DP>
-__device__ __forceinline__ float lds_val(const float* box, const LSample<DP>& L, int BX, int BXY) {
+__device__ __forceinline__ void lds_plane(const float* box, const LSample<DP>& L, int P, int Q, int k,
+                                          float (&v)[DP][DP]) {
+  const float* pl = box + L.base + Q * k;
+#pragma unroll
+  for (int j = 0; j < DP; ++j)
+#pragma unroll
+    for (int i = 0; i < DP; ++i) v[j][i] = pl[P * j + i];
+  sample_fence();
+}
+
+template <int DP>
+__device__ __forceinline__ float lds_val(const float* box, const LSample<DP>& L, int P, int Q) {
   float acc = 0.f;
 #pragma unroll
   for (int k = 0; k < DP; ++k) {
+    float v[DP][DP];
+    lds_plane<DP>(box, L, P, Q, k, v);
     float p = 0.f;
 #pragma unroll
     for (int j = 0; j < DP; ++j) {
-      const float* row = box + L.base + BX * j + BXY * k;
       float r = 0.f;
 #pragma unroll
-      for (int i = 0; i < DP; ++i) r += L.wx[i] * row[i];
+      for (int i = 0; i < DP; ++i) r += L.wx[i] * v[j][i];
       p += L.wy[j] * r;
     }
     acc += L.wz[k] * p;
@@ -655,21 +787,21 @@ __device__ __forceinline__ float lds_val(const float* box, const LSample<DP>& L,
 }
 
 template <int DP, bool VAL>
-__device__ __forceinline__ void lds_grad(const float* box, const LSample<DP>& L, int BX, int BXY, float& val,
+__device__ __forceinline__ void lds_grad(const float* box, const LSample<DP>& L, int P, int Q, float& val,
                                          float& gx, float& gy, float& gz) {
   val = gx = gy = gz = 0.f;
 #pragma unroll
   for (int k = 0; k < DP; ++k) {
+    float v[DP][DP];
+    lds_plane<DP>(box, L, P, Q, k, v);
     float pv = 0.f, px = 0.f, py = 0.f;  // plane k: sum_j wy r, sum_j wy rd, sum_j dy r
 #pragma unroll
     for (int j = 0; j < DP; ++j) {
-      const float* row = box + L.base + BX * j + BXY * k;
       float r = 0.f, rd = 0.f;
 #pragma unroll
       for (int i = 0; i < DP; ++i) {
-        const float v = row[i];
-        r += L.wx[i] * v;
-        rd += L.dx[i] * v;
+        r += L.wx[i] * v[j][i];
+        rd += L.dx[i] * v[j][i];
       }
       pv += L.wy[j] * r;
       px += L.wy[j] * rd;
@@ -684,22 +816,22 @@ __device__ __forceinline__ void lds_grad(const float* box, const LSample<DP>& L,
 
 // Hessian (xx, yy, zz, xy, xz, yz) from the box, one axis at a time as lds_grad.
 template <int DP>
-__device__ __forceinline__ void lds_hess(const float* box, const LSampleH<DP>& L, int BX, int BXY, float (&h)[6]) {
+__device__ __forceinline__ void lds_hess(const float* box, const LSampleH<DP>& L, int P, int Q, float (&h)[6]) {
 #pragma unroll
   for (int q = 0; q < 6; ++q) h[q] = 0.f;
 #pragma unroll
   for (int k = 0; k < DP; ++k) {
+    float v[DP][DP];
+    lds_plane<DP>(box, L, P, Q, k, v);
     float pw = 0.f, pwd = 0.f, pwh = 0.f, pd = 0.f, pdd = 0.f, ph = 0.f;
 #pragma unroll
     for (int j = 0; j < DP; ++j) {
-      const float* row = box + L.base + BX * j + BXY * k;
       float r = 0.f, rd = 0.f, rh = 0.f;
 #pragma unroll
       for (int i = 0; i < DP; ++i) {
-        const float v = row[i];
-        r += L.wx[i] * v;
-        rd += L.dx[i] * v;
-        rh += L.hx[i] * v;
+        r += L.wx[i] * v[j][i];
+        rd += L.dx[i] * v[j][i];
+        rh += L.hx[i] * v[j][i];
       }
       pw += L.wy[j] * r;
       pwd += L.wy[j] * rd;
@@ -736,22 +868,17 @@ __device__ __forceinline__ float outlier_grad(const KArgs& a, const float* __res
   return w;
 }
 
-// LDS box of a gather kind, in floats. The scattering kinds' box first holds the staged
-// image (float) and then, reused, the scatter accumulator (twice as large in 64-bit cells).
-template <int CAP, int KIND>
-constexpr int box_floats() {
-  return (KIND == 2 || KIND == 3) && !narrow_cells<false>() ? 2 * CAP : CAP;
-}
 
 // One tile of pull (KIND 0), the pull pullback (1), the fused objective (2), the VJP
 // of pull (3: dphi = sum_c delta_c grad f_c and df0 += push(delta) from one staging),
 // pull_grad (4: the three components of grad f_c per channel), pull_hess (5: the 3x3
 // Hessian per channel) or the pullback of pull_grad (6: dphi_b = sum_{a,c} delta_{a,c} H_{a,b,c}).
-// box: box_floats<CAP, KIND>() floats (8-byte aligned); wsum: kTB/64 floats.
+// box: cap floats (8-byte aligned; the scattering kinds reuse it for the accumulator);
+// wsum: kTB/64 floats.
 // The pullback of every sample is summed over channels in registers (gsum), and the
 // objective keeps each sample's residual in registers between its two passes.
-template <int DP, int TX, int TY, int TZ, int CAP, int KIND>
-__device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, float* box, float* wsum,
+template <int DP, int TX, int TY, int TZ, int KIND>
+__device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, int cap, float* box, float* wsum,
                                             TileState& S) {
   constexpr int VPT = Shape<TX, TY, TZ>::VPT;
   constexpr bool AUX = KIND >= 1 && KIND <= 3;  // reads an upstream gradient or target per sample
@@ -764,14 +891,20 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
       auxv[q] = (AUX && ((W.inside >> q) & 1u)) ? a.aux[bb * a.aux_bs + W.v[q]] : 0.f;
   };
   Vox<VPT> V;
-  tile_setup<DP, TX, TY, TZ, CAP>(a, phi, tile, V, S, prefetch);
+  tile_setup<DP, TX, TY, TZ, true, KIND == 2 || KIND == 3>(a, phi, tile, cap, V, S, prefetch);
   const bool lds = S.lds != 0;
-  const int BX = S.b[0], BXY = S.b[0] * S.b[1];
+  const int P = S.Pf, Q = S.Qf;
   const long long n1 = a.n1;
   float gsum[3][VPT];
 #pragma unroll
   for (int q = 0; q < VPT; ++q) gsum[0][q] = gsum[1][q] = gsum[2][q] = 0.f;
   float lsum = 0.f;
+  const float nan = __builtin_nanf("");
+  // samples read from the box, and valid samples taking the global path (outside the
+  // box's window, or the whole tile when its box exceeds the budget)
+  const unsigned lmask = lds ? (V.valid & V.inl) : 0u;
+  const unsigned omask = V.valid & ~lmask;
+  const bool any_outlier = __any(omask != 0u);
   for (int c = 0; c < a.nchan; ++c) {
     const float* src = a.f + bb * a.f_bs + c * a.f_cs;
     // KIND 2/3: in-window samples with a finite weight, scattered via LDS; their weight
@@ -785,26 +918,29 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
       stage_box(box, S, src, a, chan_signs(a, c));
       __syncthreads();
     }
-#pragma unroll 1
-    for (int q = 0; q < VPT; ++q, V.rotate(), rot(auxv), rot_rows(gsum)) {
-      if (!((V.inside >> q) & 1u)) continue;
-      const int v = V.v[0];
-      if (!((V.valid >> q) & 1u)) {
-        if (KIND == 0) a.out[bb * a.out_bs + c * a.out_cs + v] = __builtin_nanf("");
-        if (KIND == 4) put3(a.out + bb * a.out_bs + c * a.out_cs + v, n1, __builtin_nanf(""), __builtin_nanf(""),
-                            __builtin_nanf(""), 0);
+    // invalid samples: NaN outputs
+    if (KIND == 0 || KIND == 4 || KIND == 5) {
+#pragma unroll
+      for (int q = 0; q < VPT; ++q) {
+        if (!((V.inside >> q) & 1u) || ((V.valid >> q) & 1u)) continue;
+        float* o = a.out + bb * a.out_bs + c * a.out_cs + V.v[q];
+        if (KIND == 0) o[0] = nan;
+        if (KIND == 4) put3(o, n1, nan, nan, nan, 0);
         if (KIND == 5)
-          for (int e = 0; e < 9; ++e) a.out[bb * a.out_bs + c * a.out_cs + v + e * n1] = __builtin_nanf("");
-        continue;
+          for (int e = 0; e < 9; ++e) o[e * n1] = nan;
       }
-      const float x = V.x[0], y = V.y[0], z = V.z[0];
-      const bool use_lds = lds && ((V.inl >> q) & 1u);
+    }
+    // One valid sample: array slot qi (its slot q, or 0 in the rotating outlier loop),
+    // from the box (use_lds) or through global memory.
+    auto body = [&](const int qi, const int q, const bool use_lds) {
+      const int v = V.v[qi];
+      const float x = V.x[qi], y = V.y[qi], z = V.z[qi];
       if (KIND == 0) {
         float val;
         if (use_lds) {
           LSample<DP> L;
-          lsample<DP, false>(x, y, z, S, L);
-          val = lds_val<DP>(box, L, BX, BXY);
+          lsample<DP, false>(x, y, z, S, P, Q, L);
+          val = lds_val<DP>(box, L, P, Q);
         } else {
           Axis<DP> ax, ay, az;
           make_axis<DP, kW>(x, DP, a.d0x, 1, chan_code(a, c, 0, a.bx), ax);
@@ -813,18 +949,19 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
           val = sample_val<DP, true>(src, ax, ay, az);
         }
         if (a.vscale != 1.0f || a.add_id) {
-          const int l = threadIdx.x + kTB * q;
-          val = pull_out(a, val, c, S.org[0] + l % TX, S.org[1] + (l / TX) % TY, S.org[2] + l / (TX * TY));
+          int px, py, pz;
+          tile_pos<TX, TY, TZ>(threadIdx.x + kTB * q, px, py, pz);
+          val = pull_out(a, val, c, S.org[0] + px, S.org[1] + py, S.org[2] + pz);
         }
         a.out[bb * a.out_bs + c * a.out_cs + v] = val;
-        continue;
+        return;
       }
       if (KIND == 5 || KIND == 6) {
         float h[6];
         if (use_lds) {
           LSampleH<DP> L;
-          lsample_h<DP>(x, y, z, S, L);
-          lds_hess<DP>(box, L, BX, BXY, h);
+          lsample_h<DP>(x, y, z, S, P, Q, L);
+          lds_hess<DP>(box, L, P, Q, h);
         } else {
           Axis<DP> ax, ay, az;
           make_axis<DP, kW | kDW | kHW>(x, DP, a.d0x, 1, a.bx, ax);
@@ -841,19 +978,19 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
         } else {
           const float* dl = a.aux + bb * a.aux_bs + c * a.aux_cs + v;
           const float d0 = dl[0], d1 = dl[n1], d2 = dl[2 * n1];
-          gsum[0][0] += d0 * h[0] + d1 * h[3] + d2 * h[4];
-          gsum[1][0] += d0 * h[3] + d1 * h[1] + d2 * h[5];
-          gsum[2][0] += d0 * h[4] + d1 * h[5] + d2 * h[2];
+          gsum[0][qi] += d0 * h[0] + d1 * h[3] + d2 * h[4];
+          gsum[1][qi] += d0 * h[3] + d1 * h[1] + d2 * h[5];
+          gsum[2][qi] += d0 * h[4] + d1 * h[5] + d2 * h[2];
         }
-        continue;
+        return;
       }
       if (KIND == 4) {
         float gx, gy, gz;
         if (use_lds) {
           LSample<DP> L;
-          lsample<DP, true>(x, y, z, S, L);
+          lsample<DP, true>(x, y, z, S, P, Q, L);
           float val;
-          lds_grad<DP, false>(box, L, BX, BXY, val, gx, gy, gz);
+          lds_grad<DP, false>(box, L, P, Q, val, gx, gy, gz);
         } else {
           Axis<DP> ax, ay, az;
           make_axis<DP, kW | kDW>(x, DP, a.d0x, 1, a.bx, ax);
@@ -862,59 +999,71 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
           sample_grad<DP, true>(src, ax, ay, az, gx, gy, gz);
         }
         put3(a.out + bb * a.out_bs + c * a.out_cs + v, n1, gx, gy, gz, 0);
-        continue;
+        return;
       }
-      const float aux = auxv[0];
+      const float aux = auxv[qi];
       float val = 0.f, gx, gy, gz, w;
       if (use_lds) {
         LSample<DP> L;
-        lsample<DP, true>(x, y, z, S, L);
-        lds_grad<DP, KIND == 2>(box, L, BX, BXY, val, gx, gy, gz);
+        lsample<DP, true>(x, y, z, S, P, Q, L);
+        lds_grad<DP, KIND == 2>(box, L, P, Q, val, gx, gy, gz);
         w = (KIND == 2) ? val - aux : aux;
         if ((KIND == 2 || KIND == 3) && isfinite(w)) {
-          if (KIND == 2) auxv[0] = w;
+          if (KIND == 2) auxv[qi] = w;
           rfin |= 1u << q;
         }
       } else {
         w = outlier_grad<DP, KIND>(a, src, a.out2 + bb * a.out2_bs + c * a.out2_cs, x, y, z, aux, gx, gy, gz);
       }
       if (KIND == 2 && isfinite(w)) lsum += 0.5f * w * w;
-      gsum[0][0] += w * gx;
-      gsum[1][0] += w * gy;
-      gsum[2][0] += w * gz;
+      gsum[0][qi] += w * gx;
+      gsum[1][qi] += w * gy;
+      gsum[2][qi] += w * gz;
+    };
+#pragma unroll
+    for (int q = 0; q < VPT; ++q) {
+      if ((lmask >> q) & 1u) body(q, q, true);
+      sample_fence();
+    }
+    if (any_outlier) {  // rare: one slot at a time, rotating the arrays (VPT rotations restore them)
+#pragma unroll 1
+      for (int q = 0; q < VPT; ++q, V.rotate(), rot(auxv), rot_rows(gsum))
+        if ((omask >> q) & 1u) body(0, q, false);
     }
     if (lds) {
       __syncthreads();
       if (KIND == 2 || KIND == 3) {  // scatter the residuals (gradients) of the in-window samples via the fixed-point box
-        constexpr bool NW = narrow_cells<false>();
-        typedef typename CellOf<NW>::T C;
         float vb = 0.f;
 #pragma unroll
         for (int q = 0; q < VPT; ++q)
-          if ((rfin >> q) & 1u) {
-            const float r = fabsf(auxv[q]);
-            vb = NW ? vb + r : fmaxf(vb, r);
-          }
-        const float m = block_max<NW>(vb, wsum);  // also orders the staging reads
-        const double scale = NW ? fixed_scale32<DP>(m) : fixed_scale(m);
-        C* cbox = reinterpret_cast<C*>(box);
-        zero_acc(cbox, S.cells);
+          if ((rfin >> q) & 1u) vb = fmaxf(vb, fabsf(auxv[q]));
+        const float m = block_max(vb, wsum);  // also orders the staging reads
+        const double scale = fixed_scale(m);
+        acc_t* cbox = reinterpret_cast<acc_t*>(box);
+        zero_acc(cbox, S.cells_c);
         __syncthreads();
-#pragma unroll 1
-        for (int q = 0; q < VPT; ++q, V.rotate(), rot(auxv)) {
-          if (!((rfin >> q) & 1u)) continue;
+        // slots in the lane's rotated order (scatter_rot)
+        const int rot = scatter_rot<TX, TY, TZ>();
+        float sx[VPT], sy[VPT], sz[VPT], sw[VPT];
+#pragma unroll
+        for (int q = 0; q < VPT; ++q) {
+          sx[q] = V.x[q];
+          sy[q] = V.y[q];
+          sz[q] = V.z[q];
+          sw[q] = auxv[q];
+        }
+        rotate_by(sx, rot);
+        rotate_by(sy, rot);
+        rotate_by(sz, rot);
+        rotate_by(sw, rot);
+        const unsigned sfin = rotate_bits<VPT>(rfin, rot);
+#pragma unroll
+        for (int q = 0; q < VPT; ++q) {
+          if (!((sfin >> q) & 1u)) continue;
           LSample<DP> L;
-          lsample<DP, false>(V.x[0], V.y[0], V.z[0], S, L);
-          const double ws = (double)auxv[0] * scale;
-#pragma unroll
-          for (int k = 0; k < DP; ++k)
-#pragma unroll
-            for (int j = 0; j < DP; ++j) {
-              C* row = cbox + L.base + BX * j + BXY * k;
-              const double wv = (double)(L.wz[k] * L.wy[j]) * ws;
-#pragma unroll
-              for (int i = 0; i < DP; ++i) atomicAdd(row + i, (C)fix1(L.wx[i], wv));
-            }
+          lsample<DP, false>(sx[q], sy[q], sz[q], S, S.Pc, S.Qc, L);
+          lds_scatter_val<DP>(cbox, L, S.Pc, S.Qc, (double)sw[q] * scale);
+          sample_fence();
         }
         __syncthreads();
         flush_acc(cbox, S, scale, a.out2 + bb * a.out2_bs + c * a.out2_cs, a, chan_signs(a, c));
@@ -923,12 +1072,11 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
     }
   }
   if (RED) {
-    const float nan = __builtin_nanf("");
-#pragma unroll 1
-    for (int q = 0; q < VPT; ++q, V.rotate(), rot_rows(gsum)) {
+#pragma unroll
+    for (int q = 0; q < VPT; ++q) {
       if (!((V.inside >> q) & 1u)) continue;
-      const int v = V.v[0];
-      if ((V.valid >> q) & 1u) put3(a.out + bb * a.out_bs + v, n1, gsum[0][0], gsum[1][0], gsum[2][0], a.accum);
+      const int v = V.v[q];
+      if ((V.valid >> q) & 1u) put3(a.out + bb * a.out_bs + v, n1, gsum[0][q], gsum[1][q], gsum[2][q], a.accum);
       else put3(a.out + bb * a.out_bs + v, n1, nan, nan, nan, 0);
     }
   }
@@ -945,25 +1093,38 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, fl
   __syncthreads();  // the tile state and boxes may be reused by the next tile
 }
 
-template <int DP, int TX, int TY, int TZ, int CAP, int KIND>
-__global__ __launch_bounds__(kTB, (tile_waves<DP, KIND>())) void k_gather_tile(KArgs a) {
-  __shared__ __align__(8) float box[box_floats<CAP, KIND>()];
+template <int DP, int TX, int TY, int TZ, int KIND>
+__global__ __launch_bounds__(kTB, (tile_waves<DP, KIND>())) void k_gather_tile(KArgs a, int cap) {
   __shared__ TileState S;
   __shared__ float wsum[kTB / 64];
-  gather_tile<DP, TX, TY, TZ, CAP, KIND>(a, xcd_block(blockIdx.x, gridDim.x), blockIdx.y, box, wsum, S);
+  gather_tile<DP, TX, TY, TZ, KIND>(a, xcd_block(blockIdx.x, gridDim.x), blockIdx.y, cap, pp_box, wsum, S);
 }
 
 // ---- launcher ------------------------------------------------------------------
-// Box budgets in cells, per degree: small boxes let more workgroups share a CU, whose
-// latency chains (load phi, size the box, stage, compute) then overlap. A tile whose box
-// exceeds the budget runs per sample through global memory. Gathers stage floats;
-// scatters and the objective accumulate fixed-point cells (32-bit, push_grad 64-bit).
-template <int DP, int KIND>
-constexpr int cap_cells() {
-  // (the cubic pull: 7680 floats, so that five workgroups of 91 registers share a CU)
-  return KIND == 0 && DP == 4 ? 7680 : (KIND == 0 || KIND == 1) ? (DP <= 2 ? 4096 : DP == 3 ? 6400 : 9472)
-                                  : (DP <= 2 ? 3200 : DP == 3 ? 4800 : 6400);  // DP 5 as DP 4
+// Box budgets (floats of dynamic LDS per workgroup), per op and degree: small boxes let
+// more workgroups share a CU, whose latency chains (load phi, size the box, stage,
+// compute) then overlap; a box beyond the budget is clipped to it and the samples that
+// leave it run per sample through global memory. Each default is the largest budget that
+// keeps the workgroups per CU its registers allow (160 KiB of LDS per CU, 224 bytes of
+// static LDS per workgroup): gathers stage floats; scatters, the VJP and the objective
+// accumulate 64-bit cells (two floats each); the row table takes two floats a box row.
+// A caller may pass its own budget (the autotuner does).
+int default_cap(Op op, int dp) {
+  // (workgroups per CU by LDS: 6 up to 6770 floats, 5 up to 8136, 4 up to 10184, 3 up to 13596)
+  const int d = dp < 2 ? 2 : dp > 5 ? 5 : dp;
+  switch (op) {
+    case OP_PULL:
+    case OP_PULLG:
+    case OP_PULLH: return d == 2 ? 4608 : d == 3 ? 6770 : d == 4 ? 8136 : 10184;
+    case OP_PULL_BWD:
+    case OP_PULLG_BWD: return d == 2 ? 4608 : d == 3 ? 8136 : 10184;
+    case OP_PUSH:
+    case OP_PUSHG: return d <= 3 ? 8136 : d == 4 ? 10184 : 13596;  // 64-bit cells
+    default: return d <= 4 ? 10184 : 13596;                         // the VJP and the objective
+  }
 }
+
+static size_t cap_bytes(int cap) { return (size_t)cap * sizeof(float); }
 
 template <int TX, int TY, int TZ>
 static int tiles_of(const KArgs& a) {
@@ -971,36 +1132,19 @@ static int tiles_of(const KArgs& a) {
 }
 
 template <int DP, int TX, int TY, int TZ>
-static hipError_t launch_shape(Op op, const KArgs& a, int nbatch, hipStream_t s) {
+static hipError_t launch_shape(Op op, const KArgs& a, int nbatch, hipStream_t s, int cap) {
   const dim3 grid((unsigned)tiles_of<TX, TY, TZ>(a), (unsigned)nbatch, 1), block(kTB);
+  const size_t sh = cap_bytes(cap);
   switch (op) {
-    case OP_PULL:
-      hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, cap_cells<DP, 0>(), 0>), grid, block, 0, s, a);
-      break;
-    case OP_PULLG:
-      hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, cap_cells<DP, 0>(), 4>), grid, block, 0, s, a);
-      break;
-    case OP_PULLH:
-      hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, cap_cells<DP, 0>(), 5>), grid, block, 0, s, a);
-      break;
-    case OP_PULLG_BWD:
-      hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, cap_cells<DP, 1>(), 6>), grid, block, 0, s, a);
-      break;
-    case OP_PULL_BWD:
-      hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, cap_cells<DP, 1>(), 1>), grid, block, 0, s, a);
-      break;
-    case OP_OBJGRAD:
-      hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, cap_cells<DP, 2>(), 2>), grid, block, 0, s, a);
-      break;
-    case OP_PULL_VJP:
-      hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, cap_cells<DP, 2>(), 3>), grid, block, 0, s, a);
-      break;
-    case OP_PUSH:
-      hipLaunchKernelGGL((k_push_tile<DP, TX, TY, TZ, cap_cells<DP, 3>(), false>), grid, block, 0, s, a);
-      break;
-    case OP_PUSHG:
-      hipLaunchKernelGGL((k_push_tile<DP, TX, TY, TZ, cap_cells<DP, 4>(), true>), grid, block, 0, s, a);
-      break;
+    case OP_PULL: hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, 0>), grid, block, sh, s, a, cap); break;
+    case OP_PULLG: hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, 4>), grid, block, sh, s, a, cap); break;
+    case OP_PULLH: hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, 5>), grid, block, sh, s, a, cap); break;
+    case OP_PULLG_BWD: hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, 6>), grid, block, sh, s, a, cap); break;
+    case OP_PULL_BWD: hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, 1>), grid, block, sh, s, a, cap); break;
+    case OP_OBJGRAD: hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, 2>), grid, block, sh, s, a, cap); break;
+    case OP_PULL_VJP: hipLaunchKernelGGL((k_gather_tile<DP, TX, TY, TZ, 3>), grid, block, sh, s, a, cap); break;
+    case OP_PUSH: hipLaunchKernelGGL((k_push_tile<DP, TX, TY, TZ, false>), grid, block, sh, s, a, cap); break;
+    case OP_PUSHG: hipLaunchKernelGGL((k_push_tile<DP, TX, TY, TZ, true>), grid, block, sh, s, a, cap); break;
     default: return hipErrorNotSupported;
   }
   return hipGetLastError();
@@ -1008,10 +1152,13 @@ static hipError_t launch_shape(Op op, const KArgs& a, int nbatch, hipStream_t s)
 
 template <int DP>
 static hipError_t launch_deg(Op op, const KArgs& a, int nbatch, hipStream_t s, TileCfg cfg) {
+  const int cap = cfg.cap > 0 ? cfg.cap : default_cap(op, DP);
   switch (cfg.shape) {
-    case 1: return launch_shape<DP, 32, 8, 4>(op, a, nbatch, s);
-    case 2: return launch_shape<DP, 16, 16, 4>(op, a, nbatch, s);
-    default: return launch_shape<DP, 16, 8, 8>(op, a, nbatch, s);
+#ifndef PP_DEFAULT_SHAPE_ONLY
+    case 1: return launch_shape<DP, 32, 8, 4>(op, a, nbatch, s, cap);
+    case 2: return launch_shape<DP, 16, 16, 4>(op, a, nbatch, s, cap);
+#endif
+    default: return launch_shape<DP, 16, 8, 8>(op, a, nbatch, s, cap);
   }
 }
 
@@ -1032,12 +1179,9 @@ constexpr int batch_waves_kind(int kind) { return kind <= 2 ? kind : kind == 5 ?
 
 template <int DP, int KIND>
 __global__ __launch_bounds__(kTB, (tile_waves<DP, batch_waves_kind(KIND)>())) void k_batch(
-    const BatchVol* __restrict__ vols, int nvol, XcdSplit split) {
+    const BatchVol* __restrict__ vols, int nvol, XcdSplit split, int cap) {
   constexpr int KG = batch_gather_kind(KIND) >= 0 ? batch_gather_kind(KIND) : 0;
   constexpr bool GATHER = batch_gather_kind(KIND) >= 0;
-  constexpr int CAPE = (DP >= 2 && DP <= 5) ? cap_cells<DP, KIND>() : 0;
-  constexpr int BOXF = GATHER ? box_floats<CAPE, KG>() : narrow_cells<KIND == 4>() ? CAPE : 2 * CAPE;  // push cells
-  __shared__ __align__(8) float box[BOXF > 0 ? BOXF : 2];
   __shared__ TileState S;
   __shared__ float wsum[kTB / 64];
   // workgroup b runs on XCD b % 8 and takes the next tile of that XCD's range
@@ -1052,8 +1196,9 @@ __global__ __launch_bounds__(kTB, (tile_waves<DP, batch_waves_kind(KIND)>())) vo
   }
   const KArgs a = vols[lo].a;
   const int tile = t - vols[lo].tile0;
-  if (GATHER) gather_tile<DP, kBTX, kBTY, kBTZ, CAPE, KG>(a, tile, 0, box, wsum, S);
-  else push_tile<DP, kBTX, kBTY, kBTZ, CAPE, KIND == 4>(a, tile, 0, reinterpret_cast<acc_t*>(box), wsum, S);
+  const int c = DP >= 2 ? cap : 0;  // degree 0: no box
+  if (GATHER) gather_tile<DP, kBTX, kBTY, kBTZ, KG>(a, tile, 0, c, pp_box, wsum, S);
+  else push_tile<DP, kBTX, kBTY, kBTZ, KIND == 4>(a, tile, 0, c, reinterpret_cast<acc_t*>(pp_box), wsum, S);
 }
 
 int batch_tiles(const KArgs& a) { return tiles_of<kBTX, kBTY, kBTZ>(a); }
@@ -1092,7 +1237,7 @@ __global__ __launch_bounds__(kTB) void k_lds_add_rate(acc_t* __restrict__ out, i
 }
 
 hipError_t launch_lds_add_rate(unsigned long long* out, int blocks, int iters, hipStream_t s) {
-  hipLaunchKernelGGL((k_lds_add_rate<typename CellOf<narrow_cells<false>()>::T>), dim3((unsigned)blocks), dim3(kTB),
+  hipLaunchKernelGGL((k_lds_add_rate<acc_t>), dim3((unsigned)blocks), dim3(kTB),
                      0, s, out, iters);
   return hipGetLastError();
 }
@@ -1136,17 +1281,20 @@ hipError_t launch_batch_zero(Op op, const BatchVol* dvols, int nvol, hipStream_t
   return hipGetLastError();
 }
 
-hipError_t launch_batch_group(Op op, int dp, const BatchVol* dvols, int nvol, const XcdSplit& split, hipStream_t s) {
+hipError_t launch_batch_group(Op op, int dp, const BatchVol* dvols, int nvol, const XcdSplit& split, hipStream_t s,
+                              int cap_override) {
   int longest = 0;
   for (int x = 0; x < kBatchXcds; ++x) longest = std::max(longest, split.s[x + 1] - split.s[x]);
   if (nvol <= 0 || longest <= 0) return hipSuccess;
   const void* k = batch_kernel_for(op, dp);
   if (!k) return hipErrorNotSupported;
   XcdSplit sp = split;
-  void* args[] = {(void*)&dvols, (void*)&nvol, (void*)&sp};
+  int cap = cap_override > 0 ? cap_override : default_cap(op, dp);
+  if (dp < 2) cap = 2;  // degree 0 runs per sample (no box)
+  void* args[] = {(void*)&dvols, (void*)&nvol, (void*)&sp, (void*)&cap};
   const long long grid = (long long)longest * kBatchXcds;
   if (grid > INT_MAX) return hipErrorInvalidValue;
-  return hipLaunchKernel(k, dim3((unsigned)grid), dim3(kTB), args, 0, s);
+  return hipLaunchKernel(k, dim3((unsigned)grid), dim3(kTB), args, cap_bytes(cap), s);
 }
 
 hipError_t launch_tiled(Op op, const KArgs& a, int nbatch, hipStream_t s, TileCfg cfg) {

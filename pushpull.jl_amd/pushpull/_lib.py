@@ -18,6 +18,7 @@ PP_OK, PP_EINVAL, PP_EDEVICE, PP_ENOMEM, PP_ERANGE = 0, -1, -2, -3, -4
 PP_OPT_TILING = 1
 PP_OPT_TILE_SHAPE = 2
 PP_OPT_USE_TUNED = 3
+PP_OPT_BOX_CAP = 4
 
 
 class PPDesc(ct.Structure):
