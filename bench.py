@@ -212,22 +212,18 @@ def _time_ms_median(torch, fn, reps):
     return float(np.median([a.elapsed_time(b) for a, b in ev]))
 
 
-def bench_batched(torch, pp, reps=5, seed=4):
-    """24 volumes of uneven size (X 96-224, Y 96-256, Z 96-192), 1-4 channels, degree 1
-    or 3, each with its own smooth deformation (max 5 voxels): the fused objective (pull,
-    residual, push, pullback) for all volumes as one persistent launch replayed from a HIP
-    graph, against a per-volume loop of the same fused call, against the per-volume loop of
-    separate pull / residual / push / pullback calls, and against the reference's structure:
-    one call per (volume, channel) slice with the phi gradient from pull_grad."""
-    from pushpull import _lib, ops
-    from pushpull import batch as ppb
+def make_c4_volumes(torch, pp, seed=4, n=24):
+    """The uneven batch: n volumes of sizes X 96-224, Y 96-256, Z 96-192 (seeded), 1-4
+    channels, degree 1 or 3, each with its own smooth deformation (max 5 voxels); image f
+    and target g are smoothed noise. Returns (volumes as dicts for BatchPlan, descriptors)."""
+    from pushpull import ops
 
     rng = np.random.default_rng(seed)
     dev = torch.device("cuda:0")
     gen = torch.Generator(device=dev)
     gen.manual_seed(seed)
     vols, descs = [], []
-    for _ in range(24):
+    for _ in range(n):
         X, Y, Z = int(rng.integers(96, 225)), int(rng.integers(96, 257)), int(rng.integers(96, 193))
         C, deg = int(rng.integers(1, 5)), int(rng.choice([1, 3]))
         d = (Z, Y, X)
@@ -238,6 +234,37 @@ def bench_batched(torch, pp, reps=5, seed=4):
         phi = (torch.from_numpy(pp.id(dim=(X, Y, Z))).to(dev) + u).contiguous()
         vols.append(dict(f=f, phi=phi, aux=g, deg=deg, bnd=(1, 1, 1), ext=1, C=C, d=d))
         descs.append(ops._desc(pp.Settings(deg, 1, 1), d, d, C, 1))
+    return vols, descs
+
+
+def make_c5_inputs(torch, pp, seed=5, B=8, C=4):
+    """The matching loop's subjects: B subjects at 256^3 with C channels (f, g smoothed
+    noise), phi = identity + smooth displacement of max 3 voxels."""
+    dev = torch.device("cuda:0")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    d = (N, N, N)
+    f = _smooth_noise(torch, (B, C) + d, 2.0, gen, dev)
+    g = _smooth_noise(torch, (B, C) + d, 2.0, gen, dev)
+    u = _smooth_noise(torch, (B, 3) + d, 4.0, gen, dev)
+    u *= 3.0 / u.abs().max()
+    phi = (torch.from_numpy(pp.id(dim=d)).to(dev)[None] + u).contiguous()
+    del u
+    return f, g, phi
+
+
+def bench_batched(torch, pp, reps=5, seed=4):
+    """24 volumes of uneven size (X 96-224, Y 96-256, Z 96-192), 1-4 channels, degree 1
+    or 3, each with its own smooth deformation (max 5 voxels): the fused objective (pull,
+    residual, push, pullback) for all volumes as one persistent launch replayed from a HIP
+    graph, against a per-volume loop of the same fused call, against the per-volume loop of
+    separate pull / residual / push / pullback calls, and against the reference's structure:
+    one call per (volume, channel) slice with the phi gradient from pull_grad."""
+    from pushpull import _lib, ops
+    from pushpull import batch as ppb
+
+    dev = torch.device("cuda:0")
+    vols, descs = make_c4_volumes(torch, pp, seed)
     chan_vox = sum(v["C"] * int(np.prod(v["d"])) for v in vols)
     _progress(f"batched: {len(vols)} volumes, {chan_vox / 1e6:.0f} M channel-voxels")
     plan = ppb.BatchPlan("objective", vols)
@@ -513,15 +540,8 @@ def bench_matching(torch, pp, iters=50, alpha=0.02, seed=5, check=True):
     from pushpull import batch as ppb
 
     dev = torch.device("cuda:0")
-    gen = torch.Generator(device=dev)
-    gen.manual_seed(seed)
-    B, C, d = 8, 4, (N, N, N)
-    f = _smooth_noise(torch, (B, C) + d, 2.0, gen, dev)
-    g = _smooth_noise(torch, (B, C) + d, 2.0, gen, dev)
-    u = _smooth_noise(torch, (B, 3) + d, 4.0, gen, dev)
-    u *= 3.0 / u.abs().max()
-    phi = (torch.from_numpy(pp.id(dim=d)).to(dev)[None] + u).contiguous()
-    del u
+    B, C = 8, 4
+    f, g, phi = make_c5_inputs(torch, pp, seed, B, C)
     dphi = torch.empty_like(phi)
     df = torch.empty_like(f)
     loss = torch.zeros(B, dtype=torch.float64, device=dev)

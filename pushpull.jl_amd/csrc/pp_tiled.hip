@@ -395,11 +395,12 @@ __device__ __forceinline__ void stage_box(float* box, const TileState& S, const 
 // Per channel and tile, s = 50 - e where 2^e bounds the largest |value|: every scaled
 // contribution is below 2^50 (tap weights are at most 1) and no cell can overflow (at
 // most 2^10 samples per tile touch a cell). Integer sums are exact and order-independent;
-// each cell is rounded to float once, when it is flushed. The quantum 2^(e-50) is below
-// 2^-49 of the tile's largest value, so a cell's error stays far below the float
-// rounding of the partial sums that the flush adds into the image -- relative to the
-// cell's own magnitude, as with the reference's float atomics, down to cells 2^-25 of
-// the tile's largest value.
+// each cell is rounded to float once, when it is flushed. A contribution errs by at most
+// half the quantum 2^(e-50), i.e. 2^-50 of the tile's largest |value|, so a cell that
+// receives n contributions from the tile carries at most n 2^-50 of it: far below the
+// float rounding of the partial sums the flush adds into the image for every cell within
+// about 2^-20 of the tile's largest value (relative precision, as with the reference's
+// float atomics), an absolute 2^-44 of that value (n = 64) for the cells below.
 
 // A channel's conversion factor mul: contribution c -> round(c * mul); cell q -> q / mul.
 __device__ __forceinline__ double fixed_scale(float vmax) {
@@ -610,7 +611,7 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, int 
   tile_setup<DP, TX, TY, TZ, false, true>(a, phi, tile, cap, V, S, prefetch, bound0, box, &bound_c0);
   const bool lds = S.lds != 0;
   const int P = S.Pc, Q = S.Qc;
-  const int rot = scatter_rot<TX, TY, TZ>();
+  const int srot = scatter_rot<TX, TY, TZ>();
   // push with a count image (out2): one more channel of ones after the nchan channels
   const int nsrc = a.nchan + ((!GRAD && a.out2) ? 1 : 0);
   for (int c = 0; c < nsrc; ++c) {
@@ -657,26 +658,6 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, int 
         __syncthreads();
       }
     }
-    // samples outside the box (rare): through global memory, per sample
-    unsigned glob = V.valid & ~use;
-    if (!GRAD) {
-#pragma unroll
-      for (int q = 0; q < VPT; ++q)
-        if (!isfinite(g[0][q])) glob &= ~(1u << q);  // push skips non-finite values (pushpull.c:155)
-    }
-    if (__any(glob != 0u)) {  // one slot at a time, rotating the arrays (VPT rotations restore them)
-#pragma unroll 1
-      for (int q = 0; q < VPT; ++q, V.rotate(), rot_rows(g)) {
-        if (!((glob >> q) & 1u)) continue;
-        Axis<DP> ax, ay, az;
-        constexpr int NEED = GRAD ? (kW | kDW) : kW;
-        make_axis<DP, NEED>(V.x[0], DP, a.d0x, 1, chan_code(a, c, 0, a.bx), ax);
-        make_axis<DP, NEED>(V.y[0], DP, a.d0y, a.d0x, chan_code(a, c, 1, a.by), ay);
-        make_axis<DP, NEED>(V.z[0], DP, a.d0z, a.d0x * a.d0y, chan_code(a, c, 2, a.bz), az);
-        if (GRAD) scatter_grad<DP>(dst, ax, ay, az, g[0][0], g[NG > 1 ? 1 : 0][0], g[NG > 2 ? 2 : 0][0]);
-        else scatter_val<DP>(dst, ax, ay, az, g[0][0]);
-      }
-    }
     if (lds) {
       // the box, one pass per slot in the lane's order (scatter_rot)
       float sx[VPT], sy[VPT], sz[VPT], sg[NG][VPT];
@@ -688,12 +669,12 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, int 
 #pragma unroll
         for (int e = 0; e < NG; ++e) sg[e][q] = g[e][q];
       }
-      rotate_by(sx, rot);
-      rotate_by(sy, rot);
-      rotate_by(sz, rot);
+      rotate_by(sx, srot);
+      rotate_by(sy, srot);
+      rotate_by(sz, srot);
 #pragma unroll
-      for (int e = 0; e < NG; ++e) rotate_by(sg[e], rot);
-      const unsigned suse = rotate_bits<VPT>(use, rot);
+      for (int e = 0; e < NG; ++e) rotate_by(sg[e], srot);
+      const unsigned suse = rotate_bits<VPT>(use, srot);
 #pragma unroll
       for (int q = 0; q < VPT; ++q) {
         if (!((suse >> q) & 1u)) continue;
@@ -718,6 +699,28 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, int 
         }
         sample_fence();
       }
+    }
+    // samples outside the box (rare): through global memory, per sample
+    unsigned glob = V.valid & ~use;
+    if (!GRAD) {
+#pragma unroll
+      for (int q = 0; q < VPT; ++q)
+        if (!isfinite(g[0][q])) glob &= ~(1u << q);  // push skips non-finite values (pushpull.c:155)
+    }
+    if (__any(glob != 0u)) {  // one slot at a time, rotating the arrays (VPT rotations restore them)
+#pragma unroll 1
+      for (int q = 0; q < VPT; ++q, V.rotate(), rot_rows(g)) {
+        if (!((glob >> q) & 1u)) continue;
+        Axis<DP> ax, ay, az;
+        constexpr int NEED = GRAD ? (kW | kDW) : kW;
+        make_axis<DP, NEED>(V.x[0], DP, a.d0x, 1, chan_code(a, c, 0, a.bx), ax);
+        make_axis<DP, NEED>(V.y[0], DP, a.d0y, a.d0x, chan_code(a, c, 1, a.by), ay);
+        make_axis<DP, NEED>(V.z[0], DP, a.d0z, a.d0x * a.d0y, chan_code(a, c, 2, a.bz), az);
+        if (GRAD) scatter_grad<DP>(dst, ax, ay, az, g[0][0], g[NG > 1 ? 1 : 0][0], g[NG > 2 ? 2 : 0][0]);
+        else scatter_val<DP>(dst, ax, ay, az, g[0][0]);
+      }
+    }
+    if (lds) {
       __syncthreads();
       flush_acc(box, S, scale, dst, a, chan_signs(a, c));
       __syncthreads();
@@ -1043,7 +1046,7 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, in
         zero_acc(cbox, S.cells_c);
         __syncthreads();
         // slots in the lane's rotated order (scatter_rot)
-        const int rot = scatter_rot<TX, TY, TZ>();
+        const int srot = scatter_rot<TX, TY, TZ>();
         float sx[VPT], sy[VPT], sz[VPT], sw[VPT];
 #pragma unroll
         for (int q = 0; q < VPT; ++q) {
@@ -1052,11 +1055,11 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, in
           sz[q] = V.z[q];
           sw[q] = auxv[q];
         }
-        rotate_by(sx, rot);
-        rotate_by(sy, rot);
-        rotate_by(sz, rot);
-        rotate_by(sw, rot);
-        const unsigned sfin = rotate_bits<VPT>(rfin, rot);
+        rotate_by(sx, srot);
+        rotate_by(sy, srot);
+        rotate_by(sz, srot);
+        rotate_by(sw, srot);
+        const unsigned sfin = rotate_bits<VPT>(rfin, srot);
 #pragma unroll
         for (int q = 0; q < VPT; ++q) {
           if (!((sfin >> q) & 1u)) continue;
