@@ -53,59 +53,120 @@ def _p(t):
     return ct.c_void_p(t.data_ptr()) if t is not None else ct.c_void_p(0)
 
 
+def _shape3(v, key):
+    t = v.get(key)
+    if t is not None and not isinstance(t, torch.Tensor):
+        raise TypeError(f"`{key}` must be a torch tensor")
+    return t
+
+
+def _volume_records(op: str, volumes: list[dict]):
+    """Validate a list of volumes and describe them as a pp_volume array.
+
+    pp_batch_plan_create and pp_batch_run_static see only pointers and sizes, so every
+    shape, dtype, device and layout is checked here: a mismatch would make the kernels
+    read or write outside a tensor. Shapes first, then dtypes, then devices and layout.
+    Returns (records, ctypes array); the records keep every tensor the array points at
+    alive (inputs made contiguous, outputs allocated where not given)."""
+    if op not in OPS:
+        raise ValueError(f"unknown batch op {op!r}; one of {sorted(OPS)}")
+    if not volumes:
+        raise ValueError("a batch needs at least one volume")
+    arr = (PPVolume * len(volumes))()
+    recs = []
+    for i, v in enumerate(volumes):
+        f, phi, aux = _shape3(v, "f"), _shape3(v, "phi"), _shape3(v, "aux")
+        out, out2, loss = _shape3(v, "out"), _shape3(v, "out2"), _shape3(v, "loss")
+        where = f"volume {i}"
+        if f is None or phi is None:
+            raise ValueError(f"{where}: `f` and `phi` are required")
+        if phi.ndim != 4 or phi.shape[0] != 3:
+            raise ValueError(f"{where}: `phi` must be (3, Z, Y, X), got {tuple(phi.shape)}")
+        d1 = tuple(phi.shape[1:])
+        deg = int(v["deg"])
+        if not 0 <= deg <= 4:
+            raise ValueError(f"{where}: degree {deg} outside 0..4")
+        bnd = tuple(int(b) for b in v.get("bnd", (1, 1, 1)))
+        if len(bnd) != 3 or any(b not in (0, 1, 2) for b in bnd):
+            raise ValueError(f"{where}: `bnd` must be three codes in 0..2, got {bnd}")
+        if op in ("push", "push_grad"):
+            if "dim" not in v:
+                raise ValueError(f"{where}: push needs `dim`, the (Z, Y, X) output grid")
+            d0 = tuple(int(x) for x in v["dim"])
+            if len(d0) != 3 or min(d0) < 1:
+                raise ValueError(f"{where}: `dim` must be three positive sizes, got {d0}")
+            lead = f.shape[:1] if op == "push" else f.shape[:2]
+            want_f = (lead + d1) if op == "push" else (f.shape[:1] + (3,) + d1)
+            if f.ndim != len(want_f) or tuple(f.shape) != tuple(want_f) or f.shape[0] < 1:
+                shape = "(C, Z1, Y1, X1)" if op == "push" else "(C, 3, Z1, Y1, X1)"
+                raise ValueError(f"{where}: `f` must be {shape} on phi's grid {d1}, got {tuple(f.shape)}")
+        else:
+            if f.ndim != 4 or f.shape[0] < 1:
+                raise ValueError(f"{where}: `f` must be (C, Z0, Y0, X0), got {tuple(f.shape)}")
+            d0 = tuple(f.shape[1:])
+        C = int(f.shape[0])
+        if op in ("pull_backward", "objective", "pull_vjp"):
+            if aux is None:
+                raise ValueError(f"{where}: `aux` is required for {op}")
+            if tuple(aux.shape) != (C,) + d1:
+                raise ValueError(f"{where}: `aux` must be {(C,) + d1}, got {tuple(aux.shape)}")
+        out_shape = {"pull": (C,) + d1, "push": (C,) + d0, "push_grad": (C,) + d0}.get(op, (3,) + d1)
+        if out is not None and tuple(out.shape) != out_shape:
+            raise ValueError(f"{where}: `out` must be {out_shape}, got {tuple(out.shape)}")
+        if op in ("objective", "pull_vjp") and out2 is not None and tuple(out2.shape) != (C,) + d0:
+            raise ValueError(f"{where}: `out2` must be {(C,) + d0}, got {tuple(out2.shape)}")
+        if op == "objective" and loss is not None and tuple(loss.shape) not in ((), (1,)):
+            raise ValueError(f"{where}: `loss` must hold one value, got shape {tuple(loss.shape)}")
+        # dtypes
+        for key, t, dt in (("f", f, torch.float32), ("phi", phi, torch.float32), ("aux", aux, torch.float32),
+                           ("out", out, torch.float32), ("out2", out2, torch.float32),
+                           ("loss", loss, torch.float64)):
+            if t is not None and t.dtype != dt:
+                raise TypeError(f"{where}: `{key}` must be {dt}, got {t.dtype}")
+        # devices and layout
+        if not f.is_cuda:
+            raise TypeError(f"{where}: batched volumes are device tensors")
+        for key, t in (("phi", phi), ("aux", aux), ("out", out), ("out2", out2), ("loss", loss)):
+            if t is not None and t.device != f.device:
+                raise ValueError(f"{where}: `{key}` is on {t.device}, `f` on {f.device}")
+        for key, t in (("out", out), ("out2", out2), ("loss", loss)):
+            if t is not None and not t.is_contiguous():
+                raise ValueError(f"{where}: `{key}` must be contiguous (it is written in place)")
+        f, phi = f.contiguous(), phi.contiguous()
+        aux = aux.contiguous() if aux is not None else None
+        if out is None:
+            out = torch.empty(out_shape, dtype=torch.float32, device=f.device)
+        if op in ("objective", "pull_vjp") and out2 is None:
+            out2 = torch.empty((C,) + d0, dtype=torch.float32, device=f.device)
+        if op == "objective" and loss is None:
+            loss = torch.zeros((), dtype=torch.float64, device=f.device)
+        if op not in ("objective", "pull_vjp"):
+            out2 = None
+        if op != "objective":
+            loss = None
+        rec = dict(f=f, phi=phi, aux=aux, out=out, out2=out2, loss=loss)
+        recs.append(rec)
+        s = arr[i]
+        s.f, s.phi, s.aux, s.out, s.out2, s.loss = (_p(rec[k]) for k in ("f", "phi", "aux", "out", "out2", "loss"))
+        for q in range(3):
+            s.d0[q] = int(d0[2 - q])
+            s.d1[q] = int(d1[2 - q])
+            s.bnd[q] = bnd[q]
+        s.nchan, s.deg, s.ext = C, deg, int(v.get("ext", 1))
+    return recs, arr
+
+
 class BatchPlan:
-    """One op over a list of volumes (dicts with keys f, phi, deg, bnd, ext, aux)."""
+    """One op over a list of volumes (dicts with keys f, phi, deg, bnd, ext, aux, dim and
+    optionally the outputs out, out2, loss)."""
 
     def __init__(self, op: str, volumes: list[dict]):
+        self.handle = None
+        self.vols, self._arr = _volume_records(op, volumes)
         self.L = _sigs()
         self.op = op
-        self.vols = []
-        arr = (PPVolume * len(volumes))()
-        for i, v in enumerate(volumes):
-            f = v["f"].contiguous()
-            phi = v["phi"].contiguous()
-            if f.dtype != torch.float32 or phi.dtype != torch.float32 or not f.is_cuda:
-                raise TypeError("batched volumes are float32 device tensors")
-            if phi.ndim != 4 or phi.shape[0] != 3:
-                raise AssertionError("`ϕ` must be (3, Z, Y, X)")
-            d1 = tuple(phi.shape[1:])
-            C = f.shape[0]
-            aux = v.get("aux")
-            aux = aux.contiguous() if aux is not None else None
-            if op in ("pull", "pull_backward", "objective", "pull_vjp"):
-                d0 = tuple(f.shape[-3:])
-            else:
-                d0 = tuple(v["dim"])
-            if op == "pull":
-                shape = (C,) + d1
-            elif op in ("push", "push_grad"):
-                shape = (C,) + d0
-            else:
-                shape = (3,) + d1
-            out = v.get("out")
-            if out is None:
-                out = torch.empty(shape, dtype=torch.float32, device=f.device)
-            elif tuple(out.shape) != shape or not out.is_contiguous():
-                raise AssertionError(f"`out` must be a contiguous {shape} tensor")
-            out2 = loss = None
-            if op in ("objective", "pull_vjp"):
-                out2 = v.get("out2")
-                out2 = torch.empty_like(f) if out2 is None else out2
-            if op == "objective":
-                loss = v.get("loss")
-                loss = torch.zeros((), dtype=torch.float64, device=f.device) if loss is None else loss
-            rec = dict(f=f, phi=phi, aux=aux, out=out, out2=out2, loss=loss)
-            self.vols.append(rec)
-            s = arr[i]
-            s.f, s.phi, s.aux, s.out, s.out2, s.loss = (_p(rec[k]) for k in ("f", "phi", "aux", "out", "out2", "loss"))
-            for q in range(3):
-                s.d0[q] = int(d0[2 - q])
-                s.d1[q] = int(d1[2 - q])
-                s.bnd[q] = int(v.get("bnd", (1, 1, 1))[q])
-            s.nchan, s.deg, s.ext = int(C), int(v["deg"]), int(v.get("ext", 1))
-        self._arr = arr
         h = ct.c_void_p()
-        _lib.check(self.L.pp_batch_plan_create(OPS[op], ct.cast(arr, ct.c_void_p), len(volumes), ct.byref(h)),
+        _lib.check(self.L.pp_batch_plan_create(OPS[op], ct.cast(self._arr, ct.c_void_p), len(volumes), ct.byref(h)),
                    "pp_batch_plan_create")
         self.handle = h
 

@@ -187,3 +187,61 @@ def test_batch_plan_validation_without_gpu():
     assert L.pp_batch_plan_create(0, ct.cast(v, ct.c_void_p), 1, ct.byref(h)) == -1
     assert "degree" in _lib.last_error()
     assert L.pp_batch_plan_create(9, ct.cast(v, ct.c_void_p), 1, ct.byref(h)) == -1
+
+
+def _cpu_vol(op, C=2, d1=(5, 6, 7), d0=(4, 5, 6), **over):
+    import torch
+
+    v = dict(phi=torch.zeros((3,) + d1), deg=3, bnd=(1, 1, 1), ext=1)
+    if op == "push":
+        v.update(f=torch.zeros((C,) + d1), dim=d0)
+    elif op == "push_grad":
+        v.update(f=torch.zeros((C, 3) + d1), dim=d0)
+    else:
+        v.update(f=torch.zeros((C,) + d0), aux=torch.zeros((C,) + d1))
+    v.update(over)
+    return v
+
+
+@pytest.mark.parametrize("op,over,exc,msg", [
+    ("pull", dict(phi=None), ValueError, "required"),
+    ("pull", dict(deg=5), ValueError, "degree"),
+    ("pull", dict(bnd=(1, 3, 1)), ValueError, "bnd"),
+    ("pull", dict(out=np.float32), TypeError, "torch tensor"),
+    ("push", dict(dim=(4, 5)), ValueError, "dim"),
+    ("push", dict(f="grid"), ValueError, "phi's grid"),
+    ("push_grad", dict(f="no3"), ValueError, "(C, 3, Z1, Y1, X1)"),
+    ("objective", dict(aux=None), ValueError, "`aux` is required"),
+    ("objective", dict(aux="grid"), ValueError, "`aux` must be"),
+    ("pull", dict(out="grid"), ValueError, "`out` must be"),
+    ("push", dict(out="grid"), ValueError, "`out` must be"),
+    ("pull_vjp", dict(out2="grid"), ValueError, "`out2` must be"),
+    ("objective", dict(loss="shape"), ValueError, "`loss` must hold one value"),
+    ("objective", dict(loss="f32"), TypeError, "torch.float64"),
+    ("pull", dict(f="f64"), TypeError, "torch.float32"),
+    ("pull", dict(), TypeError, "device tensors"),
+])
+def test_batch_volume_validation(op, over, exc, msg):
+    """Every per-volume tensor is checked before a pointer reaches the library: shapes,
+    then dtypes, then devices (the last check is the one a CPU tensor fails here)."""
+    import torch
+    from pushpull import batch as ppb
+
+    d1, d0, C = (5, 6, 7), (4, 5, 6), 2
+    subs = {
+        "grid": lambda k: torch.zeros((C,) + tuple(x + 1 for x in d1)),
+        "no3": lambda k: torch.zeros((C, 2) + d1),
+        "shape": lambda k: torch.zeros(2, dtype=torch.float64),
+        "f32": lambda k: torch.zeros((), dtype=torch.float32),
+        "f64": lambda k: torch.zeros((C,) + d0, dtype=torch.float64),
+    }
+    over = {k: (subs[v](k) if isinstance(v, str) and v in subs else v) for k, v in over.items()}
+    if "out" in over and over["out"] is np.float32:
+        over["out"] = np.zeros((C,) + d1, np.float32)
+    v = _cpu_vol(op, C, d1, d0, **over)
+    if v.get("phi") is None:
+        v.pop("phi")
+    if v.get("aux") is None:
+        v.pop("aux", None)
+    with pytest.raises(exc, match=msg.replace("(", r"\(").replace(")", r"\)")):
+        ppb._volume_records(op, [v])
