@@ -268,6 +268,16 @@ int pp_batch_plan_create(int op, const pp_volume *vols, int nvol, pp_batch_plan 
 /* Stream-ordered; allocates nothing; capturable. */
 int pp_batch_run(const pp_batch_plan *plan, void *stream);
 int pp_batch_plan_destroy(pp_batch_plan *plan);
+/* The same batch in one call that allocates nothing: no counterpart in the reference. For
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Runs op over vols exactly as a plan of them
+ * would (same kernels, same results), taking the descriptor table from a fixed per-device
+ * array of at most 256 volumes (nvol > 256: PP_ERANGE) and uploading it through kernel
+ * arguments on the stream. Every call on one device shares that array and leaves the last
+ * call's table in it: calls on one stream are ordered, but calls that overlap on one GPU
+ * (other streams, other threads without a stream order between them) race. The pointers
+ * in vols must stay valid until the stream has run the call. */
+int pp_batch_run_static(int op, const pp_volume *vols, int nvol, void *stream);
 
 /* ---- HIP graph capture of any sequence of the calls above (stream must not be NULL) ---- */
 int pp_graph_begin(void *stream);

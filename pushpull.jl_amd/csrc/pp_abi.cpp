@@ -20,16 +20,14 @@
 
 namespace {
 
-thread_local std::string t_err;
+thread_local char t_err[512];  // fixed storage: reporting an error allocates nothing
 thread_local int t_status = 0;
 
 int fail(int code, const char* fmt, ...) {
-  char buf[512];
   va_list ap;
   va_start(ap, fmt);
-  vsnprintf(buf, sizeof buf, fmt, ap);
+  vsnprintf(t_err, sizeof t_err, fmt, ap);
   va_end(ap);
-  t_err = buf;
   t_status = code;
   return code;
 }
@@ -754,17 +752,24 @@ struct pp_batch_plan {
   pp::BatchVol* dvols;
 };
 
-int pp_batch_plan_create(int op, const pp_volume* vols, int nvol, pp_batch_plan** plan) {
-  if (!plan) return fail(PP_EINVAL, "null plan pointer");
-  *plan = nullptr;
+namespace {
+
+// Validate a batch and lay it out: hv[0..nvol) receives the volumes' kernel arguments
+// ordered by degree (stable: input order within a degree), groups[0..*ngroups) one launch
+// each (at most five degrees). Writes only into the caller's arrays (no allocation).
+int build_batch(int op, const pp_volume* vols, int nvol, pp::BatchVol* hv, pp_batch_group* groups, int* ngroups) {
   if (!vols || nvol < 1) return fail(PP_EINVAL, "need at least one volume");
   if (op != PP_OP_PULL && op != PP_OP_PUSH && op != PP_OP_PUSH_GRAD && op != PP_OP_PULL_BACKWARD &&
       op != PP_OP_OBJECTIVE && op != PP_OP_PULL_VJP)
     return fail(PP_EINVAL, "unsupported batch op %d", op);
-  std::vector<pp::BatchVol> hv((size_t)nvol);
+  for (int i = 0; i < nvol; ++i)
+    if (vols[i].deg < 0 || vols[i].deg > 4) return fail(PP_EINVAL, "volume %d: degree %d outside 0..4", i, vols[i].deg);
   long long total = 0;
+  int pos = 0;
+  for (int deg = 0; deg <= 4; ++deg)
   for (int i = 0; i < nvol; ++i) {
     const pp_volume& v = vols[i];
+    if (v.deg != deg) continue;
     long long n0 = 1, n1 = 1;
     for (int q = 0; q < 3; ++q) {
       if (v.d0[q] < 1 || v.d1[q] < 1 || v.d0[q] > kMaxDim || v.d1[q] > kMaxDim)
@@ -775,13 +780,13 @@ int pp_batch_plan_create(int op, const pp_volume* vols, int nvol, pp_batch_plan*
     }
     if (n0 > kMaxVox || n1 > kMaxVox) return fail(PP_ERANGE, "volume %d holds more than 2^31-1 voxels", i);
     if (v.nchan < 1) return fail(PP_EINVAL, "volume %d: nchan must be >= 1", i);
-    if (v.deg < 0 || v.deg > 4) return fail(PP_EINVAL, "volume %d: degree %d outside 0..4", i, v.deg);
     if (!v.f || !v.phi || !v.out) return fail(PP_EINVAL, "volume %d: null array", i);
     if ((op == PP_OP_PULL_BACKWARD || op == PP_OP_OBJECTIVE || op == PP_OP_PULL_VJP) && !v.aux)
       return fail(PP_EINVAL, "volume %d: null aux", i);
     if (op == PP_OP_OBJECTIVE && (!v.out2 || !v.loss)) return fail(PP_EINVAL, "volume %d: null out2/loss", i);
     if (op == PP_OP_PULL_VJP && !v.out2) return fail(PP_EINVAL, "volume %d: null out2", i);
-    pp::KArgs& a = hv[i].a;
+    pp::BatchVol& bv = hv[pos++];
+    pp::KArgs& a = bv.a;
     std::memset(&a, 0, sizeof a);
     a.n1 = (int)n1;
     a.d1x = (int)v.d1[0];
@@ -804,11 +809,11 @@ int pp_batch_plan_create(int op, const pp_volume* vols, int nvol, pp_batch_plan*
     a.phi_bs = 3 * n1;
     a.f = v.f;
     a.out = v.out;
-    hv[i].zero_n = 0;
+    bv.zero_n = 0;
     switch (op) {
       case PP_OP_PULL: a.f_cs = n0; a.out_cs = n1; break;
-      case PP_OP_PUSH: a.f_cs = n1; a.out_cs = n0; hv[i].zero_n = n0 * v.nchan; break;
-      case PP_OP_PUSH_GRAD: a.f_cs = 3 * n1; a.out_cs = n0; hv[i].zero_n = n0 * v.nchan; break;
+      case PP_OP_PUSH: a.f_cs = n1; a.out_cs = n0; bv.zero_n = n0 * v.nchan; break;
+      case PP_OP_PUSH_GRAD: a.f_cs = 3 * n1; a.out_cs = n0; bv.zero_n = n0 * v.nchan; break;
       case PP_OP_PULL_BACKWARD: a.f_cs = n0; a.aux = v.aux; a.aux_cs = n1; break;
       default:
         a.f_cs = n0;
@@ -817,26 +822,26 @@ int pp_batch_plan_create(int op, const pp_volume* vols, int nvol, pp_batch_plan*
         a.out2 = v.out2;
         a.out2_cs = n0;
         a.red = op == PP_OP_OBJECTIVE ? v.loss : nullptr;
-        hv[i].zero_n = n0 * v.nchan;
+        bv.zero_n = n0 * v.nchan;
         break;
     }
-    hv[i].dp = v.deg + 1;
+    bv.dp = v.deg + 1;
     total += pp::batch_tiles(a);
     if (total > kMaxVox) return fail(PP_ERANGE, "too many tiles in one batch");
   }
-  // order by degree (stable), number tiles within each degree group
-  std::stable_sort(hv.begin(), hv.end(), [](const pp::BatchVol& x, const pp::BatchVol& y) { return x.dp < y.dp; });
-  std::vector<pp_batch_group> groups;
+  // number tiles within each degree group
+  int ng = 0;
   for (int i = 0; i < nvol; ++i) {
-    if (groups.empty() || groups.back().dp != hv[i].dp) groups.push_back(pp_batch_group{hv[i].dp, i, 0, 0, {}});
-    pp_batch_group& g = groups.back();
+    if (ng == 0 || groups[ng - 1].dp != hv[i].dp) groups[ng++] = pp_batch_group{hv[i].dp, i, 0, 0, {}};
+    pp_batch_group& g = groups[ng - 1];
     hv[i].tile0 = g.total;
     g.total += pp::batch_tiles(hv[i].a);
     g.nvol += 1;
   }
   // per-XCD ranges of equal work: a tile costs its volume's channel count plus one (the
   // per-tile setup); the cuts fall on tile boundaries
-  for (pp_batch_group& g : groups) {
+  for (int gi = 0; gi < ng; ++gi) {
+    pp_batch_group& g = groups[gi];
     double work = 0.0;
     for (int i = g.first; i < g.first + g.nvol; ++i) work += (double)pp::batch_tiles(hv[i].a) * (hv[i].a.nchan + 1);
     int x = 1;
@@ -854,6 +859,32 @@ int pp_batch_plan_create(int op, const pp_volume* vols, int nvol, pp_batch_plan*
     while (x <= pp::kBatchXcds) g.split.s[x++] = g.total;
     for (int q = 1; q <= pp::kBatchXcds; ++q) g.split.s[q] = std::max(g.split.s[q], g.split.s[q - 1]);
   }
+  *ngroups = ng;
+  return PP_OK;
+}
+
+// Zero fill and one persistent launch per degree group of a laid-out batch in device memory.
+hipError_t run_batch(pp::Op op, const pp::BatchVol* dvols, int nvol, const pp_batch_group* groups, int ngroups,
+                     hipStream_t s) {
+  hipError_t e = pp::launch_batch_zero(op, dvols, nvol, s);
+  for (int gi = 0; gi < ngroups && e == hipSuccess; ++gi)
+    e = pp::launch_batch_group(op, groups[gi].dp, dvols + groups[gi].first, groups[gi].nvol, groups[gi].split, s);
+  return e;
+}
+
+constexpr int kDegGroups = 5;  // degrees 0..4
+
+}  // namespace
+
+int pp_batch_plan_create(int op, const pp_volume* vols, int nvol, pp_batch_plan** plan) {
+  if (!plan) return fail(PP_EINVAL, "null plan pointer");
+  *plan = nullptr;
+  if (!vols || nvol < 1) return fail(PP_EINVAL, "need at least one volume");
+  std::vector<pp::BatchVol> hv((size_t)nvol);
+  pp_batch_group g[kDegGroups];
+  int ng = 0;
+  const int st = build_batch(op, vols, nvol, hv.data(), g, &ng);
+  if (st != PP_OK) return st;
   pp::BatchVol* d = nullptr;
   hipError_t e = hipMalloc((void**)&d, sizeof(pp::BatchVol) * (size_t)nvol);
   if (e != hipSuccess) return fail(PP_ENOMEM, "hipMalloc: %s", hipGetErrorString(e));
@@ -862,17 +893,13 @@ int pp_batch_plan_create(int op, const pp_volume* vols, int nvol, pp_batch_plan*
     (void)hipFree(d);
     return hip_fail(e, "upload batch descriptors");
   }
-  *plan = new pp_batch_plan{static_cast<pp::Op>(op), nvol, std::move(groups), d};
+  *plan = new pp_batch_plan{static_cast<pp::Op>(op), nvol, std::vector<pp_batch_group>(g, g + ng), d};
   return ok();
 }
 
 int pp_batch_run(const pp_batch_plan* plan, void* s) {
   if (!plan) return fail(PP_EINVAL, "null plan");
-  hipError_t e = pp::launch_batch_zero(plan->op, plan->dvols, plan->nvol, S(s));
-  for (const pp_batch_group& g : plan->groups) {
-    if (e != hipSuccess) break;
-    e = pp::launch_batch_group(plan->op, g.dp, plan->dvols + g.first, g.nvol, g.split, S(s));
-  }
+  const hipError_t e = run_batch(plan->op, plan->dvols, plan->nvol, plan->groups.data(), (int)plan->groups.size(), S(s));
   return e == hipSuccess ? ok() : hip_fail(e, "batch launch");
 }
 
@@ -881,6 +908,76 @@ int pp_batch_plan_destroy(pp_batch_plan* plan) {
   hipError_t e = hipFree(plan->dvols);
   delete plan;
   return e == hipSuccess ? ok() : hip_fail(e, "hipFree");
+}
+
+// ---- a batch without allocation ----------------------------------------------------
+// pp_batch_run_static keeps the descriptor table in a fixed device array instead of a
+// plan's hipMalloc'ed one; the descriptors travel as kernel arguments of a small upload
+// kernel (no host staging buffer, nothing whose lifetime outlives the call).
+}  // extern "C"
+
+namespace {
+
+constexpr int kStaticVols = 256;  // volumes per pp_batch_run_static call
+constexpr int kUploadVols = 8;    // descriptors per upload launch
+constexpr int kMaxDevices = 64;
+
+// The table, one copy per device (held until the process ends). Host code takes each
+// device's address from hipGetSymbolAddress and never uses this name as an address.
+static __device__ __align__(256) pp::BatchVol scratch[kStaticVols];
+
+struct VolChunk {
+  pp::BatchVol v[kUploadVols];
+};
+static_assert(sizeof(VolChunk) <= 4000, "upload chunk must fit the kernel-argument limit");
+static_assert(sizeof(pp::BatchVol) % 4 == 0, "descriptors are copied as 32-bit words");
+
+__global__ void __launch_bounds__(256) k_upload_vols(pp::BatchVol* dst, VolChunk src, int n) {
+  const unsigned* w = reinterpret_cast<const unsigned*>(&src);
+  unsigned* d = reinterpret_cast<unsigned*>(dst);
+  const int words = n * (int)(sizeof(pp::BatchVol) / 4);
+  for (int i = threadIdx.x; i < words; i += blockDim.x) d[i] = w[i];
+}
+
+std::mutex g_static_mu;  // guards the two arrays below (host side of the call only)
+pp::BatchVol* g_static_ptr[kMaxDevices];
+pp::BatchVol g_static_host[kStaticVols];
+
+}  // namespace
+
+extern "C" {
+
+int pp_batch_run_static(int op, const pp_volume* vols, int nvol, void* stream) {
+  if (!vols || nvol < 1) return fail(PP_EINVAL, "need at least one volume");
+  if (nvol > kStaticVols)
+    return fail(PP_ERANGE, "pp_batch_run_static takes at most %d volumes (got %d); use a batch plan", kStaticVols,
+                nvol);
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) return fail(PP_EDEVICE, "device %d beyond the %d supported", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_static_mu);
+  if (!g_static_ptr[dev]) {  // first call on this device, which is current
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_static_ptr[dev] = static_cast<pp::BatchVol*>(p);
+  }
+  pp::BatchVol* d = g_static_ptr[dev];
+  pp_batch_group g[kDegGroups];
+  int ng = 0;
+  const int st = build_batch(op, vols, nvol, g_static_host, g, &ng);
+  if (st != PP_OK) return st;
+  for (int i = 0; i < nvol; i += kUploadVols) {
+    const int n = std::min(kUploadVols, nvol - i);
+    VolChunk c{};
+    std::memcpy(c.v, g_static_host + i, sizeof(pp::BatchVol) * (size_t)n);
+    hipLaunchKernelGGL(k_upload_vols, dim3(1), dim3(256), 0, S(stream), d + i, c, n);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "descriptor upload");
+  }
+  e = run_batch(static_cast<pp::Op>(op), d, nvol, g, ng, S(stream));
+  return e == hipSuccess ? ok() : hip_fail(e, "batch launch");
 }
 
 // ---- HIP graphs --------------------------------------------------------------------
@@ -1087,7 +1184,7 @@ int pp_tune_load(const char* path) {
   return ok();
 }
 
-const char* pp_last_error(void) { return t_err.c_str(); }
+const char* pp_last_error(void) { return t_err; }
 int pp_last_status(void) { return t_status; }
 const char* pp_version(void) { return "pushpull_hip 0.1.0 (gfx950)"; }
 

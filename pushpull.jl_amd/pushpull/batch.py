@@ -4,7 +4,8 @@ The reference only batches same-size volumes, one native call per (channel, batc
 (src/pushpull_cpu.jl:42-51). ``BatchPlan`` takes a list of volumes with their own sizes,
 channel counts, degrees and deformations and runs one op over all of them in one launch
 (``pp_batch_plan_create`` / ``pp_batch_run``). ``Graph`` captures any sequence of calls on a
-side stream into a HIP graph (``pp_graph_*``) and replays it.
+side stream into a HIP graph (``pp_graph_*``) and replays it. ``run_static`` runs one op
+over a list of volumes without any device allocation (``pp_batch_run_static``).
 
 Per volume (C-order shapes, as everywhere in this package):
   f    image (C, Z0, Y0, X0) for pull-type ops; values (C, Z1, Y1, X1) for push,
@@ -41,6 +42,8 @@ def _sigs():
     L.pp_batch_run.restype = ct.c_int
     L.pp_batch_plan_destroy.argtypes = [ct.c_void_p]
     L.pp_batch_plan_destroy.restype = ct.c_int
+    L.pp_batch_run_static.argtypes = [ct.c_int, ct.c_void_p, ct.c_int, ct.c_void_p]
+    L.pp_batch_run_static.restype = ct.c_int
     for n, a in (("pp_graph_begin", [ct.c_void_p]), ("pp_graph_end", [ct.c_void_p, ct.POINTER(ct.c_void_p)]),
                  ("pp_graph_launch", [ct.c_void_p, ct.c_void_p]), ("pp_graph_destroy", [ct.c_void_p])):
         fn = getattr(L, n)
@@ -185,6 +188,25 @@ class BatchPlan:
             self.close()
         except Exception:
             pass
+
+
+# volumes per pp_batch_run_static call (the size of its fixed device table)
+STATIC_MAX_VOLUMES = 256
+
+
+def run_static(op: str, volumes: list[dict], stream=None):
+    """One op over a list of volumes through pp_batch_run_static: the results of a
+    ``BatchPlan`` of the same volumes, without the plan's device allocation (the library
+    keeps the descriptor table in a fixed per-device array of 256 volumes). Pass every
+    output (out, out2, loss) to keep the call free of allocations, e.g. under graph
+    capture. Calls on one stream are ordered; calls that overlap on one GPU race on the
+    shared table. Returns the per-volume records (keep them until the stream has run)."""
+    recs, arr = _volume_records(op, volumes)
+    L = _sigs()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    _lib.check(L.pp_batch_run_static(OPS[op], ct.cast(arr, ct.c_void_p), len(volumes), ct.c_void_p(s.cuda_stream)),
+               "pp_batch_run_static")
+    return recs
 
 
 class Graph:
