@@ -7,7 +7,7 @@ ChainRulesCore rrules in src/pushpull_pb.jl:11-51.
 
 The phi pullbacks are single fused kernels (pp_pull_backward / pp_pull_grad_backward):
 the 3C-channel pull_grad the Julia rule materialises before contracting
-(src/pushpull_pb.jl:113-117) is never written to memory. When both gradients of pull are
+(src/pushpull_pb.jl:11-20) is never written to memory. When both gradients of pull are
 wanted, one launch computes them (pp_pull_vjp: each tile of f0 is staged once, phi and
 delta are read once). For pull_grad and push, when both gradients are wanted on the GPU,
 the pullback runs on a second stream while the other gradient runs on the current one
@@ -157,7 +157,14 @@ def _as5(t):
 
 
 def _wants_grad(*ts):
-    return torch.is_grad_enabled() and any(isinstance(t, torch.Tensor) and t.requires_grad for t in ts)
+    """True when the call must record an autograd node. The rrules run on the GPU only: a
+    CPU tensor that requires grad is refused rather than silently dropping the graph."""
+    if not torch.is_grad_enabled() or not any(isinstance(t, torch.Tensor) and t.requires_grad for t in ts):
+        return False
+    if not all(t.is_cuda for t in ts if isinstance(t, torch.Tensor)):
+        raise TypeError("differentiable pull/pull_grad/push take device tensors; "
+                        "a CPU tensor that requires grad is not supported (detach it or move it to the GPU)")
+    return True
 
 
 def pull(f0, phi, sett=None):

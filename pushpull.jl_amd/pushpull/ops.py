@@ -212,15 +212,35 @@ def push_grad(g1, phi, dim, sett=None):
     return _push_family("pushg", 3, g1, phi, dim, sett or Settings())
 
 
+def _same_device(*named):
+    dev = named[0][1].device
+    for name, t in named[1:]:
+        _req(t.device == dev, f"`{name}` must be on the device of `{named[0][0]}` ({dev}), not {t.device}")
+
+
+def _out_arg(t, shape, like, name):
+    """A caller-supplied output: float32, contiguous, the expected shape, on like's device."""
+    _req(isinstance(t, torch.Tensor) and t.dtype == torch.float32, f"`{name}` must be a Float32 tensor")
+    _req(tuple(t.shape) == tuple(shape), f"`{name}` must be {tuple(shape)}, got {tuple(t.shape)}")
+    _req(t.is_contiguous(), f"`{name}` must be contiguous (it is written in place)")
+    _same_device(("f₀", like), (name, t))
+    return t
+
+
 def _backward(kind, f0, phi, delta, sett):
     """Device-only channel contractions used by the pullbacks."""
     f0 = _f32(f0, "f₀")
     phi = _f32(phi, "ϕ")
     delta = _f32(delta, "δ")
     _req(_is_dev(f0) and _is_dev(phi) and _is_dev(delta), "pullback kernels take device tensors")
+    _same_device(("f₀", f0), ("ϕ", phi), ("δ", delta))
     lead, d0 = _dims5(f0.shape, 3, 5, "f₀")
     nb, d1 = _phi_dims(phi)
+    _req(nb == (f0.shape[0] if f0.ndim == 5 else 1), "`ϕ` & `f₀` must have the same batch size")
     nc = f0.shape[-4] if f0.ndim >= 4 else 1
+    want = lead + d1 if kind == 1 else lead + (3,) + d1
+    _req(tuple(delta.shape) == want,
+         f"δ must have the shape of {'pull' if kind == 1 else 'pull_grad'}(f0, phi), {want}, got {tuple(delta.shape)}")
     out = torch.empty((nb, 3) + d1 if phi.ndim == 5 else (3,) + d1, dtype=torch.float32, device=f0.device)
     d = _desc(sett, d0, d1, nc, nb)
     fn = _lib.lib().pp_pull_backward if kind == 1 else _lib.lib().pp_pull_grad_backward
@@ -234,7 +254,7 @@ def pull_backward(f0, phi, delta, sett=None):
 
 
 def pull_grad_backward(f0, phi, delta, sett=None):
-    """sum_{a,c} delta_{a,c} * pull_hess(f0, phi)_{a,.,c} -> shape of phi (src/pushpull_pb.jl:128-132)."""
+    """sum_{a,c} delta_{a,c} * pull_hess(f0, phi)_{a,.,c} -> shape of phi (src/pushpull_pb.jl:26-35)."""
     return _backward(3, f0, phi, delta, sett or Settings())
 
 
@@ -247,13 +267,17 @@ def pull_vjp(f0, phi, delta, sett=None, df0=None, accumulate=False):
     phi = _f32(phi, "ϕ")
     delta = _f32(delta, "δ")
     _req(_is_dev(f0) and _is_dev(phi) and _is_dev(delta), "pull_vjp takes device tensors")
+    _same_device(("f₀", f0), ("ϕ", phi), ("δ", delta))
     lead, d0 = _dims5(f0.shape, 3, 5, "f₀")
     nb, d1 = _phi_dims(phi)
+    _req(nb == (f0.shape[0] if f0.ndim == 5 else 1), "`ϕ` & `f₀` must have the same batch size")
     nc = f0.shape[-4] if f0.ndim >= 4 else 1
     _req(tuple(delta.shape) == lead + d1, "δ must have the shape of pull(f0, phi)")
     dphi = torch.empty(tuple(phi.shape), dtype=torch.float32, device=f0.device)
     if df0 is None:
         df0 = torch.empty(tuple(f0.shape), dtype=torch.float32, device=f0.device)
+    else:
+        _out_arg(df0, f0.shape, f0, "df0")
     d = _desc(sett, d0, d1, nc, nb)
     _lib.check(_lib.lib().pp_pull_vjp(ct.byref(d), _ptr(dphi), _ptr(df0), _ptr(phi), _ptr(f0), _ptr(delta),
                                       int(accumulate), _stream(f0)), "pp_pull_vjp")
@@ -269,13 +293,17 @@ def objective_grad(f0, phi, g, sett=None, df0=None, accumulate=False):
     phi = _f32(phi, "ϕ")
     g = _f32(g, "g")
     _req(_is_dev(f0) and _is_dev(phi) and _is_dev(g), "objective_grad takes device tensors")
+    _same_device(("f₀", f0), ("ϕ", phi), ("g", g))
     lead, d0 = _dims5(f0.shape, 3, 5, "f₀")
     nb, d1 = _phi_dims(phi)
+    _req(nb == (f0.shape[0] if f0.ndim == 5 else 1), "`ϕ` & `f₀` must have the same batch size")
     nc = f0.shape[-4] if f0.ndim >= 4 else 1
     _req(tuple(g.shape) == lead + d1, "g must have the shape of pull(f0, phi)")
     dphi = torch.empty(tuple(phi.shape), dtype=torch.float32, device=f0.device)
     if df0 is None:
         df0 = torch.empty(tuple(f0.shape), dtype=torch.float32, device=f0.device)
+    else:
+        _out_arg(df0, f0.shape, f0, "df0")
     loss = torch.empty((), dtype=torch.float64, device=f0.device)
     d = _desc(sett, d0, d1, nc, nb)
     _lib.check(_lib.lib().pp_objective_grad(ct.byref(d), _ptr(dphi), _ptr(df0), _ptr(loss), _ptr(phi), _ptr(f0),

@@ -165,7 +165,13 @@ def test_tune_table_file_format(tmp_path):
     q = tmp_path / "u.txt"
     assert L.pp_tune_save(str(q).encode()) == 0
     rows = sorted(l.split() for l in q.read_text().splitlines() if not l.startswith("#"))
-    assert rows == [["0", "4", "24", "2"], ["3", "2", "20", "0"]]
+    assert rows == [["0", "4", "24", "2", "0"], ["3", "2", "20", "0", "0"]]
+    # five columns: the box budget of a tiled choice
+    p.write_text("7 4 24 1 10184\n")
+    assert L.pp_tune_load(str(p).encode()) == 0 and L.pp_tune_save(str(q).encode()) == 0
+    assert [l.split() for l in q.read_text().splitlines() if l.startswith("7")] == [["7", "4", "24", "1", "10184"]]
+    p.write_text("7 4 24 1 100\n")  # a budget below 256 floats
+    assert L.pp_tune_load(str(p).encode()) == -1
     bad = tmp_path / "bad.txt"
     bad.write_text("0 4 24 9\n")
     assert L.pp_tune_load(str(bad).encode()) == -1 and "malformed" in _lib.last_error()
@@ -245,3 +251,18 @@ def test_batch_volume_validation(op, over, exc, msg):
         v.pop("aux", None)
     with pytest.raises(exc, match=msg.replace("(", r"\(").replace(")", r"\)")):
         ppb._volume_records(op, [v])
+
+
+@pytest.mark.parametrize("fn", ["pull", "pull_grad", "push"])
+def test_autograd_refuses_cpu_tensors_that_require_grad(fn):
+    import torch
+
+    d = (4, 5, 6)
+    phi = torch.from_numpy(pp.id(dim=d[::-1])).requires_grad_(True)
+    f = torch.zeros((1,) + d)
+    s = pp.Settings(1, 1, 1)
+    with pytest.raises(TypeError, match="device tensors"):
+        if fn == "push":
+            pp.push(f, phi, d, s)
+        else:
+            getattr(pp, fn)(f, phi, s)
