@@ -7,8 +7,8 @@
  *
  * Two tiers.
  *
- * Tier A: symbol-compatible with the reference's native library (artifacts/C/pushpull.h:24-31
- *   plus pullh, artifacts/C/pushpull.c:111). Host pointers, one (channel, batch) slice per
+ * Tier A: symbol-compatible with the reference's native library (artifacts/C/pushpull.h:3-14
+ *   plus pullh, artifacts/C/pushpull.c:70). Host pointers, one (channel, batch) slice per
  *   call, synchronous: upload, launch on the GPU, download. The reference's Julia wrappers
  *   (src/pushpull_cpu.jl:45-49, :100-104, :140-144, :195-199, :251-255) can ccall these
  *   unchanged once `pplib` (src/pushpull_init.jl:4) points at this library. Like the
@@ -21,6 +21,15 @@
  *   unlike the reference CUDA module's __constant__ settings, artifacts/C/pushpull.cu:16-24).
  *   Every function returns 0 on success or a negative PP_E* code, with a message in
  *   pp_last_error() (thread-local).
+ *
+ * Tier B memory contract: every array (and every pointer in a pp_volume) is
+ *   coarse-grained device memory of the current device -- hipMalloc / hipMallocAsync
+ *   memory, which is what torch and most allocators hand out. The scatters accumulate with
+ *   device-scope float atomics that are exact read-modify-writes only there. Host memory
+ *   (pinned or not), managed or unified memory and memory of another device are rejected
+ *   with PP_EINVAL (hipPointerGetAttributes, checked on the host at every call).
+ *   Fine-grained device memory (hipExtMallocWithFlags + hipDeviceMallocFinegrained) is
+ *   outside the contract too but reports the same attributes and is not detected.
  *
  * Array layout (column-major, as Julia): image (X0,Y0,Z0,C,B); deformation phi
  * (X1,Y1,Z1,3,B) holding 1-BASED voxel coordinates into the image grid; pull output
@@ -42,35 +51,35 @@ typedef unsigned long long pp_usize;
 /* ---------------------------------------------------------------- Tier A ---- */
 
 /* f1[n1] = pull of f0 (grid d0) at phi (3*n1, 1-based). NaN where the sample is invalid.
- * Replaces artifacts/C/pushpull.c:55 `pull`. dp = degree+1 per axis, in [1,5]. */
+ * Replaces artifacts/C/pushpull.c:14 `pull`. dp = degree+1 per axis, in [1,5]. */
 void pull(float *f1, const float *phi, const float *f0, const pp_usize *d0, pp_usize n1,
           const int *bnd, const pp_usize *dp, int ext);
 
-/* g1[3*n1] = spatial gradient of the pulled image.  Replaces pushpull.c:78 `pullg`. */
+/* g1[3*n1] = spatial gradient of the pulled image.  Replaces pushpull.c:37 `pullg`. */
 void pullg(float *g1, const float *phi, const float *f0, const pp_usize *d0, pp_usize n1,
            const int *bnd, const pp_usize *dp, int ext);
 
-/* h1[9*n1] = Hessian, column-major 3x3 with symmetric copies. Replaces pushpull.c:111 `pullh`. */
+/* h1[9*n1] = Hessian, column-major 3x3 with symmetric copies. Replaces pushpull.c:70 `pullh`. */
 void pullh(float *h1, const float *phi, const float *f0, const pp_usize *d0, pp_usize n1,
            const int *bnd, const pp_usize *dp, int ext);
 
 /* f0 (grid d0) += push of f1[n1] at phi: the adjoint of pull. Accumulates into the caller's
- * f0 like the reference. Replaces pushpull.c:147 `push`. */
+ * f0 like the reference. Replaces pushpull.c:106 `push`. */
 void push(float *f0, const float *phi, const float *f1, const pp_usize *d0, pp_usize n1,
           const int *bnd, const pp_usize *dp, int ext);
 
-/* f0 += push of gradients g1[3*n1]: the adjoint of pullg. Replaces pushpull.c:168 `pushg`
+/* f0 += push of gradients g1[3*n1]: the adjoint of pullg. Replaces pushpull.c:127 `pushg`
  * with the semantics of artifacts/C/pushpull.cu:145-163 (the CPU version's inverted
- * finiteness test at pushpull.c:185 is not reproduced). */
+ * finiteness test at pushpull.c:144 is not reproduced). */
 void pushg(float *f0, const float *phi, const float *g1, const pp_usize *d0, pp_usize n1,
            const int *bnd, const pp_usize *dp, int ext);
 
 /* f1 (grid d1) = pull of f0 at x = A*(i,j,k,1), A the 0-based 3x4 affine (column-major,
- * 12 floats). Replaces pushpull.c:217 `pull_affine`; the loop runs over d1 (n1 is not used). */
+ * 12 floats). Replaces pushpull.c:176 `pull_affine`; the loop runs over d1 (n1 is not used). */
 void pull_affine(float *f1, const float *f0, const pp_usize *d0, pp_usize n1, const pp_usize *d1,
                  const float *Aff, const int *bnd, const pp_usize *dp, int ext);
 
-/* f0 (grid d0) += push of f1 (grid d1) at x = A*(i,j,k,1). Replaces pushpull.c:254 `push_affine`. */
+/* f0 (grid d0) += push of f1 (grid d1) at x = A*(i,j,k,1). Replaces pushpull.c:213 `push_affine`. */
 void push_affine(float *f0, const float *f1, const pp_usize *d0, pp_usize n1, const pp_usize *d1,
                  const float *Aff, const int *bnd, const pp_usize *dp, int ext);
 
@@ -138,11 +147,11 @@ int pp_affine_pull(const pp_desc *d, float *f1, const float *f0, const float *af
 int pp_affine_push(const pp_desc *d, float *f0, const float *f1, const float *aff, int accumulate, void *stream);
 
 /* Pullback of pull wrt phi (src/pushpull_pb.jl:13-17): dphi(d1,3,B) = sum_c delta_c * pull_grad(f0)_c,
- * NaN where the sample is invalid. Also the phi-pullback of push (src/pushpull_pb.jl:144-147)
+ * NaN where the sample is invalid. Also the phi-pullback of push (src/pushpull_pb.jl:41-51)
  * with (f0, delta) := (upstream gradient, pushed image). */
 int pp_pull_backward(const pp_desc *d, float *dphi, const float *phi, const float *f0, const float *delta,
                      void *stream);
-/* Pullback of pull_grad wrt phi (src/pushpull_pb.jl:128-132):
+/* Pullback of pull_grad wrt phi (src/pushpull_pb.jl:26-35):
  * dphi_b = sum_c sum_a delta(a,c) * pull_hess(f0)(a,b,c); delta is (d1,3,C,B). */
 int pp_pull_grad_backward(const pp_desc *d, float *dphi, const float *phi, const float *f0, const float *delta,
                           void *stream);
@@ -321,17 +330,24 @@ int pp_set_option(int key, int value);
 int pp_get_option(int key);
 
 /* ---- autotuner (no counterpart in the reference) ----------------------------
- * Times every kernel variant of `op` (PP_OP_PULL, _PULL_GRAD, _PUSH, _PUSH_GRAD,
+ * Times the kernel candidates of `op` (PP_OP_PULL, _PULL_GRAD, _PUSH, _PUSH_GRAD,
  * _PULL_BACKWARD, _OBJECTIVE, _PULL_VJP) on the caller's buffers -- one warm-up and `reps`
  * timed calls each, HIP events on `stream` -- and records the fastest for (op, degree,
- * log2 of the phi-grid size); later calls with that key use it. The buffers are those of
- * the op's Tier B function (out2 for the objective and the VJP, loss for the objective,
- * aux for the pullback, the objective and the VJP).
+ * log2 of the phi-grid size); later calls with that key use it. Candidates: one work-item
+ * per sample, and each tile shape at the op's default LDS box budget and at the budgets
+ * that let 6, 5, 4 and 3 workgroups share a CU (isotropic degrees >= 1). The buffers are
+ * those of the op's Tier B function (out2 for the objective and the VJP, loss for the
+ * objective, aux for the pullback, the objective and the VJP). The descriptor's channels
+ * must share one boundary triple (PP_EINVAL otherwise: other mixes run fixed kernels).
  * *variant receives 0 (one work-item per sample) or 1+k (tiled, tile shape k). */
 int pp_tune(int op, const pp_desc *d, float *out, float *out2, double *loss, const float *phi, const float *f,
             const float *aux, void *stream, int reps, int *variant);
-int pp_tune_save(const char *path);  /* text table: op dp log2(voxels) variant */
-int pp_tune_load(const char *path);
+/* The candidates of this thread's last pp_tune, in the order timed: variant, box budget in
+ * floats (0: the default) and mean ms per call, up to `max` of each (arrays may be NULL).
+ * Returns how many were timed. */
+int pp_tune_results(int max, int *variants, int *caps, float *ms);
+int pp_tune_save(const char *path);  /* text table: op dp log2(voxels) variant box_budget */
+int pp_tune_load(const char *path);  /* also reads 4-column tables (budget: default) */
 int pp_tune_clear(void);
 
 /* ---- diagnostics ---------------------------------------------------------- */

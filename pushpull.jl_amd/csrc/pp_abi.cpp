@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -15,6 +16,7 @@
 #include <vector>
 
 #include "pp_kernels.hpp"
+#include "pp_memcheck.hpp"
 #include "pp_sparse.hpp"
 #include "pushpull_hip.h"
 
@@ -41,6 +43,23 @@ int hip_fail(hipError_t e, const char* where) {
   return fail(PP_EDEVICE, "%s: %s", where, hipGetErrorString(e));
 }
 
+// The Tier B memory contract: every non-null array is device memory of the current device.
+struct Arr {
+  const void* p;
+  const char* name;
+};
+int check_mem(std::initializer_list<Arr> arrs) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  for (const Arr& a : arrs) {
+    if (!a.p) continue;
+    if (const char* why = pp::device_memory_issue(a.p, dev))
+      return fail(PP_EINVAL, "`%s` is %s; device arrays must be hipMalloc'ed memory of device %d", a.name, why, dev);
+  }
+  return PP_OK;
+}
+
 constexpr long long kMaxVox = (1LL << 31) - 1;  // per-volume voxel count (32-bit in-volume offsets)
 constexpr long long kMaxDim = 1LL << 29;        // per-axis size (2*d must fit an int)
 constexpr int kBlock = 256;
@@ -53,11 +72,27 @@ std::atomic<int> g_tiling{1};  // PP_OPT_TILING: 0 per sample, 1 automatic, 2 ti
 std::atomic<int> g_shape{0};   // PP_OPT_TILE_SHAPE (default when nothing is tuned)
 std::atomic<int> g_tuned{1};   // PP_OPT_USE_TUNED
 std::atomic<int> g_cap{0};     // PP_OPT_BOX_CAP: LDS box budget in floats (0: per-op default)
-thread_local int t_force = -1; // variant forced while pp_tune times candidates
 
-// Autotuner table: (op, dp, log2 of the phi-grid size) -> variant.
+// A kernel choice: the variant and, for the tiled ones, the LDS box budget in floats
+// (0: PP_OPT_BOX_CAP, else the op's default). The budget sets how many workgroups share a
+// CU (LDS occupancy) against how much of a tile's footprint stays in LDS.
+struct Choice {
+  int variant;
+  int cap;
+};
+thread_local Choice t_force{-1, 0};  // forced while pp_tune times candidates
+
+// Autotuner table: (op, dp, log2 of the phi-grid size) -> choice.
 std::mutex g_tune_mu;
-std::map<std::tuple<int, int, int>, int> g_tune;
+std::map<std::tuple<int, int, int>, Choice> g_tune;
+
+// Budgets pp_tune tries besides the op's default: the largest box that still lets 6, 5, 4
+// and 3 workgroups share a CU's 160 KiB of LDS.
+constexpr int kTuneCaps[] = {6770, 8136, 10184, 13596};
+constexpr int kMaxCandidates = 1 + (kVariants - 1) * (1 + (int)(sizeof kTuneCaps / sizeof kTuneCaps[0]));
+thread_local int t_ncand = 0;
+thread_local Choice t_cand[kMaxCandidates];
+thread_local float t_cand_ms[kMaxCandidates];
 
 int size_class(long long n) {
   int c = 0;
@@ -65,16 +100,17 @@ int size_class(long long n) {
   return c;
 }
 
-hipError_t launch_variant(pp::Op op, const pp::KArgs& a, int nbatch, hipStream_t s, int variant) {
+hipError_t launch_variant(pp::Op op, const pp::KArgs& a, int nbatch, hipStream_t s, int variant, int cap = 0) {
   if (variant > 0) {
-    hipError_t e = pp::launch_tiled(op, a, nbatch, s, pp::TileCfg{variant - 1, g_cap.load(std::memory_order_relaxed)});
+    if (cap <= 0) cap = g_cap.load(std::memory_order_relaxed);
+    hipError_t e = pp::launch_tiled(op, a, nbatch, s, pp::TileCfg{variant - 1, cap});
     if (e != hipErrorNotSupported) return e;
   }
   return pp::launch_op(op, a, nbatch, s, kBlock);
 }
 
 hipError_t launch(pp::Op op, const pp::KArgs& a, int nbatch, hipStream_t s) {
-  if (t_force >= 0) return launch_variant(op, a, nbatch, s, t_force);
+  if (t_force.variant >= 0) return launch_variant(op, a, nbatch, s, t_force.variant, t_force.cap);
   const int tiling = g_tiling.load(std::memory_order_relaxed);
   const int tiled = 1 + g_shape.load(std::memory_order_relaxed);
   if (tiling == 0) return launch_variant(op, a, nbatch, s, 0);
@@ -83,7 +119,7 @@ hipError_t launch(pp::Op op, const pp::KArgs& a, int nbatch, hipStream_t s) {
     const bool iso = a.dpx == a.dpy && a.dpx == a.dpz;
     std::lock_guard<std::mutex> lock(g_tune_mu);
     auto it = g_tune.find({(int)op, iso ? a.dpx : 0, size_class(a.n1)});
-    if (it != g_tune.end()) return launch_variant(op, a, nbatch, s, it->second);
+    if (it != g_tune.end()) return launch_variant(op, a, nbatch, s, it->second.variant, it->second.cap);
   }
   // Built-in choice: trilinear gathers (8 taps, served by L1/L2) run per sample, which
   // beats staging a box at that degree on gfx950; everything else runs tiled.
@@ -228,6 +264,7 @@ int pull_family(pp::Op op, int ncomp, const pp_desc* d, float* out, const float*
   Geo g;
   if (int r = check_desc(d, g)) return r;
   if (!out || !phi || !f0) return fail(PP_EINVAL, "null array");
+  if (int r = check_mem({{out, "out"}, {phi, "phi"}, {f0, "f0"}})) return r;
   pp::KArgs a;
   base_args(d, a, g);
   a.f = f0;
@@ -245,6 +282,7 @@ int push_family(pp::Op op, int ncomp, const pp_desc* d, float* f0, const float* 
   Geo g;
   if (int r = check_desc(d, g)) return r;
   if (!f0 || !phi || !f1) return fail(PP_EINVAL, "null array");
+  if (int r = check_mem({{f0, "f0"}, {phi, "phi"}, {f1, "f1"}})) return r;
   if (!accumulate) {
     hipError_t e = pp::launch_fill(f0, g.n0 * d->nchan * d->nbatch, 0.0f, S(s));
     if (e != hipSuccess) return hip_fail(e, "zero fill");
@@ -266,6 +304,7 @@ int backward_family(pp::Op op, int ncomp, const pp_desc* d, float* dphi, const f
   Geo g;
   if (int r = check_desc(d, g)) return r;
   if (!dphi || !phi || !f0 || !delta) return fail(PP_EINVAL, "null array");
+  if (int r = check_mem({{dphi, "dphi"}, {phi, "phi"}, {f0, "f0"}, {delta, "delta"}})) return r;
   pp::KArgs a;
   base_args(d, a, g);
   a.f = f0;
@@ -326,7 +365,7 @@ int tierA_args(const pp_usize* d0, pp_usize n1, const int* bnd, const pp_usize* 
 
 // in_n: floats of the gathered/pushed input; out_n: floats of the output; the output is
 // uploaded first when `upload_out` (push accumulates into the caller's array).
-// aff/d1: affine variant (pushpull.c:217-285): coordinates from the 0-based 3x4 matrix
+// aff/d1: affine variant (pushpull.c:176-244): coordinates from the 0-based 3x4 matrix
 // over the output grid d1 instead of from phi.
 int tierA(pp::Op op, float* out, long long out_n, bool upload_out, const float* phi, const float* in, long long in_n,
           const pp_usize* d0, pp_usize n1, const int* bnd, const pp_usize* dp, int ext,
@@ -342,7 +381,7 @@ int tierA(pp::Op op, float* out, long long out_n, bool upload_out, const float* 
       m *= (long long)d1[q];
     }
     if (m > kMaxVox) return fail(PP_ERANGE, "output grid larger than 2^31-1 voxels");
-    // the reference loops over d1 (pushpull.c:226-249); n1 only sizes the caller's buffer
+    // the reference loops over d1 (pushpull.c:185-208); n1 only sizes the caller's buffer
     n1 = (pp_usize)m;
     a.n1 = (int)m;
     a.d1x = (int)d1[0];
@@ -471,6 +510,7 @@ int pp_affine_pull(const pp_desc* d, float* f1, const float* f0, const float* af
   Geo g;
   if (int r = check_desc(d, g)) return r;
   if (!f1 || !f0 || !aff) return fail(PP_EINVAL, "null array");
+  if (int r = check_mem({{f1, "f1"}, {f0, "f0"}})) return r;
   pp::KArgs a;
   base_args(d, a, g);
   a.affine = 1;
@@ -490,6 +530,7 @@ int pp_affine_push(const pp_desc* d, float* f0, const float* f1, const float* af
   Geo g;
   if (int r = check_desc(d, g)) return r;
   if (!f0 || !f1 || !aff) return fail(PP_EINVAL, "null array");
+  if (int r = check_mem({{f0, "f0"}, {f1, "f1"}})) return r;
   if (!accumulate) {
     hipError_t e = pp::launch_fill(f0, g.n0 * d->nchan * d->nbatch, 0.0f, S(s));
     if (e != hipSuccess) return hip_fail(e, "zero fill");
@@ -522,6 +563,7 @@ int pp_objective_grad(const pp_desc* d, float* dphi, float* df0, double* loss, c
   Geo g;
   if (int r = check_desc(d, g)) return r;
   if (!dphi || !df0 || !loss || !phi || !f0 || !gt) return fail(PP_EINVAL, "null array");
+  if (int r = check_mem({{dphi, "dphi"}, {df0, "df0"}, {loss, "loss"}, {phi, "phi"}, {f0, "f0"}, {gt, "g"}})) return r;
   if (!accumulate) {
     hipError_t e = pp::launch_fill(df0, g.n0 * d->nchan * d->nbatch, 0.0f, S(s));
     if (e == hipSuccess) e = pp::launch_fill_d(loss, 1, 0.0, S(s));
@@ -550,6 +592,7 @@ int pp_pull_vjp(const pp_desc* d, float* dphi, float* df0, const float* phi, con
   Geo g;
   if (int r = check_desc(d, g)) return r;
   if (!dphi || !df0 || !phi || !f0 || !delta) return fail(PP_EINVAL, "null array");
+  if (int r = check_mem({{dphi, "dphi"}, {df0, "df0"}, {phi, "phi"}, {f0, "f0"}, {delta, "delta"}})) return r;
   if (!accumulate) {
     hipError_t e = pp::launch_fill(df0, g.n0 * d->nchan * d->nbatch, 0.0f, S(s));
     if (e != hipSuccess) return hip_fail(e, "zero fill");
@@ -590,6 +633,7 @@ int push_count_impl(const pp_desc* d, float* f0, float* c0, const float* phi, co
   Geo g;
   if (int r = check_desc(&dc, g)) return r;
   if (!f0 || !c0 || (!aff && !phi)) return fail(PP_EINVAL, "null array");
+  if (int r = check_mem({{f0, "f0"}, {c0, "c0"}, {phi, "phi"}, {f1, "f1"}})) return r;
   if (!f1 && d->nchan != 3) return fail(PP_EINVAL, "pushing the identity needs nchan == 3, not %d", d->nchan);
   const int nc = d->nchan, nb = d->nbatch;
   if (!accumulate) {
@@ -694,6 +738,7 @@ bool overlap(const float* p, const float* q, long long n) { return p < q + n && 
 int pp_compose(float* psi, const float* phi, const float* theta, const long long d0[3], const long long d1[3],
                int nbatch, void* s) {
   if (!psi || !phi || !theta || !d0 || !d1) return fail(PP_EINVAL, "null argument");
+  if (int r = check_mem({{psi, "psi"}, {phi, "phi"}, {theta, "theta"}})) return r;
   pp_desc d;
   compose_desc(d, d0, d1, nbatch);
   Geo g;
@@ -705,6 +750,7 @@ int pp_compose(float* psi, const float* phi, const float* theta, const long long
 
 int pp_exp(float* out, const float* v, float* work, const long long dim[3], int nbatch, int nsq, void* s) {
   if (!out || !v || !work || !dim) return fail(PP_EINVAL, "null argument");
+  if (int r = check_mem({{out, "out"}, {v, "v"}, {work, "work"}})) return r;
   if (nsq < 1 || nsq > 30) return fail(PP_EINVAL, "nsq must be in 1..30, not %d", nsq);
   pp_desc d;
   compose_desc(d, dim, dim, nbatch);
@@ -785,6 +831,14 @@ int build_batch(int op, const pp_volume* vols, int nvol, pp::BatchVol* hv, pp_ba
       return fail(PP_EINVAL, "volume %d: null aux", i);
     if (op == PP_OP_OBJECTIVE && (!v.out2 || !v.loss)) return fail(PP_EINVAL, "volume %d: null out2/loss", i);
     if (op == PP_OP_PULL_VJP && !v.out2) return fail(PP_EINVAL, "volume %d: null out2", i);
+    for (const Arr& m : {Arr{v.f, "f"}, Arr{v.phi, "phi"}, Arr{v.aux, "aux"}, Arr{v.out, "out"}, Arr{v.out2, "out2"},
+                         Arr{v.loss, "loss"}}) {
+      if (int r = check_mem({m})) {
+        char why[sizeof t_err];
+        std::snprintf(why, sizeof why, "%s", t_err);
+        return fail(r, "volume %d: %s", i, why);
+      }
+    }
     pp::BatchVol& bv = hv[pos++];
     pp::KArgs& a = bv.a;
     std::memset(&a, 0, sizeof a);
@@ -1032,11 +1086,13 @@ int pp_memcpy_d2h(void* dst, const void* src, size_t bytes, void* s) {
   return e == hipSuccess ? ok() : hip_fail(e, "memcpy d2h");
 }
 int pp_memset_zero(float* p, size_t count, void* s) {
+  if (int r = check_mem({{p, "p"}})) return r;
   hipError_t e = pp::launch_fill(p, (long long)count, 0.0f, S(s));
   return e == hipSuccess ? ok() : hip_fail(e, "zero fill");
 }
 int pp_axpy(float* y, const float* x, float a, size_t count, void* s) {
   if (!y || !x) return fail(PP_EINVAL, "null array");
+  if (int r = check_mem({{y, "y"}, {x, "x"}})) return r;
   hipError_t e = pp::launch_axpy(y, x, a, (long long)count, S(s));
   return e == hipSuccess ? ok() : hip_fail(e, "axpy");
 }
@@ -1092,7 +1148,11 @@ int pp_get_option(int key) {
 int pp_tune(int op, const pp_desc* d, float* out, float* out2, double* loss, const float* phi, const float* f,
             const float* aux, void* s, int reps, int* variant) {
   Geo g;
+  t_ncand = 0;
   if (int r = check_desc(d, g)) return r;
+  if (!uniform_bnd(d))
+    return fail(PP_EINVAL, "pp_tune times descriptors whose channels share one boundary triple "
+                           "(other mixes run fixed kernels and are not looked up in the table)");
   if (reps < 1) reps = 3;
   auto call = [&]() -> int {
     switch (op) {
@@ -1106,13 +1166,25 @@ int pp_tune(int op, const pp_desc* d, float* out, float* out2, double* loss, con
       default: return fail(PP_EINVAL, "op %d has a single kernel variant", op);
     }
   };
+  // candidates: per sample; each tile shape at the default budget and at each other budget
+  const bool iso = d->deg[0] == d->deg[1] && d->deg[0] == d->deg[2];
+  const int dflt = iso && d->deg[0] >= 1 ? pp::default_cap(static_cast<pp::Op>(op), d->deg[0] + 1) : 0;
+  Choice cand[kMaxCandidates];
+  int nc = 0;
+  cand[nc++] = Choice{0, 0};
+  for (int v = 1; v < kVariants; ++v) {
+    cand[nc++] = Choice{v, 0};
+    if (!iso || d->deg[0] < 1) continue;  // no box at degree 0 or anisotropic degrees
+    for (int c : kTuneCaps)
+      if (c != dflt) cand[nc++] = Choice{v, c};
+  }
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return fail(PP_EDEVICE, "hipEventCreate");
   float best_ms = 0.f;
   int best = -1, rc = PP_OK;
-  for (int v = 0; v < kVariants && rc == PP_OK; ++v) {
-    t_force = v;
-    rc = call();  // warm-up (also compiles nothing: all variants are prebuilt)
+  for (int k = 0; k < nc && rc == PP_OK; ++k) {
+    t_force = cand[k];
+    rc = call();  // warm-up (all variants are prebuilt: nothing compiles)
     if (rc != PP_OK) break;
     if (hipEventRecord(e0, S(s)) != hipSuccess) {
       rc = fail(PP_EDEVICE, "hipEventRecord");
@@ -1122,23 +1194,38 @@ int pp_tune(int op, const pp_desc* d, float* out, float* out2, double* loss, con
     if (rc == PP_OK && (hipEventRecord(e1, S(s)) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
       rc = fail(PP_EDEVICE, "hipEventRecord/hipEventSynchronize");
     float ms = 0.f;
-    if (rc == PP_OK && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && (best < 0 || ms < best_ms)) {
-      best_ms = ms;
-      best = v;
+    if (rc == PP_OK && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) {
+      t_cand[t_ncand] = cand[k];
+      t_cand_ms[t_ncand++] = ms / reps;
+      if (best < 0 || ms < best_ms) {
+        best_ms = ms;
+        best = k;
+      }
     }
   }
-  t_force = -1;
+  t_force = Choice{-1, 0};
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc != PP_OK) return rc;
-  const bool iso = d->deg[0] == d->deg[1] && d->deg[0] == d->deg[2];
+  if (best < 0) return fail(PP_EDEVICE, "no candidate could be timed");
   const int internal = op;  // PP_OP_* codes equal the internal op numbers
   {
     std::lock_guard<std::mutex> lock(g_tune_mu);
-    g_tune[{internal, iso ? d->deg[0] + 1 : 0, size_class(g.n1)}] = best;
+    g_tune[{internal, iso ? d->deg[0] + 1 : 0, size_class(g.n1)}] = cand[best];
   }
-  if (variant) *variant = best;
+  if (variant) *variant = cand[best].variant;
   return ok();
+}
+
+int pp_tune_results(int max, int* variants, int* caps, float* ms) {
+  const int n = std::min(max, t_ncand);
+  for (int i = 0; i < n; ++i) {
+    if (variants) variants[i] = t_cand[i].variant;
+    if (caps) caps[i] = t_cand[i].cap;
+    if (ms) ms[i] = t_cand_ms[i];
+  }
+  t_status = PP_OK;
+  return t_ncand;
 }
 
 int pp_tune_clear(void) {
@@ -1151,11 +1238,12 @@ int pp_tune_save(const char* path) {
   if (!path) return fail(PP_EINVAL, "null path");
   FILE* fp = std::fopen(path, "w");
   if (!fp) return fail(PP_EINVAL, "cannot write %s", path);
-  std::fprintf(fp, "# pushpull_hip tuning table: op dp log2(voxels) variant\n");
+  std::fprintf(fp, "# pushpull_hip tuning table: op dp log2(voxels) variant box_budget\n");
   {
     std::lock_guard<std::mutex> lock(g_tune_mu);
     for (const auto& kv : g_tune)
-      std::fprintf(fp, "%d %d %d %d\n", std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), kv.second);
+      std::fprintf(fp, "%d %d %d %d %d\n", std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first),
+                   kv.second.variant, kv.second.cap);
   }
   std::fclose(fp);
   return ok();
@@ -1167,15 +1255,16 @@ int pp_tune_load(const char* path) {
   if (!fp) return fail(PP_EINVAL, "cannot read %s", path);
   char line[256];
   int n = 0;
-  std::map<std::tuple<int, int, int>, int> t;
+  std::map<std::tuple<int, int, int>, Choice> t;
   while (std::fgets(line, sizeof line, fp)) {
     if (line[0] == '#') continue;
-    int op, dp, sc, v;
-    if (std::sscanf(line, "%d %d %d %d", &op, &dp, &sc, &v) != 4 || v < 0 || v >= kVariants) {
+    int op, dp, sc, v, cap = 0;
+    const int got = std::sscanf(line, "%d %d %d %d %d", &op, &dp, &sc, &v, &cap);
+    if (got < 4 || v < 0 || v >= kVariants || (cap != 0 && (cap < 256 || cap > kMaxCap))) {
       std::fclose(fp);
       return fail(PP_EINVAL, "%s: malformed line %d", path, n + 1);
     }
-    t[{op, dp, sc}] = v;
+    t[{op, dp, sc}] = Choice{v, cap};
     ++n;
   }
   std::fclose(fp);

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "pp_sparse.hpp"
+#include "pp_memcheck.hpp"
 #include "pushpull_hip.h"
 
 // A sparse stencil on one device, for volumes of size d.
@@ -38,6 +39,26 @@ int hip_fail(hipError_t e, const char* where) {
   char buf[512];
   std::snprintf(buf, sizeof buf, "%s: %s", where, hipGetErrorString(e));
   return pp::abi_fail(PP_EDEVICE, buf);
+}
+
+// The device-array memory contract (pp_memcheck.hpp) for up to three arrays.
+int check_mem(const void* a, const char* na, const void* b, const char* nb, const void* c = nullptr,
+              const char* nc = nullptr) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  const void* p[3] = {a, b, c};
+  const char* n[3] = {na, nb, nc};
+  for (int i = 0; i < 3; ++i) {
+    if (!p[i]) continue;
+    if (const char* why = pp::device_memory_issue(p[i], dev)) {
+      char buf[512];
+      std::snprintf(buf, sizeof buf, "`%s` is %s; device arrays must be hipMalloc'ed memory of device %d", n[i], why,
+                    dev);
+      return pp::abi_fail(PP_EINVAL, buf);
+    }
+  }
+  return PP_OK;
 }
 
 // Checks the list arrays (the `sparsify` tuple, src/operator.jl:206-296) and builds the
@@ -176,6 +197,7 @@ pp::SpLattice lattice(const pp_sparse_kernel* k, const int start[3], const int s
 int vel2mom_impl(const pp_sparse_kernel* k, float* u, const float* v, const int* bnd, int accum, hipStream_t s) {
   if (int r = check_kernel(k)) return r;
   if (!u || !v || !bnd) return fail(PP_EINVAL, "null argument");
+  if (int r = check_mem(u, "u", v, "v")) return r;
   if (u == v) return fail(PP_EINVAL, "u and v must be different arrays");
   pp::SpArgs a = base_args(k, bnd);
   a.v = const_cast<float*>(v);
@@ -249,6 +271,7 @@ int relax_impl(const pp_sparse_kernel* k, float* v, const float* g, const float*
   if (!v || !g || !bnd) return fail(PP_EINVAL, "null argument");
   if (hcode < 0 || hcode > 2) return fail(PP_EINVAL, "hcode=%lld outside 0..2", hcode);
   if (hcode && !h) return fail(PP_EINVAL, "null h with hcode %lld", hcode);
+  if (int r = check_mem(v, "v", g, "g", hcode ? h : nullptr, "h")) return r;
   if (v == g || (hcode && v == h)) return fail(PP_EINVAL, "v must not alias g or h");
   if (nit < 0) return fail(PP_EINVAL, "nit=%lld is negative", nit);
   pp::SpArgs a = base_args(k, bnd);

@@ -83,6 +83,7 @@ _SIGS = {
     "pp_set_option": ([ct.c_int, ct.c_int], ct.c_int),
     "pp_get_option": ([ct.c_int], ct.c_int),
     "pp_tune": ([_P, _P, _P, _P, _P, _P, _P, _P, _P, ct.c_int, ct.POINTER(ct.c_int)], ct.c_int),
+    "pp_tune_results": ([ct.c_int, _P, _P, _P], ct.c_int),
     "pp_tune_save": ([ct.c_char_p], ct.c_int),
     "pp_tune_load": ([ct.c_char_p], ct.c_int),
     "pp_tune_clear": ([], ct.c_int),

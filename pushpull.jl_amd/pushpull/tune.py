@@ -1,9 +1,10 @@
-"""Autotuner front end (``pp_tune``): pick the fastest kernel variant for an op, degree
-and volume-size class by timing every variant on the caller's own device buffers.
+"""Autotuner front end (``pp_tune``): pick the fastest kernel choice for an op, degree
+and volume-size class by timing every candidate on the caller's own device buffers.
 
 Variants: 0 = one work-item per sample; 1, 2, 3 = LDS-tiled with 16x8x8, 32x8x4, 16x16x4
-tiles of the deformation grid. Choices live in the library (per process) and can be
-saved to / loaded from a small text table.
+tiles of the deformation grid, each at the op's default LDS box budget and at the budgets
+that let 6, 5, 4 or 3 workgroups share a CU. Choices live in the library (per process)
+and can be saved to / loaded from a small text table.
 """
 from __future__ import annotations
 
@@ -56,6 +57,16 @@ def tune(op: str, f, phi, sett: Settings | None = None, aux=None, dim=None, reps
     _lib.check(L.pp_tune(OPS[op], ct.byref(desc), _p(out), _p(out2), _p(loss), _p(phi), _p(f),
                          _p(aux.contiguous() if aux is not None else None), s, int(reps), ct.byref(v)), "pp_tune")
     return int(v.value)
+
+
+def results() -> list[dict]:
+    """The candidates of this thread's last ``tune``: variant, box budget (0: the op's
+    default) and mean ms per call, in the order timed."""
+    L = _lib.lib()
+    n = L.pp_tune_results(0, None, None, None)
+    v, c, m = (ct.c_int * max(n, 1))(), (ct.c_int * max(n, 1))(), (ct.c_float * max(n, 1))()
+    L.pp_tune_results(n, v, c, m)
+    return [dict(variant=int(v[i]), name=VARIANTS[v[i]], cap=int(c[i]), ms=float(m[i])) for i in range(n)]
 
 
 def save(path: str) -> None:
