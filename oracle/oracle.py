@@ -31,7 +31,7 @@ REF_SO = os.path.join(_HERE, "_ref", "libpushpull_ref.so")
 
 _P = ct.c_void_p
 _SIG = [_P, _P, _P, _P, ct.c_ulonglong, _P, _P, ct.c_int]
-# pull_affine / push_affine (artifacts/C/pushpull.h:32-35)
+# pull_affine / push_affine (artifacts/C/pushpull.h:11-14)
 _SIG_AFF = [_P, _P, _P, ct.c_ulonglong, _P, _P, _P, _P, ct.c_int]
 
 _lib = None
@@ -246,7 +246,7 @@ def _aff12(A):
 
 def affine_pull(f0, A0, d1, sett, prec="f32"):
     """affine_pull with an already 0-based 3x4 matrix A0 (see adjust_affine):
-    f0 (B,C,Z0,Y0,X0) -> (B,C)+d1, d1 in array order (Z1,Y1,X1). pushpull.c:217-250."""
+    f0 (B,C,Z0,Y0,X0) -> (B,C)+d1, d1 in array order (Z1,Y1,X1). pushpull.c:176-209."""
     dt = _dtype(prec)
     f0 = np.ascontiguousarray(_as5(f0, 5), dtype=dt)
     B, C = f0.shape[:2]
@@ -267,7 +267,7 @@ def affine_pull(f0, A0, d1, sett, prec="f32"):
 
 
 def affine_push(f1, A0, d0, sett, prec="f32"):
-    """affine_push: f1 (B,C,Z1,Y1,X1) -> (B,C)+d0, pushpull.c:254-285."""
+    """affine_push: f1 (B,C,Z1,Y1,X1) -> (B,C)+d0, pushpull.c:213-244."""
     dt = _dtype(prec)
     f1 = np.ascontiguousarray(_as5(f1, 5), dtype=dt)
     B, C = f1.shape[:2]

@@ -13,12 +13,12 @@
  *                                          mask and the knot ceilf(x - dp/2) are taken in
  *                                          FP32 as the reference takes them; weights,
  *                                          gathers and sums are in double.
- * Signatures follow artifacts/C/pushpull.h:24-31 (plus pullh, pushpull.c:111): the
+ * Signatures follow artifacts/C/pushpull.h:3-14 (plus pullh, pushpull.c:70): the
  * caller passes one (channel, batch) slice; d0 is the source grid for the pull family
  * and the destination grid for the push family; dp = degree + 1 in [1, 5].
  *
  * Deliberate deviation from the reference CPU code: ppo_pushg_* is the true adjoint of
- * ppo_pullg_* (pushpull.cu:145-163). The CPU pushg at pushpull.c:185 tests
+ * ppo_pullg_* (pushpull.cu:145-163). The CPU pushg at pushpull.c:144 tests
  * `!ISFINITE(g[1])` and therefore pushes nothing for finite input.
  */
 #include <math.h>
@@ -62,7 +62,7 @@ static void ppo_fold(int code, long long i, u64 d, u64 *b, int *s)
     }
 }
 
-/* VOXOK (pushpull.c:47-50): finite coordinates and, unless extrapolating, inside
+/* VOXOK (pushpull.c:6-9): finite coordinates and, unless extrapolating, inside
  * [-0.01, d-0.99] on every axis, all in FP32. */
 static int ppo_voxok(const float *x, const u64 *d0, int ext)
 {

@@ -12,9 +12,9 @@
  *   artifacts/C/patch.cu:5-139      boundary folding, gather and scatter of a patch
  *   artifacts/C/pushpull_dev.cu:7-179  patch contractions (value / gradient / Hessian),
  *                                   push1 / pushg1
- *   artifacts/C/pushpull.c:6-193    per-voxel loops, VOXOK validity, NaN rules
+ *   artifacts/C/pushpull.c:6-152    per-voxel loops, VOXOK validity, NaN rules
  *   artifacts/C/pushpull.cu:145-163 push_grad semantics (the CPU pushg at
- *                                   pushpull.c:185 inverts a finiteness test and is
+ *                                   pushpull.c:144 inverts a finiteness test and is
  *                                   not followed; see pp_oracle.c)
  */
 
@@ -262,7 +262,7 @@ static void FN(setup)(const float *x, const u64 *d0, const int *bnd, const int *
     for (a = 0; a < 3; a++) FN(axis)(x[a], dp[a], d0[a], bnd[a], want_d, want_h, &A[a]);
 }
 
-/* ---- per-slice entry points, reference signatures (pushpull.c:55-193) ----- */
+/* ---- per-slice entry points, reference signatures (pushpull.c:14-152) ----- */
 
 void FN(ppo_pull)(REAL *f1, const float *phi, const REAL *f0, const u64 *d0, u64 n1,
                   const int *bnd, const u64 *dpu, int ext)
@@ -302,7 +302,7 @@ void FN(ppo_pullg)(REAL *g1, const float *phi, const REAL *f0, const u64 *d0, u6
     }
 }
 
-/* Hessian, column-major 3x3 with symmetric copies (pushpull.c:111-143) */
+/* Hessian, column-major 3x3 with symmetric copies (pushpull.c:70-102) */
 void FN(ppo_pullh)(REAL *h1, const float *phi, const REAL *f0, const u64 *d0, u64 n1,
                    const int *bnd, const u64 *dpu, int ext)
 {
@@ -336,7 +336,7 @@ void FN(ppo_pullh)(REAL *h1, const float *phi, const REAL *f0, const u64 *d0, u6
 }
 
 /* Push: accumulate into f0 (caller zeroes it); skip non-finite f1 and invalid x
- * (pushpull.c:147-164). Sequential, so deterministic. */
+ * (pushpull.c:106-123). Sequential, so deterministic. */
 void FN(ppo_push)(REAL *f0, const float *phi, const REAL *f1, const u64 *d0, u64 n1,
                   const int *bnd, const u64 *dpu, int ext)
 {
@@ -371,7 +371,7 @@ void FN(ppo_pushg)(REAL *f0, const float *phi, const REAL *g1, const u64 *d0, u6
     }
 }
 
-/* Affine variants (pushpull.c:217-285): coordinates x = A*(i,j,k,1) over the output
+/* Affine variants (pushpull.c:176-244): coordinates x = A*(i,j,k,1) over the output
  * grid d1 from the 0-based 3x4 matrix A (column-major), in the reference's order
  * xj = (A3 j + A6 k) + A9, x = xj + A0 i, all in FP32. */
 static void FN(affine_coords)(const float *A, int i, int j, int k, float *x)

@@ -240,7 +240,7 @@ __device__ __forceinline__ void make_axis(float x, int dp_rt, int d, int stride,
   }
 }
 
-// VOXOK (artifacts/C/pushpull.c:47-50), in FP32 exactly as the reference.
+// VOXOK (artifacts/C/pushpull.c:6-9), in FP32 exactly as the reference.
 __device__ __forceinline__ bool voxok(float x, float y, float z, int ext, float lx, float ly, float lz) {
   const bool fin = isfinite(x) && isfinite(y) && isfinite(z);
   if (!fin) return false;

@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void k_pullh(KArgs a) {
     float h[6];
     sample_hess<DP>(f + c * a.f_cs, ax, ay, az, h);
     float* o = out + c * a.out_cs;
-    // column-major 3x3 (pushpull.c:129-134)
+    // column-major 3x3 (pushpull.c:88-93)
     o[0] = h[0]; o[4 * n1] = h[1]; o[8 * n1] = h[2];
     o[1 * n1] = h[3]; o[3 * n1] = h[3];
     o[2 * n1] = h[4]; o[6 * n1] = h[4];
@@ -265,7 +265,7 @@ __global__ __launch_bounds__(256) void k_objgrad(KArgs a) {
 
 // Vector-Jacobian product of pull (both cotangents of the rrule, src/pushpull_pb.jl:11-17):
 // dphi = sum_c delta_c grad f_c (NaN where invalid); df0 += push(delta) (skips non-finite
-// delta, pushpull.c:155).
+// delta, pushpull.c:114).
 template <int DP>
 __global__ __launch_bounds__(256) void k_pull_vjp(KArgs a) {
   const int bb = blockIdx.y;

@@ -7,13 +7,13 @@ namespace pp {
 
 // Operations on the path. Names follow artifacts/C/pushpull.c / src/pushpull_pb.jl.
 enum Op : int {
-  OP_PULL = 0,      // f1 = pull(f0, phi)                          pushpull.c:55
-  OP_PULLG = 1,     // g1 = pull_grad(f0, phi)        (3 comps)    pushpull.c:78
-  OP_PULLH = 2,     // h1 = pull_hess(f0, phi)        (9 comps)    pushpull.c:111
-  OP_PUSH = 3,      // f0 += push(f1, phi)                         pushpull.c:147
+  OP_PULL = 0,      // f1 = pull(f0, phi)                          pushpull.c:14
+  OP_PULLG = 1,     // g1 = pull_grad(f0, phi)        (3 comps)    pushpull.c:37
+  OP_PULLH = 2,     // h1 = pull_hess(f0, phi)        (9 comps)    pushpull.c:70
+  OP_PUSH = 3,      // f0 += push(f1, phi)                         pushpull.c:106
   OP_PUSHG = 4,     // f0 += push_grad(g1, phi)                    pushpull.cu:145
   OP_PULL_BWD = 5,  // dphi = sum_c delta_c grad f_c               pushpull_pb.jl:13-17
-  OP_PULLG_BWD = 6, // dphi = sum_{a,c} delta_{a,c} H_{a,.,c}      pushpull_pb.jl:128-132
+  OP_PULLG_BWD = 6, // dphi = sum_{a,c} delta_{a,c} H_{a,.,c}      pushpull_pb.jl:26-35
   OP_OBJGRAD = 7,   // r = pull(f0)-g; dphi = sum_c r_c grad f_c; df0 += push(r); loss += r^2/2
   OP_PULL_VJP = 8,  // dphi = sum_c delta_c grad f_c; df0 += push(delta)  pushpull_pb.jl:11-17
   OP_COUNT = 9
@@ -63,7 +63,7 @@ struct KArgs {
   int accum;                 // channel-reducing ops: add to `out` instead of overwriting
   float lx, ly, lz;          // d0 - 0.99f per axis (VOXOK upper bounds)
   int affine;                // 1: sample coordinates come from aff, not from phi
-  float aff[12];             // 0-based 3x4 affine, column-major (artifacts/C/pushpull.c:226-242)
+  float aff[12];             // 0-based 3x4 affine, column-major (artifacts/C/pushpull.c:185-201)
   // Per-channel boundary signs (tiled pull and push): all channels fold with bx/by/bz;
   // channel c takes the antisymmetric sign along axis a only where bit 3c+a of csign is
   // set (codes 1 and 2 fold to the same index and differ only in sign).

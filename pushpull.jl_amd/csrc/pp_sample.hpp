@@ -8,7 +8,7 @@ namespace pp {
 
 
 // Source coordinate of output voxel (i,j,k) under a 0-based affine, in the reference
-// CPU order xj = (A3 j + A6 k) + A9, x = xj + A0 i (artifacts/C/pushpull.c:231-241),
+// CPU order xj = (A3 j + A6 k) + A9, x = xj + A0 i (artifacts/C/pushpull.c:190-200),
 // rounded per operation so knots and masks match it exactly.
 #pragma clang fp contract(off)
 __device__ __forceinline__ void affine_x(const float* A, int i, int j, int k, float& x, float& y, float& z) {
@@ -19,7 +19,7 @@ __device__ __forceinline__ void affine_x(const float* A, int i, int j, int k, fl
 }
 #pragma clang fp contract(fast)
 
-// Coordinates (0-based) of sample v at grid index (i, j, k): phi - 1 (pushpull.c:65-67),
+// Coordinates (0-based) of sample v at grid index (i, j, k): phi - 1 (pushpull.c:24-26),
 // the affine map, or, in displacement mode, (id + dscale * u) - 1 with id the 1-based
 // identity (src/compositions.jl:45: Id .+ u, then pull's phi - 1).
 __device__ __forceinline__ void sample_coords_at(const KArgs& a, const float* phi, int v, int i, int j, int k,

@@ -705,7 +705,7 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, int 
     if (!GRAD) {
 #pragma unroll
       for (int q = 0; q < VPT; ++q)
-        if (!isfinite(g[0][q])) glob &= ~(1u << q);  // push skips non-finite values (pushpull.c:155)
+        if (!isfinite(g[0][q])) glob &= ~(1u << q);  // push skips non-finite values (pushpull.c:114)
     }
     if (__any(glob != 0u)) {  // one slot at a time, rotating the arrays (VPT rotations restore them)
 #pragma unroll 1
@@ -972,7 +972,7 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, in
           make_axis<DP, kW | kDW | kHW>(z, DP, a.d0z, a.d0x * a.d0y, a.bz, az);
           sample_hess<DP>(src, ax, ay, az, h);
         }
-        if (KIND == 5) {  // column-major 3x3 (pushpull.c:129-134)
+        if (KIND == 5) {  // column-major 3x3 (pushpull.c:88-93)
           float* o = a.out + bb * a.out_bs + c * a.out_cs + v;
           o[0] = h[0]; o[4 * n1] = h[1]; o[8 * n1] = h[2];
           o[1 * n1] = h[3]; o[3 * n1] = h[3];

@@ -8,7 +8,7 @@ Run in a container that has the PushPull.jl tree:
 Every case stores its inputs (f0, phi, f1, g1 and the settings) and the outputs of the
 reference's ``pull``, ``pullg``, ``pullh``, ``push`` and ``pushg`` for one slice. The
 reference ``pushg`` output is kept to pin its inverted finiteness test
-(artifacts/C/pushpull.c:185): it is all zeros for finite input.
+(artifacts/C/pushpull.c:144): it is all zeros for finite input.
 
 Inputs are seeded (numpy PCG64); the reference's own test data (test/runtests.jl:44-50:
 8x7x1 grids, 2 channels, phi = id + randn, Settings((2,2,2),(0,1,2),1) and
@@ -143,7 +143,7 @@ def cases():
 
 
 def affine_cases():
-    """Affine pull/push (pushpull.c:217-285): rotations, shears, scalings and shifts of
+    """Affine pull/push (pushpull.c:176-244): rotations, shears, scalings and shifts of
     small grids, every degree, mixed boundaries, ext 0/1."""
     rng = np.random.default_rng(20240602)
     cs = []

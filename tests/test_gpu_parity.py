@@ -100,7 +100,7 @@ def test_golden_tier_b(golden, op, path):
             got, ref = H(pp.pull_hess(f0, phi, s)).reshape(c["pullh"].shape), c["pullh"]
         elif op == "push":
             got, ref = H(pp.push(T(c["f1"]), phi, d0, s)), c["push"]
-        else:  # no usable reference output (pushpull.c:185): the oracle's adjoint
+        else:  # no usable reference output (pushpull.c:144): the oracle's adjoint
             got = H(pp.push_grad(T(c["g1"]), phi, d0, s))
             ref = oracle.push_grad(c["g1"], c["phi"], d0, case_settings(c), prec="f64")[0, 0]
         src = {"push": c["f1"], "pushg": c["g1"]}.get(op, c["f0"])
@@ -216,7 +216,7 @@ def test_pull_grad_backward_and_hess(deg, path):
     _check(H(pp.pull_hess(tf, tphi, s)), oracle.pull_hess(fc, phi[None], so, prec="f64"), "pull_hess")
     _check(H(pp.pull_backward(tf, tphi, T(delta), s)), oracle.backward_phi(fc, phi[None], delta, so), "pull_backward")
     H64 = oracle.pull_hess(fc, phi[None], so, prec="f64")  # (B, C, 3 (b), 3 (a), ...)
-    # sum_{a,c} delta[a,c] * H[a, b]  (src/pushpull_pb.jl:128-132)
+    # sum_{a,c} delta[a,c] * H[a, b]  (src/pushpull_pb.jl:26-35)
     ref = np.zeros((1, 3) + d)
     for c in range(2):
         for a in range(3):
@@ -285,7 +285,7 @@ def test_errors_raise():
 
 
 def test_golden_affine(golden_affine, path):
-    """affine_pull / affine_push against the reference (pushpull.c:217-285), Tier B
+    """affine_pull / affine_push against the reference (pushpull.c:176-244), Tier B
     (device tensors) and Tier A (host arrays through pull_affine / push_affine)."""
     results = []
     for i, c in enumerate(golden_affine):

@@ -5,7 +5,7 @@
   several periods, coordinates beyond 2^20 and beyond the int64 range;
 * the FP64 oracle agrees with the FP32 reference to FP32 rounding;
 * the reference's CPU pushg returns zeros for finite input (its inverted test at
-  artifacts/C/pushpull.c:185), which is why push_grad is pinned by adjointness instead;
+  artifacts/C/pushpull.c:144), which is why push_grad is pinned by adjointness instead;
 * where oracle/_ref is present, a fresh random sweep compares restatement and
   reference live.
 """

@@ -268,7 +268,7 @@ def test_invalid_samples_are_nan_and_skipped():
     assert np.isfinite(p).all() and abs(p.sum() - 63) < 1e-12
 
 
-# ---- affine (src/pushpull_cpu.jl:268-362, pushpull.c:217-285) ----------------------
+# ---- affine (src/pushpull_cpu.jl:268-362, pushpull.c:176-244) ----------------------
 def test_adjust_affine_known_answers():
     import pushpull as pp
 
