@@ -25,6 +25,7 @@ of it (oracle/_build) is timed instead.
 from __future__ import annotations
 
 import argparse
+import subprocess
 import ctypes as ct
 import gc
 import json
@@ -112,8 +113,41 @@ def make_inputs(torch, dev, seed=1):
     return f.contiguous(), phi.contiguous(), delta.contiguous()
 
 
+def host_cpu():
+    """The host's processor as lscpu names it, and the threads this process may use."""
+    info = {"os_cpu_count": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                v = v.strip()
+                info[keys[k.strip()]] = int(v) if v.isdigit() else v
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if "model" not in info:
+        try:
+            with open("/proc/cpuinfo") as fh:
+                info["model"] = next((l.split(":", 1)[1].strip() for l in fh if l.startswith("model name")), "unknown")
+        except OSError:
+            info["model"] = "unknown"
+    return info
+
+
+def cpu_threads():
+    """Worker threads for the all-cores leg: every CPU this process may run on, within the
+    OMP_NUM_THREADS share when the environment sets one (a GPU host's share per GPU)."""
+    n = len(os.sched_getaffinity(0))
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(share))) if share.isdigit() and int(share) > 0 else n
+
+
 def cpu_baseline(f, phi, delta, budget_s=40.0):
-    """Reference CPU path on a z-slab of the workload, single thread."""
+    """The reference's CPU path on z-slabs of the workload: on every host thread this
+    process may use (slices are independent, as the reference's per-slice calls are), and
+    single-threaded for the per-core figure."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
@@ -152,7 +186,7 @@ def cpu_baseline(f, phi, delta, budget_s=40.0):
     # one slab per worker thread (the C calls release the interpreter lock)
     from concurrent.futures import ThreadPoolExecutor
 
-    workers = max(1, min(16, os.cpu_count() or 1))
+    workers = cpu_threads()
     nzw = max(1, nz // 2)
 
     def job(i):
@@ -171,18 +205,12 @@ def cpu_baseline(f, phi, delta, budget_s=40.0):
     with ThreadPoolExecutor(workers) as ex:
         vox_all = sum(ex.map(job, range(workers)))
     t_all = time.perf_counter() - t0
-    model = "unknown"
-    try:
-        with open("/proc/cpuinfo") as fh:
-            model = next((l.split(":", 1)[1].strip() for l in fh if l.startswith("model name")), model)
-    except OSError:
-        pass
-    return {"value": total_vox / t_total / 1e9, "unit": "Gvoxel/s", "cores": 1, "kind": kind,
-            "sample": f"pull+push+pull_grad(+contraction) on 256x256 x{nz}-slice slabs of the same inputs, "
-                      f"degrees 1-3, {t_total:.1f} s of single-thread CPU work",
-            "all_cores": {"value": vox_all / t_all / 1e9, "unit": "Gvoxel/s", "cores": workers,
-                          "sample": f"one 256x256 x{nzw}-slice slab per thread, degrees 1-3, {t_all:.1f} s wall"},
-            "cpu_model": model}
+    return {"value": vox_all / t_all / 1e9, "unit": "Gvoxel/s", "cores": workers, "kind": kind,
+            "sample": f"pull+push+pull_grad(+contraction) at degrees 1-3 on one 256x256 x{nzw}-slice slab of "
+                      f"the same inputs per thread, {workers} threads, {t_all:.1f} s wall",
+            "single_thread": {"value": total_vox / t_total / 1e9, "unit": "Gvoxel/s", "cores": 1,
+                              "sample": f"256x256 x{nz}-slice slabs, degrees 1-3, {t_total:.1f} s of CPU work"},
+            "cpu": host_cpu()}
 
 
 def _time_ms(torch, fn, reps, stream):
