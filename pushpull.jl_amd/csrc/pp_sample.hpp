@@ -22,21 +22,27 @@ __device__ __forceinline__ void affine_x(const float* A, int i, int j, int k, fl
 // Coordinates (0-based) of sample v at grid index (i, j, k): phi - 1 (pushpull.c:24-26),
 // the affine map, or, in displacement mode, (id + dscale * u) - 1 with id the 1-based
 // identity (src/compositions.jl:45: Id .+ u, then pull's phi - 1).
+// ... from the deformation's three values p0, p1, p2 at the sample (not the affine map).
+__device__ __forceinline__ void coords_from(const KArgs& a, float p0, float p1, float p2, int i, int j, int k,
+                                            float& x, float& y, float& z) {
+  if (a.disp) {
+    x = ((float)(i + 1) + a.dscale * p0) - 1.0f;
+    y = ((float)(j + 1) + a.dscale * p1) - 1.0f;
+    z = ((float)(k + 1) + a.dscale * p2) - 1.0f;
+    return;
+  }
+  x = p0 - 1.0f;
+  y = p1 - 1.0f;
+  z = p2 - 1.0f;
+}
+
 __device__ __forceinline__ void sample_coords_at(const KArgs& a, const float* phi, int v, int i, int j, int k,
                                                  float& x, float& y, float& z) {
   if (a.affine) {
     affine_x(a.aff, i, j, k, x, y, z);
     return;
   }
-  if (a.disp) {
-    x = ((float)(i + 1) + a.dscale * phi[v]) - 1.0f;
-    y = ((float)(j + 1) + a.dscale * phi[v + a.n1]) - 1.0f;
-    z = ((float)(k + 1) + a.dscale * phi[v + 2 * (long long)a.n1]) - 1.0f;
-    return;
-  }
-  x = phi[v] - 1.0f;
-  y = phi[v + a.n1] - 1.0f;
-  z = phi[v + 2 * (long long)a.n1] - 1.0f;
+  coords_from(a, phi[v], phi[v + a.n1], phi[v + 2 * (long long)a.n1], i, j, k, x, y, z);
 }
 
 __device__ __forceinline__ void sample_coords(const KArgs& a, const float* phi, int v, float& x, float& y, float& z) {

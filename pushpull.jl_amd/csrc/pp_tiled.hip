@@ -174,6 +174,20 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
     if (in) V.inside |= 1u << q;
   }
   pre(V);
+  // Every slot's deformation values are loaded before any is used (a slot outside the
+  // grid reads voxel 0, a valid address, and is masked below): loads under a per-slot
+  // condition would each be waited for before the next slot's were issued, one memory
+  // round trip per slot.
+  float px[VPT], py[VPT], pz[VPT];
+  if (!a.affine) {
+    const long long n1 = a.n1;
+#pragma unroll
+    for (int q = 0; q < VPT; ++q) {
+      px[q] = phi[V.v[q]];
+      py[q] = phi[V.v[q] + n1];
+      pz[q] = phi[V.v[q] + 2 * n1];
+    }
+  }
 #pragma unroll
   for (int q = 0; q < VPT; ++q) {
     const int l = tid + kTB * q;
@@ -185,7 +199,8 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
     float x = 0.f, y = 0.f, z = 0.f;
     bool ok = false;
     if ((V.inside >> q) & 1u) {
-      sample_coords_at(a, phi, V.v[q], gx, gy, gz, x, y, z);
+      if (a.affine) affine_x(a.aff, gx, gy, gz, x, y, z);
+      else coords_from(a, px[q], py[q], pz[q], gx, gy, gz, x, y, z);
       ok = voxok(x, y, z, a.ext, a.lx, a.ly, a.lz);
     }
     V.x[q] = x;
@@ -588,11 +603,23 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, int 
   float g[NG][VPT];
   const float* src0 = a.f + bb * a.f_bs;
   const bool read0 = GRAD || (a.vmode == 0 && a.nchan > 0);
-  auto prefetch = [&](const Vox<VPT>& W) {
+  auto prefetch = [&](const Vox<VPT>& W) {  // (unconditional loads, masked after: see tile_setup)
+    float t[NG][VPT];
+    if (read0) {
+#pragma unroll
+      for (int q = 0; q < VPT; ++q)
+#pragma unroll
+        for (int e = 0; e < NG; ++e) t[e][q] = src0[W.v[q] + e * n1];
+    } else {
+#pragma unroll
+      for (int q = 0; q < VPT; ++q)
+#pragma unroll
+        for (int e = 0; e < NG; ++e) t[e][q] = 0.f;
+    }
 #pragma unroll
     for (int q = 0; q < VPT; ++q)
 #pragma unroll
-      for (int e = 0; e < NG; ++e) g[e][q] = (read0 && ((W.inside >> q) & 1u)) ? src0[W.v[q] + e * n1] : 0.f;
+      for (int e = 0; e < NG; ++e) g[e][q] = ((W.inside >> q) & 1u) ? t[e][q] : 0.f;
   };
   // channel 0's scale bound: the largest finite per-sample bound over all samples in the
   // grid (a superset of the in-window ones, so a valid bound), reduced with the box
@@ -629,11 +656,10 @@ __device__ __forceinline__ void push_tile(const KArgs& a, int tile, int bb, int 
         tile_pos<TX, TY, TZ>(l, px, py, pz);
         const int id = ax == 0 ? S.org[0] + px : ax == 1 ? S.org[1] + py : S.org[2] + pz;
 #pragma unroll
-        for (int e = 0; e < NG; ++e)
-          g[e][q] = !((V.valid >> q) & 1u) ? 0.f
-                    : vmode == 0           ? src[V.v[q] + e * n1]
-                    : vmode == 1           ? 1.0f
-                                           : (float)(id + 1);
+        for (int e = 0; e < NG; ++e) {
+          const float t = vmode == 0 ? src[V.v[q] + e * n1] : vmode == 1 ? 1.0f : (float)(id + 1);
+          g[e][q] = ((V.valid >> q) & 1u) ? t : 0.f;
+        }
       }
     }
     unsigned use = lds ? V.inl : 0u;  // samples accumulated in LDS for this channel
@@ -888,10 +914,12 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, in
   constexpr bool RED = AUX || KIND == 6;         // writes dphi, reduced over the channels
   const float* phi = a.phi + bb * a.phi_bs;
   float auxv[VPT];  // upstream gradient / target; channel 0's is loaded before phi is read
-  auto prefetch = [&](const Vox<VPT>& W) {
+  auto prefetch = [&](const Vox<VPT>& W) {  // (unconditional loads, masked after: see tile_setup)
 #pragma unroll
-    for (int q = 0; q < VPT; ++q)
-      auxv[q] = (AUX && ((W.inside >> q) & 1u)) ? a.aux[bb * a.aux_bs + W.v[q]] : 0.f;
+    for (int q = 0; q < VPT; ++q) {
+      const float t = AUX ? a.aux[bb * a.aux_bs + W.v[q]] : 0.f;
+      auxv[q] = ((W.inside >> q) & 1u) ? t : 0.f;
+    }
   };
   Vox<VPT> V;
   tile_setup<DP, TX, TY, TZ, true, KIND == 2 || KIND == 3>(a, phi, tile, cap, V, S, prefetch);
@@ -913,10 +941,13 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, in
     // KIND 2/3: in-window samples with a finite weight, scattered via LDS; their weight
     // (the residual, which replaces the target in auxv, or the upstream gradient) is auxv
     unsigned rfin = 0u;
+    if (AUX && c > 0) {  // loaded before staging so both overlap (unconditional loads, masked after)
 #pragma unroll
-    for (int q = 0; q < VPT; ++q)
-      if (c > 0)  // loaded before staging so both overlap
-        auxv[q] = (AUX && ((V.valid >> q) & 1u)) ? a.aux[bb * a.aux_bs + c * a.aux_cs + V.v[q]] : 0.f;
+      for (int q = 0; q < VPT; ++q) {
+        const float t = a.aux[bb * a.aux_bs + c * a.aux_cs + V.v[q]];
+        auxv[q] = ((V.valid >> q) & 1u) ? t : 0.f;
+      }
+    }
     if (lds) {
       stage_box(box, S, src, a, chan_signs(a, c));
       __syncthreads();
