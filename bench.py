@@ -49,10 +49,44 @@ TILED_NAMES = {"pull": "k_gather_tile<pull>", "push": "k_push_tile", "backward":
 SAMPLE_NAMES = {"pull": "k_pull", "push": "k_push", "backward": "k_pull_bwd", "vjp": "k_pull_vjp"}
 
 
-def kernel_name(op, deg, tiling):
-    """Kernel that runs `op` at `deg` under the library's PP_OPT_TILING choice."""
-    per_sample = tiling == 0 or (tiling == 1 and deg == 1 and op in ("pull", "backward"))
+def kernel_name(op, deg, tiling, tuned=None):
+    """Kernel that runs `op` at `deg` under the library's PP_OPT_TILING choice (and the
+    autotuner's, when it chose for this op and degree)."""
+    t = (tuned or {}).get(f"{TUNE_OPS.get(op, op)} deg{deg}")
+    if tiling == 1 and t is not None:
+        per_sample = t["kernel"] == "per-sample"
+    else:
+        per_sample = tiling == 0 or (tiling == 1 and deg == 1 and op in ("pull", "backward"))
     return (SAMPLE_NAMES if per_sample else TILED_NAMES)[op]
+
+
+# bench op names -> autotuner op names
+TUNE_OPS = {"pull": "pull", "push": "push", "backward": "pull_backward", "vjp": "pull_vjp"}
+
+
+def autotune(torch, pp, f, phi, delta, schedule):
+    """Let the library's autotuner (pp_tune) choose, for every op and degree the timed step
+    launches, among the per-sample kernel and the tiled kernels at each tile shape and LDS
+    box budget, timing each candidate on these very buffers. Returns the choices."""
+    from pushpull import tune as pptune
+
+    names = ("pull", "vjp") if schedule == "vjp" else ("pull", "push", "backward")
+    out = {}
+    for deg in DEGS:
+        sett = pp.Settings(deg, 1, 1)
+        for name in names:
+            op = TUNE_OPS[name]
+            if op == "push":
+                pptune.tune(op, delta, phi, sett, dim=(N, N, N), reps=5)
+            elif op in ("pull_backward", "pull_vjp"):
+                pptune.tune(op, f, phi, sett, aux=delta, reps=5)
+            else:
+                pptune.tune(op, f, phi, sett, reps=5)
+            cands = pptune.results()
+            best = min(cands, key=lambda r: r["ms"])
+            out[f"{op} deg{deg}"] = {"kernel": best["name"], "box_budget": best["cap"] or "default",
+                                     "ms": round(best["ms"], 4), "candidates": len(cands)}
+    return out
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_256cubed.txt")
 
 
@@ -713,6 +747,8 @@ def main():
     ap.add_argument("--shape", type=int, default=0, help="tile shape index (0: 16x8x8, 1: 32x8x4, 2: 16x16x4)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched-uneven and matching-loop legs")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner; use the library's built-in kernel choice")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run k_fill and k_axpy over 64 Mi floats (reference kernels for HBM counter scaling)")
     ap.add_argument("--schedule", choices=["vjp", "overlap", "serial"], default="vjp",
@@ -844,6 +880,10 @@ def main():
             self.rows.append((deg, m))
 
     timed_step = {"vjp": step_vjp, "overlap": step_overlap, "serial": step}[args.schedule]
+    tuned = None
+    if not args.no_tune and args.tiling == 1:
+        _progress("autotuning")
+        tuned = autotune(torch, pp, f, phi, delta, args.schedule)
     _progress("warm-up")
     for _ in range(args.warmup):
         timed_step()
@@ -935,16 +975,18 @@ def main():
         "data": "synthetic (smoothed Gaussian noise image and upstream gradient; identity + smooth displacement, max 6 voxels)",
         "config": {"workload": "pull+push+backward, 256^3 x 1 channel, degrees 1,2,3 (one triple each per step), "
                                "reflect boundary, ext=1", "volume": [N, N, N], "channels": 1, "degrees": list(DEGS),
-                   "kernels": {0: "per-sample", 1: "automatic", 2: "tiled"}[args.tiling], "tile_shape": args.shape,
+                   "kernels": {0: "per-sample", 1: "automatic", 2: "tiled"}[args.tiling] +
+                              (" (autotuned during warm-up)" if tuned else ""),
+                   "tile_shape": args.shape, "autotuned": tuned,
                    "schedule": {"vjp": "per degree, pull and one pp_pull_vjp launch (push and pullback); the six "
                                        "launches on three streams: VJP 3 | VJP 2, pull 1 | VJP 1, pull 2, pull 3; "
                                        "joined per step",
                                 "overlap": "pull and push on the launch stream, pullback on a second stream, "
                                            "joined per step",
                                 "serial": "nine launches in order on one stream"}[args.schedule]},
-        "roofline": {"bound": "hbm", "kernel": f"{kernel_name(op, deg, args.tiling)} deg{deg}", "achieved": round(gbs, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
-                     "traffic": _traffic(kernel_name(op, deg, args.tiling), deg), "bytes_per_launch": BYTES[op] * n1,
+        "roofline": {"bound": "hbm", "kernel": f"{kernel_name(op, deg, args.tiling, tuned)} deg{deg}",
+                     "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                     "traffic": _traffic(kernel_name(op, deg, args.tiling, tuned), deg), "bytes_per_launch": BYTES[op] * n1,
                      "avg_ms": round(ms, 4), "op_ms_with_fill": round(ms_op, 4)},
         "serial_ms_per_step": round(ms_serial, 4),
         "roofline_on_chip": None,
