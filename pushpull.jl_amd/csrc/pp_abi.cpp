@@ -796,6 +796,13 @@ struct pp_batch_plan {
   int nvol;
   std::vector<pp_batch_group> groups;  // volumes ordered by degree, one launch per group
   pp::BatchVol* dvols;
+  // Degree groups after the first run beside it: one stream each (default priority),
+  // forked from and joined back to the caller's stream with events, so that the groups'
+  // workgroups share the device and a short group fills the tail of a long one. Made
+  // once with the plan; graph capture records the fork and join as graph edges.
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> join;
+  hipEvent_t fork = nullptr;
 };
 
 namespace {
@@ -947,18 +954,55 @@ int pp_batch_plan_create(int op, const pp_volume* vols, int nvol, pp_batch_plan*
     (void)hipFree(d);
     return hip_fail(e, "upload batch descriptors");
   }
-  *plan = new pp_batch_plan{static_cast<pp::Op>(op), nvol, std::vector<pp_batch_group>(g, g + ng), d};
+  pp_batch_plan* p = new pp_batch_plan{static_cast<pp::Op>(op), nvol, std::vector<pp_batch_group>(g, g + ng), d};
+  // side streams and events for the degree groups after the first (default priority)
+  bool made = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming) == hipSuccess;
+  for (int gi = 1; gi < ng && made; ++gi) {
+    hipStream_t ss = nullptr;
+    hipEvent_t ev = nullptr;
+    made = hipStreamCreateWithFlags(&ss, hipStreamNonBlocking) == hipSuccess;
+    if (made) p->side.push_back(ss);
+    made = made && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
+    if (made) p->join.push_back(ev);
+  }
+  if (!made) {
+    pp_batch_plan_destroy(p);
+    return fail(PP_EDEVICE, "could not create the batch plan's side streams");
+  }
+  *plan = p;
   return ok();
 }
 
 int pp_batch_run(const pp_batch_plan* plan, void* s) {
   if (!plan) return fail(PP_EINVAL, "null plan");
-  const hipError_t e = run_batch(plan->op, plan->dvols, plan->nvol, plan->groups.data(), (int)plan->groups.size(), S(s));
+  const int ng = (int)plan->groups.size();
+  if (ng <= 1 || plan->side.size() + 1 < (size_t)ng) {
+    const hipError_t e = run_batch(plan->op, plan->dvols, plan->nvol, plan->groups.data(), ng, S(s));
+    return e == hipSuccess ? ok() : hip_fail(e, "batch launch");
+  }
+  // zero fill, then fork: group 0 on the caller's stream, group g > 0 on side stream g - 1
+  hipError_t e = pp::launch_batch_zero(plan->op, plan->dvols, plan->nvol, S(s));
+  if (e == hipSuccess) e = hipEventRecord(plan->fork, S(s));
+  for (int gi = 1; gi < ng && e == hipSuccess; ++gi) {
+    const pp_batch_group& g = plan->groups[gi];
+    hipStream_t ss = plan->side[gi - 1];
+    e = hipStreamWaitEvent(ss, plan->fork, 0);
+    if (e == hipSuccess) e = pp::launch_batch_group(plan->op, g.dp, plan->dvols + g.first, g.nvol, g.split, ss);
+    if (e == hipSuccess) e = hipEventRecord(plan->join[gi - 1], ss);
+  }
+  if (e == hipSuccess) {
+    const pp_batch_group& g = plan->groups[0];
+    e = pp::launch_batch_group(plan->op, g.dp, plan->dvols + g.first, g.nvol, g.split, S(s));
+  }
+  for (int gi = 1; gi < ng && e == hipSuccess; ++gi) e = hipStreamWaitEvent(S(s), plan->join[gi - 1], 0);
   return e == hipSuccess ? ok() : hip_fail(e, "batch launch");
 }
 
 int pp_batch_plan_destroy(pp_batch_plan* plan) {
   if (!plan) return ok();
+  for (hipStream_t ss : plan->side) (void)hipStreamDestroy(ss);
+  for (hipEvent_t ev : plan->join) (void)hipEventDestroy(ev);
+  if (plan->fork) (void)hipEventDestroy(plan->fork);
   hipError_t e = hipFree(plan->dvols);
   delete plan;
   return e == hipSuccess ? ok() : hip_fail(e, "hipFree");
