@@ -115,6 +115,10 @@ struct Vox {
   }
 };
 
+// A workgroup-uniform value read from LDS, moved to a scalar register: what is computed
+// from it (the box sizing) then runs on the scalar unit instead of in every lane.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 // Wave-wide reductions (DPP lane moves, not LDS-routed permutes).
 __device__ __forceinline__ int wave_min(int v) { return __ockl_wfred_min_i32(v); }
 __device__ __forceinline__ int wave_max(int v) { return __ockl_wfred_max_i32(v); }
@@ -223,7 +227,7 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
     }
   }
   __syncthreads();
-  const int cx = S.ctr[0], cy = S.ctr[1], cz = S.ctr[2];
+  const int cx = uni(S.ctr[0]), cy = uni(S.ctr[1]), cz = uni(S.ctr[2]);
   int mn0 = INT_MAX, mn1 = INT_MAX, mn2 = INT_MAX, mx0 = INT_MIN, mx1 = INT_MIN, mx2 = INT_MIN;
 #pragma unroll
   for (int q = 0; q < VPT; ++q) {
@@ -259,8 +263,8 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
   for (int w = 0; w < kTB / 64; ++w)
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      lo[q] = min(lo[q], S.red[q][w]);
-      hi[q] = max(hi[q], S.red[3 + q][w]);
+      lo[q] = min(lo[q], uni(S.red[q][w]));
+      hi[q] = max(hi[q], uni(S.red[3 + q][w]));
     }
   bool any = lo[0] <= hi[0];  // some in-window sample
   // LDS layouts: cell (i, j, k) at i + P j + Q k with P = bx. Plane pitches are padded to
@@ -379,11 +383,11 @@ __device__ __forceinline__ bool cell_neg(int cellword, const RowWalk& w, unsigne
   return fy != fz != fx;
 }
 
-// Stage the box from image channel `src` (fold + boundary sign applied here).
-__device__ __forceinline__ void stage_box(float* box, const TileState& S, const float* __restrict__ src,
-                                          const KArgs& a, unsigned signs) {
-  RowWalk w = row_walk(S, a);
-  if (!w.active) return;
+// Stage the box from image channel `src` (fold + boundary sign applied here). FLIPS: some
+// axis folds with code 2 (the only code whose mirrored copies change sign).
+template <bool FLIPS>
+__device__ __forceinline__ void stage_rows(float* box, const TileState& S, const float* __restrict__ src,
+                                           unsigned signs, RowWalk w) {
   const RowEnt* rt = reinterpret_cast<const RowEnt*>(pp_box + S.tab);
   const int rows = S.rows;
   for (; w.r < rows; w.r += kBatch * w.rp) {
@@ -398,8 +402,17 @@ __device__ __forceinline__ void stage_box(float* box, const TileState& S, const 
     for (int u = 0; u < kBatch; ++u) v[u] = w.r + u * w.rp < rows ? src[e[u].goff + w.fx] : 0.f;
 #pragma unroll
     for (int u = 0; u < kBatch; ++u)
-      if (w.r + u * w.rp < rows) box[(e[u].cell & kCellMask) + w.c] = cell_neg(e[u].cell, w, signs) ? -v[u] : v[u];
+      if (w.r + u * w.rp < rows)
+        box[(e[u].cell & kCellMask) + w.c] = (FLIPS && cell_neg(e[u].cell, w, signs)) ? -v[u] : v[u];
   }
+}
+
+__device__ __forceinline__ void stage_box(float* box, const TileState& S, const float* __restrict__ src,
+                                          const KArgs& a, unsigned signs) {
+  RowWalk w = row_walk(S, a);
+  if (!w.active) return;
+  if (a.bx == 2 || a.by == 2 || a.bz == 2) stage_rows<true>(box, S, src, signs, w);
+  else stage_rows<false>(box, S, src, signs, w);
 }
 
 // ---- fixed-point scatter accumulation ---------------------------------------------
@@ -464,11 +477,10 @@ __device__ __forceinline__ float block_max(float v, float* wsum) {
 }
 
 // Flush the fixed-point box into image channel `dst`: one folded global float atomic per
-// non-zero cell.
-__device__ __forceinline__ void flush_acc(const acc_t* box, const TileState& S, double mul, float* __restrict__ dst,
-                                          const KArgs& a, unsigned signs) {
-  RowWalk w = row_walk(S, a);
-  if (!w.active) return;
+// non-zero cell. FLIPS as for staging.
+template <bool FLIPS>
+__device__ __forceinline__ void flush_rows(const acc_t* box, const TileState& S, double mul, float* __restrict__ dst,
+                                           unsigned signs, RowWalk w) {
   const RowEnt* rt = reinterpret_cast<const RowEnt*>(pp_box + S.tab);
   const int rows = S.rows;
   const double inv = 1.0 / mul;  // (exact: mul is a power of two)
@@ -487,9 +499,17 @@ __device__ __forceinline__ void flush_acc(const acc_t* box, const TileState& S, 
     for (int u = 0; u < kFB; ++u)
       if (q[u] != 0) {
         const float v = fixed_to_float(q[u], inv);
-        unsafeAtomicAdd(dst + e[u].goff + w.fx, cell_neg(e[u].cell, w, signs) ? -v : v);
+        unsafeAtomicAdd(dst + e[u].goff + w.fx, (FLIPS && cell_neg(e[u].cell, w, signs)) ? -v : v);
       }
   }
+}
+
+__device__ __forceinline__ void flush_acc(const acc_t* box, const TileState& S, double mul, float* __restrict__ dst,
+                                          const KArgs& a, unsigned signs) {
+  RowWalk w = row_walk(S, a);
+  if (!w.active) return;
+  if (a.bx == 2 || a.by == 2 || a.bz == 2) flush_rows<true>(box, S, mul, dst, signs, w);
+  else flush_rows<false>(box, S, mul, dst, signs, w);
 }
 
 // Unsigned, unfolded weights of one in-window sample and its box offset
