@@ -40,9 +40,9 @@ struct TileState {
   int cells_f;    // floats of the staged image: Qf * b[2]
   int cells_c;    // 64-bit cells of the accumulator: Qc * b[2]
   int lds;        // 1 when the box is used
-  int ctr[3];     // window centre (knot of the tile's centre sample)
+  float ctr[3];   // window centre: the tile's centre sample (else the tile centre, mapped)
   int org[3];     // tile origin in the phi grid
-  int red[6][4];  // per-wave partial min/max
+  float red[6][4];  // per-wave partial min/max of the in-window coordinates
   float redf[4];  // per-wave partial bound (tile_setup's `bound` hook)
   int rows;       // box rows: b[1] * b[2]
   int tab;        // float offset, in the launch's box memory, of the row table (RowEnt[rows])
@@ -118,18 +118,19 @@ struct Vox {
 // A workgroup-uniform value read from LDS, moved to a scalar register: what is computed
 // from it (the box sizing) then runs on the scalar unit instead of in every lane.
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uni(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
 
 // Wave-wide reductions (DPP lane moves, not LDS-routed permutes).
-__device__ __forceinline__ int wave_min(int v) { return __ockl_wfred_min_i32(v); }
-__device__ __forceinline__ int wave_max(int v) { return __ockl_wfred_max_i32(v); }
+__device__ __forceinline__ float wave_min(float v) { return __ockl_wfred_min_f32(v); }
+__device__ __forceinline__ float wave_max(float v) { return __ockl_wfred_max_f32(v); }
+
+// A window centre at most this far from the origin keeps every in-window coordinate
+// within kFastLimit (the range of knot_fast and the box weights).
+constexpr float kCentreLimit = 0.5f * kFastLimit;
 
 template <int DP>
 __device__ __forceinline__ int knot_fast(float x) {
   return (int)ceilf(x - 0.5f * (float)DP);
-}
-
-__device__ __forceinline__ bool is_fast3(float x, float y, float z) {
-  return fabsf(x) < kFastLimit && fabsf(y) < kFastLimit && fabsf(z) < kFastLimit;
 }
 
 typedef unsigned long long acc_t;  // fixed-point LDS accumulator cell (scatters)
@@ -211,33 +212,38 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
     V.y[q] = y;
     V.z[q] = z;
     if (ok) V.valid |= 1u << q;
-    if (l == SH::LC) {  // window centre: the centre sample's knot, else the tile centre
-      S.org[0] = x0;    // mapped by the grid-size ratio
+    if (l == SH::LC) {  // window centre: the centre sample, else the tile centre mapped by
+      S.org[0] = x0;    // the grid-size ratio
       S.org[1] = y0;
       S.org[2] = z0;
-      if (ok && is_fast3(x, y, z)) {
-        S.ctr[0] = knot_fast<DP>(x);
-        S.ctr[1] = knot_fast<DP>(y);
-        S.ctr[2] = knot_fast<DP>(z);
-      } else {  // (double: exact for grids below 2^26 per axis, and no 64-bit division)
-        S.ctr[0] = (int)((double)(x0 + TX / 2) * a.d0x / a.d1x) - (DP - 1) / 2;
-        S.ctr[1] = (int)((double)(y0 + TY / 2) * a.d0y / a.d1y) - (DP - 1) / 2;
-        S.ctr[2] = (int)((double)(z0 + TZ / 2) * a.d0z / a.d1z) - (DP - 1) / 2;
+      if (ok && fabsf(x) < kCentreLimit && fabsf(y) < kCentreLimit && fabsf(z) < kCentreLimit) {
+        S.ctr[0] = x;
+        S.ctr[1] = y;
+        S.ctr[2] = z;
+      } else {  // (double: no 64-bit division)
+        S.ctr[0] = fminf((float)((double)(x0 + TX / 2) * a.d0x / a.d1x), kCentreLimit);
+        S.ctr[1] = fminf((float)((double)(y0 + TY / 2) * a.d0y / a.d1y), kCentreLimit);
+        S.ctr[2] = fminf((float)((double)(z0 + TZ / 2) * a.d0z / a.d1z), kCentreLimit);
       }
     }
   }
   __syncthreads();
-  const int cx = uni(S.ctr[0]), cy = uni(S.ctr[1]), cz = uni(S.ctr[2]);
-  int mn0 = INT_MAX, mn1 = INT_MAX, mn2 = INT_MAX, mx0 = INT_MIN, mx1 = INT_MIN, mx2 = INT_MIN;
+  // In-window samples: valid, and within the half-widths of the centre on every axis (a
+  // NaN coordinate fails the test). The box spans the knots of their extreme coordinates
+  // (the knot is monotonic in the coordinate), so the extremes are reduced as floats and
+  // no sample's knots are computed here.
+  const float cx = uni(S.ctr[0]), cy = uni(S.ctr[1]), cz = uni(S.ctr[2]);
+  float mn0 = INFINITY, mn1 = INFINITY, mn2 = INFINITY, mx0 = -INFINITY, mx1 = -INFINITY, mx2 = -INFINITY;
 #pragma unroll
   for (int q = 0; q < VPT; ++q) {
-    if (!((V.valid >> q) & 1u) || !is_fast3(V.x[q], V.y[q], V.z[q])) continue;
-    const int ox = knot_fast<DP>(V.x[q]), oy = knot_fast<DP>(V.y[q]), oz = knot_fast<DP>(V.z[q]);
-    if (abs(ox - cx) > SH::WX || abs(oy - cy) > SH::WY || abs(oz - cz) > SH::WZ) continue;
+    const float x = V.x[q], y = V.y[q], z = V.z[q];
+    if (!((V.valid >> q) & 1u) || !(fabsf(x - cx) <= (float)SH::WX) || !(fabsf(y - cy) <= (float)SH::WY) ||
+        !(fabsf(z - cz) <= (float)SH::WZ))
+      continue;
     V.inl |= 1u << q;
-    mn0 = min(mn0, ox); mx0 = max(mx0, ox);
-    mn1 = min(mn1, oy); mx1 = max(mx1, oy);
-    mn2 = min(mn2, oz); mx2 = max(mx2, oz);
+    mn0 = fminf(mn0, x); mx0 = fmaxf(mx0, x);
+    mn1 = fminf(mn1, y); mx1 = fmaxf(mx1, y);
+    mn2 = fminf(mn2, z); mx2 = fmaxf(mx2, z);
   }
   mn0 = wave_min(mn0); mn1 = wave_min(mn1); mn2 = wave_min(mn2);
   mx0 = wave_max(mx0); mx1 = wave_max(mx1); mx2 = wave_max(mx2);
@@ -258,15 +264,21 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
     *block_bound = m;
   }
   // every thread sizes the box (a broadcast read of the partials)
-  int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+  float flo[3] = {INFINITY, INFINITY, INFINITY}, fhi[3] = {-INFINITY, -INFINITY, -INFINITY};
 #pragma unroll
   for (int w = 0; w < kTB / 64; ++w)
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      lo[q] = min(lo[q], uni(S.red[q][w]));
-      hi[q] = max(hi[q], uni(S.red[3 + q][w]));
+      flo[q] = fminf(flo[q], uni(S.red[q][w]));
+      fhi[q] = fmaxf(fhi[q], uni(S.red[3 + q][w]));
     }
-  bool any = lo[0] <= hi[0];  // some in-window sample
+  bool any = flo[0] <= fhi[0];  // some in-window sample
+  int lo[3], hi[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    lo[q] = any ? knot_fast<DP>(flo[q]) : 0;
+    hi[q] = any ? knot_fast<DP>(fhi[q]) : 0;
+  }
   // LDS layouts: cell (i, j, k) at i + P j + Q k with P = bx. Plane pitches are padded to
   // Q = TX (mod 2 TX), so that the TX x-lanes on 64 / TX planes of a wave (tile_pos) fall
   // on distinct banks.
@@ -281,7 +293,7 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
   // to the budget sends the whole tile there.)
   bool clipped = false;
   if (any && !fits(hi[0] - lo[0] + DP, hi[1] - lo[1] + DP, hi[2] - lo[2] + DP)) {
-    const int c0[3] = {S.ctr[0], S.ctr[1], S.ctr[2]};
+    const int c0[3] = {knot_fast<DP>(cx), knot_fast<DP>(cy), knot_fast<DP>(cz)};
     const int span[3] = {TX, TY, TZ};
     while (!fits(hi[0] - lo[0] + DP, hi[1] - lo[1] + DP, hi[2] - lo[2] + DP)) {
       int ax = -1;
@@ -393,11 +405,10 @@ __device__ __forceinline__ void stage_rows(float* box, const TileState& S, const
   for (; w.r < rows; w.r += kBatch * w.rp) {
     RowEnt e[kBatch];
     float v[kBatch];
+    // table entries are read unconditionally, rows past the end reading the last row's
+    // (a read under a per-row condition is waited for before the next one is issued)
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) {
-      const int r = w.r + u * w.rp;
-      e[u] = r < rows ? rt[r] : RowEnt{0, 0};
-    }
+    for (int u = 0; u < kBatch; ++u) e[u] = rt[min(w.r + u * w.rp, rows - 1)];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) v[u] = w.r + u * w.rp < rows ? src[e[u].goff + w.fx] : 0.f;
 #pragma unroll
@@ -450,13 +461,9 @@ __device__ __forceinline__ acc_t fix2(double a, double b, double c, double d) {
   return (acc_t)(__double_as_longlong(fma(a, b, fma(c, d, kMagic))) - __double_as_longlong(kMagic));
 }
 
-// q * inv rounded to float: below 2^51 in magnitude q converts exactly through the same
-// magic constant (as fix1, in reverse); larger sums take the general conversion.
+// q * inv rounded to float. The int64 -> double conversion (high word scaled by 2^32 plus
+// the low word, one rounding) is exact below 2^53 in magnitude and takes no branch.
 __device__ __forceinline__ float fixed_to_float(long long q, double inv) {
-  if (q > -(1LL << 51) && q < (1LL << 51)) {
-    const double d = __longlong_as_double(q + __double_as_longlong(kMagic)) - kMagic;
-    return (float)(d * inv);
-  }
   return (float)((double)q * inv);
 }
 
@@ -488,16 +495,15 @@ __device__ __forceinline__ void flush_rows(const acc_t* box, const TileState& S,
   for (; w.r < rows; w.r += kFB * w.rp) {
     RowEnt e[kFB];
     long long q[kFB];
+    // table entries and cells read unconditionally, as in staging (rows past the end read
+    // the last row's, a valid cell, and are masked below)
 #pragma unroll
-    for (int u = 0; u < kFB; ++u) {
-      const int r = w.r + u * w.rp;
-      e[u] = r < rows ? rt[r] : RowEnt{0, 0};
-    }
+    for (int u = 0; u < kFB; ++u) e[u] = rt[min(w.r + u * w.rp, rows - 1)];
 #pragma unroll
-    for (int u = 0; u < kFB; ++u) q[u] = w.r + u * w.rp < rows ? (long long)box[(e[u].cell & kCellMask) + w.c] : 0;
+    for (int u = 0; u < kFB; ++u) q[u] = (long long)box[(e[u].cell & kCellMask) + w.c];
 #pragma unroll
     for (int u = 0; u < kFB; ++u)
-      if (q[u] != 0) {
+      if (w.r + u * w.rp < rows && q[u] != 0) {
         const float v = fixed_to_float(q[u], inv);
         unsafeAtomicAdd(dst + e[u].goff + w.fx, (FLIPS && cell_neg(e[u].cell, w, signs)) ? -v : v);
       }
