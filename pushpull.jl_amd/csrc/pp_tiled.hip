@@ -40,6 +40,7 @@ struct TileState {
   int cells_f;    // floats of the staged image: Qf * b[2]
   int cells_c;    // 64-bit cells of the accumulator: Qc * b[2]
   int lds;        // 1 when the box is used
+  int vec;        // 1 when the staged rows move in 16-byte chunks (see tile_setup)
   float ctr[3];   // window centre: the tile's centre sample (else the tile centre, mapped)
   int org[3];     // tile origin in the phi grid
   float red[6][4];  // per-wave partial min/max of the in-window coordinates
@@ -283,8 +284,8 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
   // Q = TX (mod 2 TX), so that the TX x-lanes on 64 / TX planes of a wave (tile_pos) fall
   // on distinct banks.
   auto pitch = [](int P, int by) { return P * by + ((TX - P * by) & (2 * TX - 1)); };
-  auto fits = [&](int bx, int by, int bz) {  // box cells and the row table (two floats a row)
-    const int q = pitch(bx, by) * bz, t = 2 * by * bz;
+  auto fits = [&](int P, int by, int bz) {  // box cells (row pitch P) and the row table (two floats a row)
+    const int q = pitch(P, by) * bz, t = 2 * by * bz;
     return (!GF || q + t <= cap) && (!GC || 2 * q + t <= cap);
   };
   // A box beyond the budget is clipped, one knot plane at a time on the side farther from
@@ -321,8 +322,22 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
       if (ox < lo[0] || ox > hi[0] || oy < lo[1] || oy > hi[1] || oz < lo[2] || oz > hi[2]) V.inl &= ~(1u << q);
     }
   }
+  // Staging in 16-byte chunks: when the box lies inside the image along x (no x fold) and
+  // image rows start on 16-byte boundaries (d0x, the channel and batch strides multiples of
+  // four floats, an aligned base), the box is extended on its low x side to a multiple of
+  // four (at most three columns, never read by a sample) and its row pitch padded to a
+  // multiple of four, so that one 16-byte load and one 16-byte LDS store move four cells.
+  bool vec = false;
+  if (GF && any && lo[0] >= 0 && hi[0] + DP <= a.d0x && (a.d0x & 3) == 0 && ((a.f_cs | a.f_bs) & 3) == 0 &&
+      (reinterpret_cast<unsigned long long>(a.f) & 15) == 0) {
+    const int l0 = lo[0] & ~3;
+    if (fits((hi[0] + DP - l0 + 3) & ~3, hi[1] - lo[1] + DP, hi[2] - lo[2] + DP)) {
+      vec = true;
+      lo[0] = l0;
+    }
+  }
   const int bx = hi[0] - lo[0] + DP, by = hi[1] - lo[1] + DP, bz = hi[2] - lo[2] + DP;
-  const int Pf = bx, Pc = bx, Qf = pitch(bx, by), Qc = Qf;
+  const int Pf = vec ? (bx + 3) & ~3 : bx, Pc = Pf, Qf = pitch(Pf, by), Qc = Qf;
   const int cells_f = any ? Qf * bz : 0, cells_c = any ? Qc * bz : 0;
   const bool use = any;
   const int rows = by * bz, tab = (cap & ~1) - 2 * rows;
@@ -339,6 +354,7 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
     S.cells_f = cells_f;
     S.cells_c = cells_c;
     S.lds = use;
+    S.vec = vec;
     S.rows = rows;
     S.tab = tab;
   }
@@ -418,8 +434,51 @@ __device__ __forceinline__ void stage_rows(float* box, const TileState& S, const
   }
 }
 
+// Staging in 16-byte chunks (the box's vec layout: no x fold, rows and the box's x origin
+// aligned to four floats): thread t moves chunk t % nc of rows t / nc, t / nc + rp, ...
+// (nc chunks a row, rp = kTB / nc rows a pass); the chunks past the box's last column
+// fill the row's padding with image cells that exist (the box ends inside the row).
+constexpr int kVBatch = 4;  // rows a thread loads before storing (as many registers as kBatch floats)
+
+template <bool FLIPS>
+__device__ __forceinline__ void stage_rows_vec(float* box, const TileState& S, const float* __restrict__ src,
+                                               const KArgs& a, unsigned signs) {
+  const int nc = uni(S.Pf) >> 2, rows = uni(S.rows);
+  const int c = threadIdx.x % nc, rp = kTB / nc;
+  int r = threadIdx.x / nc;
+  if (r >= rp) return;
+  const RowEnt* rt = reinterpret_cast<const RowEnt*>(pp_box + uni(S.tab));
+  const float* s0 = src + uni(S.lo[0]) + 4 * c;
+  for (; r < rows; r += kVBatch * rp) {
+    RowEnt e[kVBatch];
+    float4 v[kVBatch];
+#pragma unroll
+    for (int u = 0; u < kVBatch; ++u) e[u] = rt[min(r + u * rp, rows - 1)];
+#pragma unroll
+    for (int u = 0; u < kVBatch; ++u)
+      if (r + u * rp < rows) v[u] = *reinterpret_cast<const float4*>(s0 + e[u].goff);
+#pragma unroll
+    for (int u = 0; u < kVBatch; ++u)
+      if (r + u * rp < rows) {
+        float4 t = v[u];
+        if (FLIPS && (((signs & 2u) && (e[u].cell & kFlipY)) != ((signs & 4u) && (e[u].cell & kFlipZ)))) {
+          t.x = -t.x;
+          t.y = -t.y;
+          t.z = -t.z;
+          t.w = -t.w;
+        }
+        *reinterpret_cast<float4*>(box + (e[u].cell & kCellMask) + 4 * c) = t;
+      }
+  }
+}
+
 __device__ __forceinline__ void stage_box(float* box, const TileState& S, const float* __restrict__ src,
                                           const KArgs& a, unsigned signs) {
+  if (uni(S.vec)) {
+    if (a.by == 2 || a.bz == 2) stage_rows_vec<true>(box, S, src, a, signs);
+    else stage_rows_vec<false>(box, S, src, a, signs);
+    return;
+  }
   RowWalk w = row_walk(S, a);
   if (!w.active) return;
   if (a.bx == 2 || a.by == 2 || a.bz == 2) stage_rows<true>(box, S, src, signs, w);
