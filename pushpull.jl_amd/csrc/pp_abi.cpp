@@ -94,6 +94,10 @@ thread_local int t_ncand = 0;
 thread_local Choice t_cand[kMaxCandidates];
 thread_local float t_cand_ms[kMaxCandidates];
 
+// Tuning-table key of a degree triple (dp = degree + 1 per axis): dp when isotropic, else
+// one key per triple.
+int dp_key(int dpx, int dpy, int dpz) { return (dpx == dpy && dpx == dpz) ? dpx : 100 + 36 * dpx + 6 * dpy + dpz; }
+
 int size_class(long long n) {
   int c = 0;
   while ((1LL << (c + 1)) <= n) ++c;
@@ -116,9 +120,8 @@ hipError_t launch(pp::Op op, const pp::KArgs& a, int nbatch, hipStream_t s) {
   if (tiling == 0) return launch_variant(op, a, nbatch, s, 0);
   if (tiling == 2) return launch_variant(op, a, nbatch, s, tiled);
   if (g_tuned.load(std::memory_order_relaxed)) {
-    const bool iso = a.dpx == a.dpy && a.dpx == a.dpz;
     std::lock_guard<std::mutex> lock(g_tune_mu);
-    auto it = g_tune.find({(int)op, iso ? a.dpx : 0, size_class(a.n1)});
+    auto it = g_tune.find({(int)op, dp_key(a.dpx, a.dpy, a.dpz), size_class(a.n1)});
     if (it != g_tune.end()) return launch_variant(op, a, nbatch, s, it->second.variant, it->second.cap);
   }
   // Built-in choice: trilinear gathers (8 taps, served by L1/L2) run per sample, which
@@ -1211,14 +1214,15 @@ int pp_tune(int op, const pp_desc* d, float* out, float* out2, double* loss, con
     }
   };
   // candidates: per sample; each tile shape at the default budget and at each other budget
-  const bool iso = d->deg[0] == d->deg[1] && d->deg[0] == d->deg[2];
-  const int dflt = iso && d->deg[0] >= 1 ? pp::default_cap(static_cast<pp::Op>(op), d->deg[0] + 1) : 0;
+  // (anisotropic degrees run on the tiled kernels of the largest)
+  const int dmax = std::max(d->deg[0], std::max(d->deg[1], d->deg[2]));
+  const int dflt = dmax >= 1 ? pp::default_cap(static_cast<pp::Op>(op), dmax + 1) : 0;
   Choice cand[kMaxCandidates];
   int nc = 0;
   cand[nc++] = Choice{0, 0};
   for (int v = 1; v < kVariants; ++v) {
     cand[nc++] = Choice{v, 0};
-    if (!iso || d->deg[0] < 1) continue;  // no box at degree 0 or anisotropic degrees
+    if (dmax < 1) continue;  // no box at degree 0
     for (int c : kTuneCaps)
       if (c != dflt) cand[nc++] = Choice{v, c};
   }
@@ -1255,7 +1259,7 @@ int pp_tune(int op, const pp_desc* d, float* out, float* out2, double* loss, con
   const int internal = op;  // PP_OP_* codes equal the internal op numbers
   {
     std::lock_guard<std::mutex> lock(g_tune_mu);
-    g_tune[{internal, iso ? d->deg[0] + 1 : 0, size_class(g.n1)}] = cand[best];
+    g_tune[{internal, dp_key(d->deg[0] + 1, d->deg[1] + 1, d->deg[2] + 1), size_class(g.n1)}] = cand[best];
   }
   if (variant) *variant = cand[best].variant;
   return ok();

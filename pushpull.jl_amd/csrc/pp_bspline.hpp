@@ -187,6 +187,7 @@ enum : int { kW = 1, kDW = 2, kHW = 4 };
 // beyond n are never touched by the callers.
 template <int DP>
 struct Axis {
+  static constexpr int kDP = DP;
   static constexpr int NP = DP > 0 ? DP : kMaxDp;
   float w[NP], dw[NP], hw[NP];
   int off[NP];

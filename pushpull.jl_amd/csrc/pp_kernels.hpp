@@ -107,6 +107,9 @@ struct TileCfg {
 // Default LDS box budget (floats) of the tiled kernel of op at isotropic dp.
 int default_cap(Op op, int dp);
 hipError_t launch_tiled(Op op, const KArgs& a, int nbatch, hipStream_t stream, TileCfg cfg);
+// ... for degrees that differ between axes (pp_tiled_aniso.hip): the 16x8x8 kernels of the
+// largest degree, every axis's taps in that window (cfg.shape is not used).
+hipError_t launch_tiled_aniso(Op op, const KArgs& a, int nbatch, hipStream_t stream, TileCfg cfg);
 
 // One volume of a batched launch (device-resident array, built by the batch plan).
 struct BatchVol {

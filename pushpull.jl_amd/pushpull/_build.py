@@ -12,7 +12,7 @@ OBJ = os.path.join(ROOT, "_build")
 LIB = os.path.join(ROOT, "lib", "libpushpull_hip.so")
 INCLUDE = os.path.join(os.path.dirname(ROOT), "include")
 
-SOURCES = ["pp_kernels.hip", "pp_tiled.hip", "pp_sparse.hip", "pp_abi.cpp", "pp_sparse_abi.cpp"]
+SOURCES = ["pp_kernels.hip", "pp_tiled.hip", "pp_tiled_aniso.hip", "pp_sparse.hip", "pp_abi.cpp", "pp_sparse_abi.cpp"]
 
 # Packed FP32 instructions are disabled for every kernel; float atomics map to the
 # hardware global_atomic_add_f32.

@@ -187,6 +187,39 @@ def test_config2_pull_push_adjoint(deg, bnd, path):
     assert abs(lhs64 - rhs64) / (abs(lhs64) + abs(rhs64)) < 1e-12
 
 
+@pytest.mark.parametrize("deg", [1, 2, 3])
+@pytest.mark.parametrize("bnd", [0, 1])
+def test_config2_ext0_nan_and_skip(deg, bnd, path):
+    """192^3 as above with extrapolate=false: the 5% of samples sent outside the field of
+    view pull to NaN (VOXOK, artifacts/C/pushpull.c:6-9, 120) and push skips them
+    (pushpull.c:114); the NaN masks must match the reference exactly, the finite values
+    its FP32 path and the FP64 oracle, and the adjoint identity holds over the valid
+    samples."""
+    rng = np.random.default_rng(400 + 10 * deg + bnd)
+    d = (192, 192, 192)
+    f, phi = _volume_case(rng, d, 2.0, 8.0, out_frac=0.05)
+    g = rng.standard_normal(d).astype(np.float32)
+    s = pp.Settings(deg, bnd, 0)
+    so = S(deg, bnd, 0)
+    tf, tphi, tg = T(f), T(phi), T(g)
+    f1 = H(pp.pull(tf, tphi, s))
+    r32 = oracle.pull(f, phi, so)[0, 0]
+    r64 = oracle.pull(f, phi, so, prec="f64")[0, 0]
+    nan_frac = float(np.isnan(r64).mean())
+    assert 0.04 < nan_frac < 0.2, nan_frac  # the outside samples (and those past the -0.01 / d-0.99 limits)
+    tag = f"config2 ext0 deg{deg} bnd{bnd} {path}"
+    report(f"{tag} pull vs f32 reference", _check(f1, r32, "pull vs f32 ref"))
+    report(f"{tag} pull vs f64", _check(f1, r64, "pull vs f64"))
+    f0 = H(pp.push(tg, tphi, d, s))
+    p64 = oracle.push(g, phi, d, so, prec="f64")[0, 0]
+    report(f"{tag} push vs f64", _check(f0, p64, "push vs f64"))
+    valid = np.isfinite(f1)
+    lhs = float(np.sum(f1[valid].astype(np.float64) * g[valid].astype(np.float64)))
+    rhs = float(np.sum(f.astype(np.float64) * f0.astype(np.float64)))
+    report(f"{tag} adjoint over valid samples", abs(lhs - rhs) / (abs(lhs) + abs(rhs)))
+    assert abs(lhs - rhs) / (abs(lhs) + abs(rhs)) < TOL
+
+
 def test_config1_nan_mask_ext0(path):
     """64^3, U[0,1), trilinear, reflect, ext=0: the NaN mask must match exactly."""
     rng = np.random.default_rng(1)
