@@ -464,11 +464,12 @@ __device__ __forceinline__ void stage_rows_vec(float* box, const TileState& S, c
 // Anisotropic degrees: the box's taps outside an axis's support are weighted 0, and 0
 // times a non-finite value is NaN where the reference, which never reads that tap, has a
 // value. Non-finite staged values are therefore replaced by 0 here (thread t checks column
-// t % P of rows t / P, t / P + kTB / P, ...), and a channel whose box held one takes the
-// global path for all its samples, which reads each axis's own taps only. Returns whether
-// this thread found one.
+// t % P of rows t / P, t / P + kTB / P, ..., P the staged columns: the row pitch in the
+// 16-byte layout, whose padding is staged too, else the box width), and a channel whose box
+// held one takes the global path for all its samples, which reads each axis's own taps
+// only. Returns whether this thread found one.
 __device__ __forceinline__ bool sanitize_box(float* box, const TileState& S) {
-  const int P = uni(S.Pf), rows = uni(S.rows);
+  const int P = uni(S.vec) ? uni(S.Pf) : uni(S.b[0]), rows = uni(S.rows);
   const RowEnt* rt = reinterpret_cast<const RowEnt*>(pp_box + uni(S.tab));
   const int c = threadIdx.x % P, rp = kTB / P;
   bool bad = false;
