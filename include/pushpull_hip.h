@@ -318,13 +318,15 @@ enum {
                             pull_grad, pull_hess and their pullbacks) and LDS-tiled kernels for
                             the rest; 2: LDS-tiled kernels wherever they exist (pull, pull_grad,
                             pull_hess, push, push_grad, the pullbacks, the pull VJP, the fused
-                            objective; isotropic degrees 1-4); 0: one work-item per sample */
+                            objective; degrees 1-4, a degree per axis running on the 16x8x8
+                            kernels of the largest); 0: one work-item per sample */
   PP_OPT_TILE_SHAPE = 2, /* tile of the phi grid: 0 = 16x8x8 (default), 1 = 32x8x4, 2 = 16x16x4 */
   PP_OPT_USE_TUNED = 3,  /* 1 (default): prefer the autotuner's choice when it has one */
   PP_OPT_BOX_CAP = 4     /* LDS box budget of the tiled kernels in floats (256..16384); 0
-                            (default): a per-op, per-degree default. A tile whose box exceeds
-                            the budget runs per sample through global memory; a larger budget
-                            leaves fewer workgroups per CU */
+                            (default): a per-op, per-degree default. A box beyond the budget
+                            is clipped to it and the samples it leaves out run per sample
+                            through global memory; a larger budget leaves fewer workgroups per
+                            CU */
 };
 int pp_set_option(int key, int value);
 int pp_get_option(int key);
@@ -335,7 +337,8 @@ int pp_get_option(int key);
  * timed calls each, HIP events on `stream` -- and records the fastest for (op, degree,
  * log2 of the phi-grid size); later calls with that key use it. Candidates: one work-item
  * per sample, and each tile shape at the op's default LDS box budget and at the budgets
- * that let 6, 5, 4 and 3 workgroups share a CU (isotropic degrees >= 1). The buffers are
+ * that let 6, 5, 4 and 3 workgroups share a CU (largest degree >= 1; a degree per axis
+ * runs on the 16x8x8 tiles whatever the shape asked for). The buffers are
  * those of the op's Tier B function (out2 for the objective and the VJP, loss for the
  * objective, aux for the pullback, the objective and the VJP). The descriptor's channels
  * must share one boundary triple (PP_EINVAL otherwise: other mixes run fixed kernels).
