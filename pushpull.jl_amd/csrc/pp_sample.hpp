@@ -66,7 +66,8 @@ __device__ __forceinline__ float pull_out(const KArgs& a, float val, int c, int 
 
 // Wave-wide reductions of the device library (DPP lane moves; the result reaches every
 // active lane).
-extern "C" __device__ __attribute__((const)) float __ockl_wfred_min_f32(float);
+extern "C" __device__ __attribute__((const)) int __ockl_wfred_min_i32(int);
+extern "C" __device__ __attribute__((const)) int __ockl_wfred_max_i32(int);
 extern "C" __device__ __attribute__((const)) float __ockl_wfred_max_f32(float);
 extern "C" __device__ __attribute__((const)) float __ockl_wfred_add_f32(float);
 

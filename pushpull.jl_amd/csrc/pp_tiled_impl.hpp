@@ -31,7 +31,7 @@ struct TileState {
   int nonfin;     // anisotropic degrees: 1 + the last channel whose staged box held a non-finite value
   float ctr[3];   // window centre: the tile's centre sample (else the tile centre, mapped)
   int org[3];     // tile origin in the phi grid
-  float red[6][4];  // per-wave partial min/max of the in-window coordinates
+  int red[6][4];  // per-wave partial min/max of the in-window knots
   float redf[4];  // per-wave partial bound (tile_setup's `bound` hook)
   int rows;       // box rows: b[1] * b[2]
   int tab;        // float offset, in the launch's box memory, of the row table (RowEnt[rows])
@@ -110,8 +110,9 @@ __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlan
 __device__ __forceinline__ float uni(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
 
 // Wave-wide reductions (DPP lane moves, not LDS-routed permutes).
-__device__ __forceinline__ float wave_min(float v) { return __ockl_wfred_min_f32(v); }
-__device__ __forceinline__ float wave_max(float v) { return __ockl_wfred_max_f32(v); }
+// (integer min/max fuse each DPP step into one instruction; the float forms take four)
+__device__ __forceinline__ int wave_min(int v) { return __ockl_wfred_min_i32(v); }
+__device__ __forceinline__ int wave_max(int v) { return __ockl_wfred_max_i32(v); }
 
 // A window centre at most this far from the origin keeps every in-window coordinate
 // within kFastLimit (the range of knot_fast and the box weights).
@@ -219,8 +220,8 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
   __syncthreads();
   // In-window samples: valid, and within the half-widths of the centre on every axis (a
   // NaN coordinate fails the test). The box spans the knots of their extreme coordinates
-  // (the knot is monotonic in the coordinate), so the extremes are reduced as floats and
-  // no sample's knots are computed here.
+  // (the knot is monotonic in the coordinate): each lane takes the extremes of its slots'
+  // coordinates and only their knots are computed, then reduced as integers.
   const float cx = uni(S.ctr[0]), cy = uni(S.ctr[1]), cz = uni(S.ctr[2]);
   float mn0 = INFINITY, mn1 = INFINITY, mn2 = INFINITY, mx0 = -INFINITY, mx1 = -INFINITY, mx2 = -INFINITY;
 #pragma unroll
@@ -234,15 +235,19 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
     mn1 = fminf(mn1, y); mx1 = fmaxf(mx1, y);
     mn2 = fminf(mn2, z); mx2 = fmaxf(mx2, z);
   }
-  mn0 = wave_min(mn0); mn1 = wave_min(mn1); mn2 = wave_min(mn2);
-  mx0 = wave_max(mx0); mx1 = wave_max(mx1); mx2 = wave_max(mx2);
+  const bool lany = mn0 <= mx0;  // (a lane without an in-window sample contributes nothing)
+  int kn0 = lany ? knot_fast<DP>(mn0) : INT_MAX, kn1 = lany ? knot_fast<DP>(mn1) : INT_MAX;
+  int kn2 = lany ? knot_fast<DP>(mn2) : INT_MAX, kx0 = lany ? knot_fast<DP>(mx0) : INT_MIN;
+  int kx1 = lany ? knot_fast<DP>(mx1) : INT_MIN, kx2 = lany ? knot_fast<DP>(mx2) : INT_MIN;
+  kn0 = wave_min(kn0); kn1 = wave_min(kn1); kn2 = wave_min(kn2);
+  kx0 = wave_max(kx0); kx1 = wave_max(kx1); kx2 = wave_max(kx2);
   float bnd = bound();
   if (block_bound)
     bnd = __ockl_wfred_max_f32(bnd);
   const int wid = tid >> 6;
   if ((tid & 63) == 0) {
-    S.red[0][wid] = mn0; S.red[1][wid] = mn1; S.red[2][wid] = mn2;
-    S.red[3][wid] = mx0; S.red[4][wid] = mx1; S.red[5][wid] = mx2;
+    S.red[0][wid] = kn0; S.red[1][wid] = kn1; S.red[2][wid] = kn2;
+    S.red[3][wid] = kx0; S.red[4][wid] = kx1; S.red[5][wid] = kx2;
     S.redf[wid] = bnd;
   }
   __syncthreads();
@@ -253,20 +258,18 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
     *block_bound = m;
   }
   // every thread sizes the box (a broadcast read of the partials)
-  float flo[3] = {INFINITY, INFINITY, INFINITY}, fhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
 #pragma unroll
   for (int w = 0; w < kTB / 64; ++w)
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      flo[q] = fminf(flo[q], uni(S.red[q][w]));
-      fhi[q] = fmaxf(fhi[q], uni(S.red[3 + q][w]));
+      lo[q] = min(lo[q], uni(S.red[q][w]));
+      hi[q] = max(hi[q], uni(S.red[3 + q][w]));
     }
-  bool any = flo[0] <= fhi[0];  // some in-window sample
-  int lo[3], hi[3];
+  bool any = lo[0] <= hi[0];  // some in-window sample
+  if (!any) {
 #pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    lo[q] = any ? knot_fast<DP>(flo[q]) : 0;
-    hi[q] = any ? knot_fast<DP>(fhi[q]) : 0;
+    for (int q = 0; q < 3; ++q) lo[q] = hi[q] = 0;
   }
   // LDS layouts: cell (i, j, k) at i + P j + Q k with P = bx. Plane pitches are padded to
   // Q = TX (mod 2 TX), so that the TX x-lanes on 64 / TX planes of a wave (tile_pos) fall
