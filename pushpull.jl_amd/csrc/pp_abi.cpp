@@ -1286,7 +1286,8 @@ int pp_tune_save(const char* path) {
   if (!path) return fail(PP_EINVAL, "null path");
   FILE* fp = std::fopen(path, "w");
   if (!fp) return fail(PP_EINVAL, "cannot write %s", path);
-  std::fprintf(fp, "# pushpull_hip tuning table: op dp log2(voxels) variant box_budget\n");
+  std::fprintf(fp, "# pushpull_hip tuning table: op dp log2(voxels) variant box_budget (dp: degree + 1; a degree "
+                  "triple: 100 + 36 dpx + 6 dpy + dpz)\n");
   {
     std::lock_guard<std::mutex> lock(g_tune_mu);
     for (const auto& kv : g_tune)

@@ -104,3 +104,64 @@ def test_aniso_non_finite_image_values(deg, path):
     gotg = H(pp.pull_grad(T(f), T(phi), s))
     refg = oracle.pull_grad(f, phi, so, prec="f64")
     assert np.array_equal(np.isnan(gotg), np.isnan(refg))
+
+
+@pytest.mark.parametrize("cap", [2048, 6770, 13596])
+@pytest.mark.parametrize("op", ["pull", "push", "pull_vjp", "objective"])
+def test_aniso_box_budgets(op, cap):
+    """Each LDS box budget on the anisotropic kernels (a small one clips boxes and sends
+    samples through the per-axis global path) against the FP64 oracle."""
+    rng = np.random.default_rng(970)
+    d = (40, 48, 56)
+    f, phi, g = _case(rng, d, 1, 6.0)
+    deg = (3, 1, 2)
+    s, so = pp.Settings(deg, 1, 1), S(deg, 1, 1)
+    L = pp._lib
+    keys = (L.PP_OPT_TILING, L.PP_OPT_BOX_CAP, L.PP_OPT_USE_TUNED)
+    old = [L.get_option(k) for k in keys]
+    try:
+        L.set_option(L.PP_OPT_TILING, 2)
+        L.set_option(L.PP_OPT_BOX_CAP, cap)
+        L.set_option(L.PP_OPT_USE_TUNED, 0)
+        sc = float(np.abs(g).max())
+        tag = f"aniso budget {cap} {op}"
+        if op == "pull":
+            e = check_close(H(pp.pull(T(f), T(phi), s)), oracle.pull(f, phi, so, prec="f64"), tag)
+        elif op == "push":
+            e = check_close(H(pp.push(T(g), T(phi), d, s)), oracle.push(g, phi, d, so, prec="f64"), tag, scale=sc)
+        elif op == "pull_vjp":
+            df, dphi = pp.pull_vjp(T(f), T(phi), T(g), s)
+            check_close(H(df), oracle.push(g, phi, d, so, prec="f64"), tag + " df", scale=sc)
+            e = check_close(H(dphi), oracle.backward_phi(f, phi, g, so), tag + " dphi")
+        else:
+            _, dphi, df = pp.objective_grad(T(f), T(phi), T(g), s)
+            r = oracle.pull(f, phi, so, prec="f64") - g
+            check_close(H(df), oracle.push(r, phi, d, so, prec="f64"), tag + " df", scale=float(np.abs(r).max()))
+            e = check_close(H(dphi), oracle.backward_phi(f, phi, r, so), tag + " dphi")
+        report(f"{tag} vs f64", e)
+    finally:
+        for k, v in zip(keys, old):
+            L.set_option(k, v)
+
+
+def test_aniso_tuner_records_its_own_key(tmp_path):
+    """The autotuner times tiled candidates for a degree triple too, under a key of its own
+    (not the isotropic degree's), and the tuned call matches the oracle."""
+    from pushpull import tune as pt
+
+    rng = np.random.default_rng(980)
+    d = (40, 48, 56)
+    f, phi, g = _case(rng, d, 1, 5.0)
+    s, so = pp.Settings((1, 3, 0), 1, 1), S((1, 3, 0), 1, 1)
+    pt.clear()
+    v = pt.tune("push", T(g), T(phi), s, dim=d)
+    cands = pt.results()
+    assert len(cands) == 13 and any(c["variant"] > 0 for c in cands), cands
+    path = str(tmp_path / "tune.txt")
+    pt.save(path)
+    lines = [l.split() for l in open(path) if not l.startswith("#")]
+    assert len(lines) == 1 and int(lines[0][3]) == v
+    assert int(lines[0][1]) not in (1, 2, 3, 4, 5)  # not an isotropic degree's key
+    got = H(pp.push(T(g), T(phi), d, s))
+    check_close(got, oracle.push(g, phi, d, so, prec="f64"), "aniso tuned push", scale=float(np.abs(g).max()))
+    pt.clear()
