@@ -145,6 +145,15 @@ __device__ __forceinline__ long long knot64(float c) {
   return (long long)c;
 }
 
+// The divisor of a modulo on the multi-period path, made opaque to the optimiser (an empty
+// asm statement on it), so that the division's setup -- a reciprocal of the divisor --
+// stays on that rarely taken path instead of being hoisted into every kernel's or tile's
+// prologue, where it would run on every call.
+__device__ __forceinline__ int rare_divisor(int d) {
+  asm volatile("" : "+v"(d));
+  return d;
+}
+
 // Fold index i onto [0, d): returns the folded index, sets neg for a mirrored copy under
 // code 2 (patch.cu:5-59; any code other than 0 and 2 reflects like code 1).
 __device__ __forceinline__ int fold32(int code, int i, int d, bool& neg) {
@@ -153,13 +162,18 @@ __device__ __forceinline__ int fold32(int code, int i, int d, bool& neg) {
   if (code == 0) {
     if (i < 0 && i >= -d) return i + d;
     if (i >= d && i < 2 * d) return i - d;
-    int r = i % d;
-    return r < 0 ? r + d : r;
+    const int dd = rare_divisor(d);
+    int r = i % dd;
+    return r < 0 ? r + dd : r;
   }
   int m;
   if (i < 0 && i >= -d) m = i + 2 * d;
   else if (i >= d && i < 2 * d) m = i;
-  else { m = i % (2 * d); if (m < 0) m += 2 * d; }
+  else {
+    const int d2 = rare_divisor(2 * d);
+    m = i % d2;
+    if (m < 0) m += d2;
+  }
   if (m < d) return m;
   neg = (code == 2);
   return 2 * d - 1 - m;
