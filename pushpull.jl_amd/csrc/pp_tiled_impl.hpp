@@ -109,6 +109,14 @@ struct Vox {
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ float uni(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
 
+// floor(a / b) for 0 <= a < 2^16 and 1 <= b <= 256 through a float reciprocal: (a + 0.5) / b
+// lies at least 0.5 / b from an integer, far beyond the few ulps of error of the product,
+// so the conversion truncates to the exact quotient (the generic integer division takes
+// some thirty instructions).
+__device__ __forceinline__ int small_div(int a, int b) {
+  return (int)(((float)a + 0.5f) * __builtin_amdgcn_rcpf((float)b));
+}
+
 // Wave-wide reductions (DPP lane moves, not LDS-routed permutes).
 // (integer min/max fuse each DPP step into one instruction; the float forms take four)
 __device__ __forceinline__ int wave_min(int v) { return __ockl_wfred_min_i32(v); }
@@ -382,11 +390,11 @@ struct RowWalk {
 };
 
 __device__ __forceinline__ RowWalk row_walk(const TileState& S, const KArgs& a) {
-  const int BX = S.b[0];
+  const int BX = uni(S.b[0]);
   RowWalk w;
-  w.c = threadIdx.x % BX;
-  w.r = threadIdx.x / BX;
-  w.rp = kTB / BX;
+  w.r = small_div(threadIdx.x, BX);
+  w.c = threadIdx.x - w.r * BX;
+  w.rp = small_div(kTB, BX);
   w.active = w.r < w.rp;
   bool nx;
   w.fx = fold32(a.bx, S.lo[0] + w.c, a.d0x, nx);
@@ -436,8 +444,9 @@ template <bool FLIPS>
 __device__ __forceinline__ void stage_rows_vec(float* box, const TileState& S, const float* __restrict__ src,
                                                const KArgs& a, unsigned signs) {
   const int nc = uni(S.Pf) >> 2, rows = uni(S.rows);
-  const int c = threadIdx.x % nc, rp = kTB / nc;
-  int r = threadIdx.x / nc;
+  const int rp = small_div(kTB, nc);
+  int r = small_div(threadIdx.x, nc);
+  const int c = threadIdx.x - r * nc;
   if (r >= rp) return;
   const RowEnt* rt = reinterpret_cast<const RowEnt*>(pp_box + uni(S.tab));
   const float* s0 = src + uni(S.lo[0]) + 4 * c;
@@ -474,10 +483,10 @@ __device__ __forceinline__ void stage_rows_vec(float* box, const TileState& S, c
 __device__ __forceinline__ bool sanitize_box(float* box, const TileState& S) {
   const int P = uni(S.vec) ? uni(S.Pf) : uni(S.b[0]), rows = uni(S.rows);
   const RowEnt* rt = reinterpret_cast<const RowEnt*>(pp_box + uni(S.tab));
-  const int c = threadIdx.x % P, rp = kTB / P;
+  const int r0 = small_div(threadIdx.x, P), c = threadIdx.x - r0 * P, rp = small_div(kTB, P);
   bool bad = false;
-  if ((int)threadIdx.x / P < rp)
-    for (int r = threadIdx.x / P; r < rows; r += rp) {
+  if (r0 < rp)
+    for (int r = r0; r < rows; r += rp) {
       float* q = box + (rt[r].cell & kCellMask) + c;
       if (!isfinite(*q)) {
         *q = 0.f;
