@@ -164,16 +164,21 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
   V.inside = 0;
   V.valid = 0;
   V.inl = 0;
+  // A thread's slots share their x and z positions and step along y by DY rows (tile_pos of
+  // tid + kTB q), so a slot's voxel index is the first slot's plus q DY d1x.
+  static_assert(kTB % (TX * TZ) == 0 && (kTB / TX) % TZ == 0, "slots differ in y only");
+  constexpr int DY = kTB / (TX * TZ);
+  int gx, gy0, gz;
+  tile_pos<TX, TY, TZ>(tid, gx, gy0, gz);
+  gx += x0;
+  gy0 += y0;
+  gz += z0;
+  const bool inxz = gx < a.d1x && gz < a.d1z;
+  const int v0 = gx + a.d1x * (gy0 + a.d1y * gz), dv = DY * a.d1x;
 #pragma unroll
   for (int q = 0; q < VPT; ++q) {
-    const int l = tid + kTB * q;
-    int gx, gy, gz;
-    tile_pos<TX, TY, TZ>(l, gx, gy, gz);
-    gx += x0;
-    gy += y0;
-    gz += z0;
-    const bool in = gx < a.d1x && gy < a.d1y && gz < a.d1z;
-    V.v[q] = in ? gx + a.d1x * (gy + a.d1y * gz) : 0;
+    const bool in = inxz && gy0 + DY * q < a.d1y;
+    V.v[q] = in ? v0 + q * dv : 0;
     if (in) V.inside |= 1u << q;
   }
   pre(V);
@@ -193,12 +198,7 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
   }
 #pragma unroll
   for (int q = 0; q < VPT; ++q) {
-    const int l = tid + kTB * q;
-    int gx, gy, gz;
-    tile_pos<TX, TY, TZ>(l, gx, gy, gz);
-    gx += x0;
-    gy += y0;
-    gz += z0;
+    const int gy = gy0 + DY * q;
     float x = 0.f, y = 0.f, z = 0.f;
     bool ok = false;
     if ((V.inside >> q) & 1u) {
@@ -210,8 +210,9 @@ __device__ __forceinline__ void tile_setup(const KArgs& a, const float* __restri
     V.y[q] = y;
     V.z[q] = z;
     if (ok) V.valid |= 1u << q;
-    if (l == SH::LC) {  // window centre: the centre sample, else the tile centre mapped by
-      S.org[0] = x0;    // the grid-size ratio
+    // window centre: the centre sample, else the tile centre mapped by the grid-size ratio
+    if (q == SH::LC / kTB && tid == SH::LC % kTB) {
+      S.org[0] = x0;
       S.org[1] = y0;
       S.org[2] = z0;
       if (ok && fabsf(x) < kCentreLimit && fabsf(y) < kCentreLimit && fabsf(z) < kCentreLimit) {
