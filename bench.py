@@ -659,7 +659,7 @@ def bench_matching(torch, pp, iters=50, alpha=0.02, seed=5, check=True):
 
 
 def lds_atomic_peak(torch, L, blocks=4096, iters=64, reps=5):
-    """Calibrated rate of 32-bit integer LDS atomic adds (adds/s over the whole GPU) under the
+    """Calibrated rate of 64-bit integer LDS atomic adds (adds/s over the whole GPU) under the
     scatters' own tap pattern (pp_lds_atomic_calibrate), the on-chip bound of push and the
     VJP; HIP events on the current stream."""
     from pushpull import _lib
@@ -994,14 +994,14 @@ def main():
         "wall_s": round(wall, 3),
     }
     if op in ("push", "vjp"):
-        # the scatters' on-chip bound: one 32-bit LDS atomic add per tap of every sample
+        # the scatters' on-chip bound: one 64-bit LDS atomic add per tap of every sample
         peak = lds_atomic_peak(torch, L)
         adds = (deg + 1) ** 3 * n1
         res["roofline_on_chip"] = {
             "bound": "lds_atomics", "kernel": res["roofline"]["kernel"], "adds_per_launch": adds,
             "achieved": round(adds / (ms * 1e-3) / 1e12, 4), "peak": round(peak / 1e12, 4), "unit": "Tadd/s",
             "frac": round(adds / (ms * 1e-3) / peak, 4),
-            "calibration": "pp_lds_atomic_calibrate: 4096 x 256 lanes x 64 cubic samples, ds_add_u32 per tap"}
+            "calibration": "pp_lds_atomic_calibrate: 4096 x 256 lanes x 64 cubic samples, ds_add_u64 per tap"}
     # GPU results on a slab, checked against the FP64 oracle in the CPU leg below
     headline_slabs = None if args.no_cpu else gpu_slab_outputs(torch, pp, L, f, phi, delta)
     pending = []

@@ -304,7 +304,7 @@ int pp_memset_zero(float *p, size_t count, void *stream);
 int pp_axpy(float *y, const float *x, float a, size_t count, void *stream);
 /* Calibration of the scatters' on-chip bound (no counterpart in the reference): `blocks`
  * workgroups of 256 lanes each add the 64 taps of `iters` cubic samples into an LDS box
- * shaped like a tile's, one integer LDS atomic per tap on 32-bit cells as the value
+ * shaped like a tile's, one integer LDS atomic per tap on 64-bit cells as the
  * scatters use (blocks * 256 * iters * 64 adds); out (blocks * 256 words, device)
  * receives checksums. Time it on `stream`. */
 int pp_lds_atomic_calibrate(unsigned long long *out, int blocks, int iters, void *stream);

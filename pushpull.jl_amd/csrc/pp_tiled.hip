@@ -98,8 +98,9 @@ __global__ __launch_bounds__(kTB, (tile_waves<DP, batch_waves_kind(KIND)>())) vo
 int batch_tiles(const KArgs& a) { return tiles_of<kBTX, kBTY, kBTZ>(a); }
 
 // ---- LDS atomic calibration --------------------------------------------------------
-// The rate of LDS integer adds on cells of type T (the cubic value scatter's: 32-bit)
-// under the scatter's own access pattern, the on-chip bound of push and the VJP: every lane adds the 4x4x4 taps of cubic samples into a box shaped like
+// The rate of LDS integer adds on cells of type T (the scatters' 64-bit cells) under the
+// scatter's own access pattern, the on-chip bound of push and the VJP: every lane adds the
+// 4x4x4 taps of cubic samples into a box shaped like
 // a 16x8x8 tile's at degree 3 (20 x 12 x 12 cells), its knots following the lane's x and y
 // (z: 2 per repetition, as a thread's samples), a third of the lanes shifted by a cell
 // along x and a sixth along y as under a smooth displacement. `iters` samples per lane, one

@@ -535,7 +535,7 @@ __device__ __forceinline__ double fixed_scale(float vmax) {
 // round(a * b) as int64 for |a * b| < 2^51: the FMA with 2^52 + 2^51 rounds the product
 // to an integer held in the low mantissa bits (one FMA instead of a float -> int64
 // conversion sequence; the low word of the constant is zero, so removing it costs one
-// 32-bit subtraction); fix2 adds two such products.
+// subtraction on the high word); fix2 adds two such products.
 constexpr double kMagic = 6755399441055744.0;  // 2^52 + 2^51
 
 __device__ __forceinline__ acc_t fix1(double a, double b) {
