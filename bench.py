@@ -755,7 +755,7 @@ def main():
                     help="vjp: pull on the launch stream, push and pullback from one pp_pull_vjp launch "
                          "on a second stream; overlap: the pullback runs on a second stream beside pull and "
                          "push; serial: the nine separate launches in order on one stream")
-    ap.add_argument("--leg", choices=["regulariser", "affine", "exp", "invert_def", "batched"], default=None,
+    ap.add_argument("--leg", choices=["regulariser", "affine", "exp", "invert_def", "batched", "matching"], default=None,
                     help="run only this secondary leg and print its JSON (for profiling)")
     ap.add_argument("--reg-op", choices=["all", *REG_OPS], default="all",
                     help="operators of the regulariser leg")
@@ -775,7 +775,9 @@ def main():
         print(json.dumps({"regulariser_256": bench_regulariser(torch, pp, ops=ops_sel)}))
         return
     if args.leg is not None:
-        leg = {"affine": bench_affine, "exp": bench_exp, "invert_def": bench_invert_def, "batched": bench_batched}
+        leg = {"affine": bench_affine, "exp": bench_exp, "invert_def": bench_invert_def, "batched": bench_batched,
+               "matching": lambda torch, pp: {k: v for k, v in bench_matching(torch, pp, iters=12, check=False).items()
+                                              if not k.startswith("_")}}
         print(json.dumps({args.leg: leg[args.leg](torch, pp)}))
         return
 
