@@ -266,7 +266,7 @@ typedef struct pp_volume {
   int64_t d0[3];     /* image grid (X,Y,Z) */
   int64_t d1[3];     /* phi grid (X,Y,Z) */
   int32_t nchan;
-  int32_t deg;       /* isotropic degree 0..4 */
+  int32_t deg[3];    /* degree 0..4 per axis (X,Y,Z), as pp_desc */
   int32_t ext;
   int32_t bnd[3];    /* shared by the volume's channels */
 } pp_volume;

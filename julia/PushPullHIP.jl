@@ -351,7 +351,7 @@ struct Volume
     d0::NTuple{3,Int64}
     d1::NTuple{3,Int64}
     nchan::Int32
-    deg::Int32
+    deg::NTuple{3,Int32}
     ext::Int32
     bnd::NTuple{3,Int32}
 end
@@ -359,13 +359,14 @@ end
 """
     Volume(f, ϕ; deg, bnd=(1,1,1), ext=1, aux, out, out2, loss, d₀)
 
-One volume of a batch; the DArrays must outlive every run of the plan.
+One volume of a batch (`deg` one degree or a degree per axis); the DArrays must outlive
+every run of the plan.
 """
-function Volume(f::DArray, ϕ::DArray; deg::Integer, bnd=(1, 1, 1), ext::Integer=1, aux=nothing, out::DArray,
-                out2=nothing, loss::Ptr{Float64}=Ptr{Float64}(C_NULL), d₀=spatial(f))
+function Volume(f::DArray, ϕ::DArray; deg::Union{Integer,NTuple{3,<:Integer}}, bnd=(1, 1, 1), ext::Integer=1,
+                aux=nothing, out::DArray, out2=nothing, loss::Ptr{Float64}=Ptr{Float64}(C_NULL), d₀=spatial(f))
     p(a) = a === nothing ? Ptr{Float32}(C_NULL) : a.ptr
     Volume(f.ptr, ϕ.ptr, p(aux), out.ptr, p(out2), loss, Int64.(Tuple(d₀)), Int64.(spatial(ϕ)),
-           Int32(size(f, 4)), Int32(deg), Int32(ext), Int32.(Tuple(bnd)))
+           Int32(size(f, 4)), Int32.(deg isa Integer ? (deg, deg, deg) : deg), Int32(ext), Int32.(Tuple(bnd)))
 end
 
 mutable struct BatchPlan

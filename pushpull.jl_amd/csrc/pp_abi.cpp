@@ -787,7 +787,8 @@ int pp_exp(float* out, const float* v, float* work, const long long dim[3], int 
 
 // ---- batches of uneven volumes -----------------------------------------------------
 struct pp_batch_group {
-  int dp;      // degree + 1 of every volume in the group
+  int dp;      // degree + 1 of every volume in the group (the largest of its axes)
+  int an;      // 1: the group's degrees differ between axes
   int first;   // index of the group's first volume in the device array
   int nvol;
   int total;   // tiles over the group's volumes
@@ -797,7 +798,7 @@ struct pp_batch_group {
 struct pp_batch_plan {
   pp::Op op;
   int nvol;
-  std::vector<pp_batch_group> groups;  // volumes ordered by degree, one launch per group
+  std::vector<pp_batch_group> groups;  // volumes ordered by degree group, one launch per group
   pp::BatchVol* dvols;
   // Degree groups after the first run beside it: one stream each (default priority),
   // forked from and joined back to the caller's stream with events, so that the groups'
@@ -810,22 +811,34 @@ struct pp_batch_plan {
 
 namespace {
 
+// A volume's degree group: its largest degree, and whether its degrees differ between axes
+// (those volumes run on kernels of their own). Groups 0-4: one degree on every axis; 6-9:
+// degrees that differ, the largest 1-4.
+int deg_group(const pp_volume& v) {
+  const int lo = std::min(v.deg[0], std::min(v.deg[1], v.deg[2]));
+  const int hi = std::max(v.deg[0], std::max(v.deg[1], v.deg[2]));
+  return hi + (lo != hi ? 5 : 0);
+}
+constexpr int kDegGroups = 10;
+
 // Validate a batch and lay it out: hv[0..nvol) receives the volumes' kernel arguments
-// ordered by degree (stable: input order within a degree), groups[0..*ngroups) one launch
-// each (at most five degrees). Writes only into the caller's arrays (no allocation).
+// ordered by degree group (stable: input order within a group), groups[0..*ngroups) one
+// launch each (at most nine groups). Writes only into the caller's arrays (no allocation).
 int build_batch(int op, const pp_volume* vols, int nvol, pp::BatchVol* hv, pp_batch_group* groups, int* ngroups) {
   if (!vols || nvol < 1) return fail(PP_EINVAL, "need at least one volume");
   if (op != PP_OP_PULL && op != PP_OP_PUSH && op != PP_OP_PUSH_GRAD && op != PP_OP_PULL_BACKWARD &&
       op != PP_OP_OBJECTIVE && op != PP_OP_PULL_VJP)
     return fail(PP_EINVAL, "unsupported batch op %d", op);
   for (int i = 0; i < nvol; ++i)
-    if (vols[i].deg < 0 || vols[i].deg > 4) return fail(PP_EINVAL, "volume %d: degree %d outside 0..4", i, vols[i].deg);
+    for (int q = 0; q < 3; ++q)
+      if (vols[i].deg[q] < 0 || vols[i].deg[q] > 4)
+        return fail(PP_EINVAL, "volume %d: degree %d outside 0..4 on axis %d", i, vols[i].deg[q], q);
   long long total = 0;
   int pos = 0;
-  for (int deg = 0; deg <= 4; ++deg)
+  for (int grp = 0; grp < kDegGroups; ++grp)
   for (int i = 0; i < nvol; ++i) {
     const pp_volume& v = vols[i];
-    if (v.deg != deg) continue;
+    if (deg_group(v) != grp) continue;
     long long n0 = 1, n1 = 1;
     for (int q = 0; q < 3; ++q) {
       if (v.d0[q] < 1 || v.d1[q] < 1 || v.d0[q] > kMaxDim || v.d1[q] > kMaxDim)
@@ -859,7 +872,9 @@ int build_batch(int op, const pp_volume* vols, int nvol, pp::BatchVol* hv, pp_ba
     a.d0x = (int)v.d0[0];
     a.d0y = (int)v.d0[1];
     a.d0z = (int)v.d0[2];
-    a.dpx = a.dpy = a.dpz = v.deg + 1;
+    a.dpx = v.deg[0] + 1;
+    a.dpy = v.deg[1] + 1;
+    a.dpz = v.deg[2] + 1;
     a.bx = v.bnd[0];
     a.by = v.bnd[1];
     a.bz = v.bnd[2];
@@ -889,14 +904,16 @@ int build_batch(int op, const pp_volume* vols, int nvol, pp::BatchVol* hv, pp_ba
         bv.zero_n = n0 * v.nchan;
         break;
     }
-    bv.dp = v.deg + 1;
+    bv.dp = std::max(a.dpx, std::max(a.dpy, a.dpz));
+    bv.an = grp >= 5;
     total += pp::batch_tiles(a);
     if (total > kMaxVox) return fail(PP_ERANGE, "too many tiles in one batch");
   }
   // number tiles within each degree group
   int ng = 0;
   for (int i = 0; i < nvol; ++i) {
-    if (ng == 0 || groups[ng - 1].dp != hv[i].dp) groups[ng++] = pp_batch_group{hv[i].dp, i, 0, 0, {}};
+    if (ng == 0 || groups[ng - 1].dp != hv[i].dp || groups[ng - 1].an != hv[i].an)
+      groups[ng++] = pp_batch_group{hv[i].dp, hv[i].an, i, 0, 0, {}};
     pp_batch_group& g = groups[ng - 1];
     hv[i].tile0 = g.total;
     g.total += pp::batch_tiles(hv[i].a);
@@ -932,11 +949,10 @@ hipError_t run_batch(pp::Op op, const pp::BatchVol* dvols, int nvol, const pp_ba
                      hipStream_t s) {
   hipError_t e = pp::launch_batch_zero(op, dvols, nvol, s);
   for (int gi = 0; gi < ngroups && e == hipSuccess; ++gi)
-    e = pp::launch_batch_group(op, groups[gi].dp, dvols + groups[gi].first, groups[gi].nvol, groups[gi].split, s);
+    e = pp::launch_batch_group(op, groups[gi].dp, groups[gi].an != 0, dvols + groups[gi].first, groups[gi].nvol,
+                               groups[gi].split, s);
   return e;
 }
-
-constexpr int kDegGroups = 5;  // degrees 0..4
 
 }  // namespace
 
@@ -990,12 +1006,13 @@ int pp_batch_run(const pp_batch_plan* plan, void* s) {
     const pp_batch_group& g = plan->groups[gi];
     hipStream_t ss = plan->side[gi - 1];
     e = hipStreamWaitEvent(ss, plan->fork, 0);
-    if (e == hipSuccess) e = pp::launch_batch_group(plan->op, g.dp, plan->dvols + g.first, g.nvol, g.split, ss);
+    if (e == hipSuccess)
+      e = pp::launch_batch_group(plan->op, g.dp, g.an != 0, plan->dvols + g.first, g.nvol, g.split, ss);
     if (e == hipSuccess) e = hipEventRecord(plan->join[gi - 1], ss);
   }
   if (e == hipSuccess) {
     const pp_batch_group& g = plan->groups[0];
-    e = pp::launch_batch_group(plan->op, g.dp, plan->dvols + g.first, g.nvol, g.split, S(s));
+    e = pp::launch_batch_group(plan->op, g.dp, g.an != 0, plan->dvols + g.first, g.nvol, g.split, S(s));
   }
   for (int gi = 1; gi < ng && e == hipSuccess; ++gi) e = hipStreamWaitEvent(S(s), plan->join[gi - 1], 0);
   return e == hipSuccess ? ok() : hip_fail(e, "batch launch");

@@ -114,7 +114,8 @@ hipError_t launch_tiled_aniso(Op op, const KArgs& a, int nbatch, hipStream_t str
 // One volume of a batched launch (device-resident array, built by the batch plan).
 struct BatchVol {
   KArgs a;          // one volume, all its channels; pointers already at the volume
-  int dp;           // isotropic dp = degree + 1
+  int dp;           // dp = degree + 1 (the largest, for a degree per axis)
+  int an;           // 1: the degrees differ between axes (kernels of their own)
   int tile0;        // first tile number of this volume
   long long zero_n; // floats zeroed before a push / objective launch
 };
@@ -128,7 +129,9 @@ struct XcdSplit {
 };
 hipError_t launch_batch_zero(Op op, const BatchVol* dvols, int nvol, hipStream_t stream);
 // One degree group: dvols[0..nvol) all have this dp; tile0 counts from the group start.
-hipError_t launch_batch_group(Op op, int dp, const BatchVol* dvols, int nvol, const XcdSplit& split,
+// ... the kernel of a group whose degrees differ between axes (pp_tiled_aniso.hip)
+const void* batch_kernel_aniso(Op op, int dp);
+hipError_t launch_batch_group(Op op, int dp, bool an, const BatchVol* dvols, int nvol, const XcdSplit& split,
                               hipStream_t stream, int cap_override = 0);
 
 // Fill n floats with v (graph-capturable; no hipMemsetAsync nodes).

@@ -56,45 +56,6 @@ static hipError_t launch_deg(Op op, const KArgs& a, int nbatch, hipStream_t s, T
   }
 }
 
-// ---- batched launch over volumes of uneven size ----------------------------------------
-// A plan orders its volumes by degree; each degree present gets one launch of a kernel
-// compiled for that degree only (a single kernel switching over five degrees would carry
-// the register footprint of all of them). Within a launch the tiles of the group's volumes
-// are numbered consecutively (BatchVol::tile0, relative to the group) and cut into eight
-// contiguous ranges of equal work, one per XCD (workgroup b runs on XCD b % 8), so that
-// neighbouring tiles share an L2 and no XCD idles while another still holds the heavy
-// (many-channel) volumes. Degrees 1-4 use the LDS tile bodies; degree 0 runs the same
-// bodies with an empty box budget, i.e. per sample through global memory.
-constexpr int kBTX = 16, kBTY = 8, kBTZ = 8;
-
-// k_batch kinds: gather kinds 0-2, push 3, push_grad 4, the pull VJP 5 (gather kind 3)
-constexpr int batch_gather_kind(int kind) { return kind <= 2 ? kind : kind == 5 ? 3 : -1; }
-constexpr int batch_waves_kind(int kind) { return kind <= 2 ? kind : kind == 5 ? 3 : kind + 4; }
-
-template <int DP, int KIND>
-__global__ __launch_bounds__(kTB, (tile_waves<DP, batch_waves_kind(KIND)>())) void k_batch(
-    const BatchVol* __restrict__ vols, int nvol, XcdSplit split, int cap) {
-  constexpr int KG = batch_gather_kind(KIND) >= 0 ? batch_gather_kind(KIND) : 0;
-  constexpr bool GATHER = batch_gather_kind(KIND) >= 0;
-  __shared__ TileState S;
-  __shared__ float wsum[kTB / 64];
-  // workgroup b runs on XCD b % 8 and takes the next tile of that XCD's range
-  const int x = blockIdx.x % kBatchXcds;
-  const int t = split.s[x] + (int)(blockIdx.x / kBatchXcds);
-  if (t >= split.s[x + 1]) return;
-  int lo = 0, hi = nvol - 1;  // last volume whose first tile is <= t
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (vols[mid].tile0 <= t) lo = mid;
-    else hi = mid - 1;
-  }
-  const KArgs a = vols[lo].a;
-  const int tile = t - vols[lo].tile0;
-  const int c = DP >= 2 ? cap : 0;  // degree 0: no box
-  if (GATHER) gather_tile<DP, kBTX, kBTY, kBTZ, KG>(a, tile, 0, c, pp_box, wsum, S);
-  else push_tile<DP, kBTX, kBTY, kBTZ, KIND == 4>(a, tile, 0, c, reinterpret_cast<acc_t*>(pp_box), wsum, S);
-}
-
 int batch_tiles(const KArgs& a) { return tiles_of<kBTX, kBTY, kBTZ>(a); }
 
 // ---- LDS atomic calibration --------------------------------------------------------
@@ -147,28 +108,6 @@ __global__ void k_batch_zero(const BatchVol* __restrict__ vols, int kind) {
   if (kind == OP_OBJGRAD && blockIdx.x == 0 && threadIdx.x == 0) b.a.red[0] = 0.0;
 }
 
-template <int KIND>
-static const void* batch_kernel(int dp) {
-  switch (dp) {
-    case 1: return reinterpret_cast<const void*>(&k_batch<1, KIND>);
-    case 2: return reinterpret_cast<const void*>(&k_batch<2, KIND>);
-    case 3: return reinterpret_cast<const void*>(&k_batch<3, KIND>);
-    case 4: return reinterpret_cast<const void*>(&k_batch<4, KIND>);
-    default: return reinterpret_cast<const void*>(&k_batch<5, KIND>);
-  }
-}
-
-static const void* batch_kernel_for(Op op, int dp) {
-  switch (op) {
-    case OP_PULL: return batch_kernel<0>(dp);
-    case OP_PULL_BWD: return batch_kernel<1>(dp);
-    case OP_OBJGRAD: return batch_kernel<2>(dp);
-    case OP_PULL_VJP: return batch_kernel<5>(dp);
-    case OP_PUSH: return batch_kernel<3>(dp);
-    case OP_PUSHG: return batch_kernel<4>(dp);
-    default: return nullptr;
-  }
-}
 
 hipError_t launch_batch_zero(Op op, const BatchVol* dvols, int nvol, hipStream_t s) {
   if (nvol <= 0 || !(op == OP_PUSH || op == OP_PUSHG || op == OP_OBJGRAD || op == OP_PULL_VJP)) return hipSuccess;
@@ -176,12 +115,12 @@ hipError_t launch_batch_zero(Op op, const BatchVol* dvols, int nvol, hipStream_t
   return hipGetLastError();
 }
 
-hipError_t launch_batch_group(Op op, int dp, const BatchVol* dvols, int nvol, const XcdSplit& split, hipStream_t s,
-                              int cap_override) {
+hipError_t launch_batch_group(Op op, int dp, bool an, const BatchVol* dvols, int nvol, const XcdSplit& split,
+                              hipStream_t s, int cap_override) {
   int longest = 0;
   for (int x = 0; x < kBatchXcds; ++x) longest = std::max(longest, split.s[x + 1] - split.s[x]);
   if (nvol <= 0 || longest <= 0) return hipSuccess;
-  const void* k = batch_kernel_for(op, dp);
+  const void* k = an ? batch_kernel_aniso(op, dp) : batch_kernel_for<false>(op, dp);
   if (!k) return hipErrorNotSupported;
   XcdSplit sp = split;
   int cap = cap_override > 0 ? cap_override : default_cap(op, dp);

@@ -31,4 +31,6 @@ hipError_t launch_tiled_aniso(Op op, const KArgs& a, int nbatch, hipStream_t s, 
   }
 }
 
+const void* batch_kernel_aniso(Op op, int dp) { return batch_kernel_for<true>(op, dp); }
+
 }  // namespace pp

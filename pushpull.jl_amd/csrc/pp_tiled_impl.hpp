@@ -1,5 +1,6 @@
 // pp_tiled_impl.hpp -- the tiled kernels' device code and per-shape launchers, shared by
-// pp_tiled.hip (isotropic degrees, batches) and pp_tiled_aniso.hip (a degree per axis).
+// pp_tiled.hip (isotropic degrees) and pp_tiled_aniso.hip (a degree per axis), each of which
+// instantiates its own single-call and batch kernels.
 #pragma once
 #define PP_KERNEL_SOURCE 1  // kernel arguments carry the global address space (pp_kernels.hpp)
 #include "pp_sample.hpp"
@@ -1327,6 +1328,74 @@ __global__ __launch_bounds__(kTB, (AN ? 0 : tile_waves<DP, KIND>())) void k_gath
   __shared__ TileState S;
   __shared__ float wsum[kTB / 64];
   gather_tile<DP, TX, TY, TZ, KIND, AN>(a, xcd_block(blockIdx.x, gridDim.x), blockIdx.y, cap, pp_box, wsum, S);
+}
+
+// ---- batched launch over volumes of uneven size ----------------------------------------
+// A plan orders its volumes by degree; each degree present (the largest, for a volume with
+// a degree per axis, which gets kernels of its own: AN) gets one launch of a kernel compiled
+// for that degree only (a single kernel switching over five degrees would carry the
+// register footprint of all of them). Within a launch the tiles of the group's volumes
+// are numbered consecutively (BatchVol::tile0, relative to the group) and cut into eight
+// contiguous ranges of equal work, one per XCD (workgroup b runs on XCD b % 8), so that
+// neighbouring tiles share an L2 and no XCD idles while another still holds the heavy
+// (many-channel) volumes. Degrees 1-4 use the LDS tile bodies; degree 0 runs the same
+// bodies with an empty box budget, i.e. per sample through global memory.
+constexpr int kBTX = 16, kBTY = 8, kBTZ = 8;
+
+// k_batch kinds: gather kinds 0-2, push 3, push_grad 4, the pull VJP 5 (gather kind 3)
+constexpr int batch_gather_kind(int kind) { return kind <= 2 ? kind : kind == 5 ? 3 : -1; }
+constexpr int batch_waves_kind(int kind) { return kind <= 2 ? kind : kind == 5 ? 3 : kind + 4; }
+
+template <int DP, int KIND, bool AN = false>
+__global__ __launch_bounds__(kTB, (AN ? 0 : tile_waves<DP, batch_waves_kind(KIND)>())) void k_batch(
+    const BatchVol* __restrict__ vols, int nvol, XcdSplit split, int cap) {
+  constexpr int KG = batch_gather_kind(KIND) >= 0 ? batch_gather_kind(KIND) : 0;
+  constexpr bool GATHER = batch_gather_kind(KIND) >= 0;
+  __shared__ TileState S;
+  __shared__ float wsum[kTB / 64];
+  // workgroup b runs on XCD b % 8 and takes the next tile of that XCD's range
+  const int x = blockIdx.x % kBatchXcds;
+  const int t = split.s[x] + (int)(blockIdx.x / kBatchXcds);
+  if (t >= split.s[x + 1]) return;
+  int lo = 0, hi = nvol - 1;  // last volume whose first tile is <= t
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (vols[mid].tile0 <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  const KArgs a = vols[lo].a;
+  const int tile = t - vols[lo].tile0;
+  const int c = DP >= 2 ? cap : 0;  // degree 0: no box
+  if (GATHER) gather_tile<DP, kBTX, kBTY, kBTZ, KG, AN>(a, tile, 0, c, pp_box, wsum, S);
+  else push_tile<DP, kBTX, kBTY, kBTZ, KIND == 4, AN>(a, tile, 0, c, reinterpret_cast<acc_t*>(pp_box), wsum, S);
+}
+
+
+// Batch kernel of op for degree dp (dp 1-5; AN: a degree per axis, dp the largest).
+template <int KIND, bool AN>
+static const void* batch_kernel(int dp) {
+  switch (dp) {
+    case 1:  // degree 0 on every axis (never a group with differing degrees)
+      if constexpr (AN) return nullptr;
+      else return reinterpret_cast<const void*>(&k_batch<1, KIND, AN>);
+    case 2: return reinterpret_cast<const void*>(&k_batch<2, KIND, AN>);
+    case 3: return reinterpret_cast<const void*>(&k_batch<3, KIND, AN>);
+    case 4: return reinterpret_cast<const void*>(&k_batch<4, KIND, AN>);
+    default: return reinterpret_cast<const void*>(&k_batch<5, KIND, AN>);
+  }
+}
+
+template <bool AN>
+static const void* batch_kernel_for(Op op, int dp) {
+  switch (op) {
+    case OP_PULL: return batch_kernel<0, AN>(dp);
+    case OP_PULL_BWD: return batch_kernel<1, AN>(dp);
+    case OP_OBJGRAD: return batch_kernel<2, AN>(dp);
+    case OP_PULL_VJP: return batch_kernel<5, AN>(dp);
+    case OP_PUSH: return batch_kernel<3, AN>(dp);
+    case OP_PUSHG: return batch_kernel<4, AN>(dp);
+    default: return nullptr;
+  }
 }
 
 // ---- per-shape launch -------------------------------------------------------------

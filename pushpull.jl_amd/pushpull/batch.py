@@ -12,7 +12,8 @@ Per volume (C-order shapes, as everywhere in this package):
        (C, 3, Z1, Y1, X1) for push_grad
   phi  (3, Z1, Y1, X1), 1-based
   aux  upstream gradient (pull_backward) or target (objective), (C, Z1, Y1, X1)
-  deg  isotropic degree; bnd a boundary triple shared by the volume's channels; ext
+  deg  degree 0..4, one for every axis or a triple (X, Y, Z) as Settings takes; bnd a
+       boundary triple shared by the volume's channels; ext
 """
 from __future__ import annotations
 
@@ -30,7 +31,7 @@ class PPVolume(ct.Structure):
         ("f", ct.c_void_p), ("phi", ct.c_void_p), ("aux", ct.c_void_p), ("out", ct.c_void_p),
         ("out2", ct.c_void_p), ("loss", ct.c_void_p),
         ("d0", ct.c_int64 * 3), ("d1", ct.c_int64 * 3),
-        ("nchan", ct.c_int32), ("deg", ct.c_int32), ("ext", ct.c_int32), ("bnd", ct.c_int32 * 3),
+        ("nchan", ct.c_int32), ("deg", ct.c_int32 * 3), ("ext", ct.c_int32), ("bnd", ct.c_int32 * 3),
     ]
 
 
@@ -86,9 +87,10 @@ def _volume_records(op: str, volumes: list[dict]):
         if phi.ndim != 4 or phi.shape[0] != 3:
             raise ValueError(f"{where}: `phi` must be (3, Z, Y, X), got {tuple(phi.shape)}")
         d1 = tuple(phi.shape[1:])
-        deg = int(v["deg"])
-        if not 0 <= deg <= 4:
-            raise ValueError(f"{where}: degree {deg} outside 0..4")
+        deg = v["deg"]
+        deg = tuple(int(x) for x in deg) if isinstance(deg, (tuple, list)) else (int(deg),) * 3
+        if len(deg) != 3 or any(not 0 <= x <= 4 for x in deg):
+            raise ValueError(f"{where}: `deg` must be a degree or three degrees in 0..4, got {v['deg']}")
         bnd = tuple(int(b) for b in v.get("bnd", (1, 1, 1)))
         if len(bnd) != 3 or any(b not in (0, 1, 2) for b in bnd):
             raise ValueError(f"{where}: `bnd` must be three codes in 0..2, got {bnd}")
@@ -155,7 +157,8 @@ def _volume_records(op: str, volumes: list[dict]):
             s.d0[q] = int(d0[2 - q])
             s.d1[q] = int(d1[2 - q])
             s.bnd[q] = bnd[q]
-        s.nchan, s.deg, s.ext = C, deg, int(v.get("ext", 1))
+            s.deg[q] = deg[q]
+        s.nchan, s.ext = C, int(v.get("ext", 1))
     return recs, arr
 
 

@@ -193,3 +193,87 @@ def test_batch_pull_vjp_matches_per_volume_calls():
             _check(H(o["out2"]), b, f"vjp replay df vol {i}")
     graph.close()
     plan.close()
+
+
+ANISO_DEGS = [(1, 3, 0), 2, (3, 1, 2), (0, 0, 2), 3, (4, 2, 1), (2, 2, 1), (1, 3, 0), 0, (3, 1, 2)]
+
+
+def _aniso_volumes(seed):
+    vols = _volumes(seed, n=len(ANISO_DEGS))
+    for v, deg in zip(vols, ANISO_DEGS):
+        v["deg"] = deg
+    return vols
+
+
+def test_batch_aniso_degrees_match_oracle():
+    """Volumes with a degree per axis beside isotropic ones in one plan: grouped by their
+    largest degree and by whether the degrees differ (each group one launch, the groups on
+    streams of their own), pull, the pullback, push and push_grad against the FP64 oracle."""
+    vols = _aniso_volumes(6)
+    outs = ppb.BatchPlan("pull", [_dev(v) for v in vols]).run()
+    torch.cuda.synchronize()
+    for i, (v, o) in enumerate(zip(vols, outs)):
+        s = S(v["deg"], v["bnd"], v["ext"])
+        _check(H(o["out"]), oracle.pull(v["f"][None], v["phi"][None], s, prec="f64")[0], f"aniso pull vol {i}")
+    outs = ppb.BatchPlan("pull_backward", [_dev(v) for v in vols]).run()
+    torch.cuda.synchronize()
+    for i, (v, o) in enumerate(zip(vols, outs)):
+        s = S(v["deg"], v["bnd"], v["ext"])
+        ref = oracle.backward_phi(v["f"][None], v["phi"][None], v["aux"][None], s)[0]
+        _check(H(o["out"]), ref, f"aniso pullback vol {i}")
+    rng = np.random.default_rng(7)
+    dev, dev3 = [], []
+    for v in vols:
+        v["dim"] = tuple(int(x) for x in np.array(v["d"]) + rng.integers(-3, 4, size=3))
+        v["g3"] = rng.standard_normal((v["C"], 3) + v["d"]).astype(np.float32)
+        common = dict(phi=T(v["phi"]), deg=v["deg"], bnd=v["bnd"], ext=v["ext"], dim=v["dim"])
+        dev.append(dict(f=T(v["aux"]), **common))
+        dev3.append(dict(f=T(v["g3"]), **common))
+    outs = ppb.BatchPlan("push", dev).run()
+    outs3 = ppb.BatchPlan("push_grad", dev3).run()
+    torch.cuda.synchronize()
+    for i, (v, o, o3) in enumerate(zip(vols, outs, outs3)):
+        s = S(v["deg"], v["bnd"], v["ext"])
+        _check(H(o["out"]), oracle.push(v["aux"][None], v["phi"][None], v["dim"], s, prec="f64")[0],
+               f"aniso push vol {i}")
+        ref = oracle.push_grad(v["g3"][None], v["phi"][None], v["dim"], s, prec="f64")[0]
+        _check(H(o3["out"]), ref, f"aniso push_grad vol {i}")
+
+
+@pytest.mark.parametrize("op", ["objective", "pull_vjp"])
+def test_batch_aniso_degrees_match_per_volume_calls(op):
+    """The fused objective and the pull VJP over volumes with a degree per axis against
+    per-volume calls with the same Settings, eagerly, through pp_batch_run_static, and on
+    graph replays."""
+    vols = _aniso_volumes(7)
+    plan = ppb.BatchPlan(op, [_dev(v) for v in vols])
+    outs = plan.run()
+    torch.cuda.synchronize()
+    for i, (v, o) in enumerate(zip(vols, outs)):
+        s = pp.Settings(v["deg"], v["bnd"], v["ext"])
+        if op == "objective":
+            loss, dphi, df = pp.objective_grad(T(v["f"][None]), T(v["phi"][None]), T(v["aux"][None]), s)
+            assert abs(float(o["loss"]) - float(loss)) <= TOL * abs(float(loss)), i
+        else:
+            df, dphi = pp.pull_vjp(T(v["f"][None]), T(v["phi"][None]), T(v["aux"][None]), s)
+        _check(H(o["out"]), H(dphi[0]), f"aniso {op} dphi vol {i}")
+        _check(H(o["out2"]), H(df[0]), f"aniso {op} df vol {i}")
+    first = [(H(o["out"]).copy(), H(o["out2"]).copy()) for o in outs]
+    static = ppb.run_static(op, [_dev(v) for v in vols])
+    torch.cuda.synchronize()
+    for i, ((a, b), o) in enumerate(zip(first, static)):
+        _check(H(o["out"]), a, f"aniso {op} static dphi vol {i}")
+        _check(H(o["out2"]), b, f"aniso {op} static df vol {i}")
+    graph = ppb.Graph(lambda st: plan.run(st))
+    for rep in range(3):
+        for o in plan.vols:
+            o["out"].fill_(123.0)
+            o["out2"].fill_(-7.0)
+        torch.cuda.synchronize()
+        graph.launch()
+        graph.stream.synchronize()
+        for i, ((a, b), o) in enumerate(zip(first, outs)):
+            _check(H(o["out"]), a, f"aniso {op} replay {rep} dphi vol {i}")
+            _check(H(o["out2"]), b, f"aniso {op} replay {rep} df vol {i}")
+    graph.close()
+    plan.close()

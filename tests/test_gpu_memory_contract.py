@@ -72,7 +72,8 @@ def test_batch_volumes_are_checked(dev_arrays):
     for q in range(3):
         vol.d0[q] = vol.d1[q] = D[2 - q]
         vol.bnd[q] = 1
-    vol.nchan, vol.deg, vol.ext = 1, 3, 1
+        vol.deg[q] = 3
+    vol.nchan, vol.ext = 1, 1
     L = ppb._sigs()
     h = ct.c_void_p()
     assert L.pp_batch_plan_create(0, ct.byref(vol), 1, ct.byref(h)) == -1

@@ -77,6 +77,25 @@ def test_desc_layout_matches_header(tmp_path):
     assert [int(v) for v in out] == [ct.sizeof(D), D.nchan.offset, D.nbnd.offset, D.bnd.offset]
 
 
+def test_volume_layout_matches_header(tmp_path):
+    """pp_volume (a batch's volumes, the degree a triple per volume) as the ctypes mirror
+    lays it out: every field's offset and size, and the struct's size."""
+    from pushpull.batch import PPVolume
+
+    fields = [f[0] for f in PPVolume._fields_]
+    body = "".join(f'printf("%zu %zu\\n", offsetof(pp_volume, {n}), sizeof(((pp_volume*)0)->{n}));' for n in fields)
+    src = tmp_path / "vol.c"
+    src.write_text('#include "pushpull_hip.h"\n#include <stdio.h>\n#include <stddef.h>\n'
+                   f'int main(void){{{body} printf("%zu 0\\n", sizeof(pp_volume)); return 0;}}\n')
+    exe = tmp_path / "vol"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    got = [(int(out[2 * i]), int(out[2 * i + 1])) for i in range(len(fields) + 1)]
+    want = [(getattr(PPVolume, n).offset, getattr(PPVolume, n).size) for n in fields] + [(ct.sizeof(PPVolume), 0)]
+    assert got == want
+    assert getattr(PPVolume, "deg").size == 12
+
+
 def _desc(**kw):
     d = _lib.PPDesc()
     for a in range(3):
@@ -186,12 +205,14 @@ def test_batch_plan_validation_without_gpu():
     h = ct.c_void_p()
     assert L.pp_batch_plan_create(0, None, 0, ct.byref(h)) == -1
     v = (ppb.PPVolume * 1)()
-    v[0].nchan, v[0].deg = 1, 7
+    v[0].nchan = 1
     for q in range(3):
         v[0].d0[q] = v[0].d1[q] = 4
+        v[0].deg[q] = 1
+    v[0].deg[1] = 7  # one axis out of range
     v[0].f = v[0].phi = v[0].out = 16
     assert L.pp_batch_plan_create(0, ct.cast(v, ct.c_void_p), 1, ct.byref(h)) == -1
-    assert "degree" in _lib.last_error()
+    assert "degree 7 outside 0..4 on axis 1" in _lib.last_error()
     assert L.pp_batch_plan_create(9, ct.cast(v, ct.c_void_p), 1, ct.byref(h)) == -1
 
 
@@ -212,6 +233,8 @@ def _cpu_vol(op, C=2, d1=(5, 6, 7), d0=(4, 5, 6), **over):
 @pytest.mark.parametrize("op,over,exc,msg", [
     ("pull", dict(phi=None), ValueError, "required"),
     ("pull", dict(deg=5), ValueError, "degree"),
+    ("pull", dict(deg=(1, 2)), ValueError, "three degrees"),
+    ("pull", dict(deg=(1, 5, 0)), ValueError, "three degrees"),
     ("pull", dict(bnd=(1, 3, 1)), ValueError, "bnd"),
     ("pull", dict(out=np.float32), TypeError, "torch tensor"),
     ("push", dict(dim=(4, 5)), ValueError, "dim"),
