@@ -15,7 +15,8 @@ and the VJP per degree) run on three HIP streams balanced by their measured time
 the end of each step. --schedule overlap runs the three
 separate entry points (pull and push on the launch stream, the pullback beside them);
 --schedule serial the nine separate launches in order. The per-op breakdown (separate
-entry points, and the VJP) comes from serial steps run after the timed ones.
+entry points, and the VJP) comes from serial steps run before the warm-up, which also keep
+the GPU busy up to the timed steps.
 
 The CPU baseline is the reference's own C (artifacts/C/pushpull.c, compiled from source
 into oracle/_ref by oracle/Makefile, single-threaded as shipped) timed on this host on a
@@ -886,16 +887,29 @@ def main():
     if not args.no_tune and args.tiling == 1:
         _progress("autotuning")
         tuned = autotune(torch, pp, f, phi, delta, args.schedule)
-    _progress("warm-up")
-    for _ in range(args.warmup):
-        timed_step()
-    torch.cuda.synchronize()
-    _progress("timed steps")
     ev = Ev()
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
+    # The GPU's clocks fall while it idles and take some 30 ms of load to come back: the host
+    # work before the timed steps (the collector's pass above all) is done first, and the GPU
+    # then stays busy -- the VJP timings, the serial steps of the per-op breakdown, the
+    # warm-up -- up to the timed steps (20 steps timed right after a collector pass and 5
+    # warm-up steps ran from 1.9 down to 1.52 ms a step, 1.66 ms on average).
     gc.collect()
-    gc.disable()  # no collector pause inside the timed launches
+    gc.disable()  # no collector pause from here to the end of the timed launches
+    vjp_ms = {}
+    for deg in DEGS:  # the VJP per degree, zero fill of df included (as push's row)
+        dd = ct.byref(descs[deg])
+        vjp_ms[deg] = _time_ms_median(
+            torch, lambda: _lib.check(L.pp_pull_vjp(dd, P(dphi), P(df), P(phi), P(f), P(delta), 0, stream), "vjp"),
+            args.steps)
+    if args.schedule != "serial":  # per-op times come from separate serial steps
+        _progress("serial steps for the per-op breakdown")
+        for _ in range(args.steps):
+            step(ev)
+    _progress("warm-up")
+    for _ in range(args.warmup):
+        timed_step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     start.record()
@@ -909,19 +923,7 @@ def main():
     wall = time.perf_counter() - t0
     gc.enable()
     ms_step = start.elapsed_time(end) / args.steps
-    if args.schedule != "serial":  # per-op times come from separate serial steps
-        _progress("serial steps for the per-op breakdown")
-        for _ in range(args.steps):
-            step(ev)
-        torch.cuda.synchronize()
     ms_serial = float(np.sum([m[0].elapsed_time(m[3]) for _, m in ev.rows])) / args.steps
-
-    vjp_ms = {}
-    for deg in DEGS:  # the VJP per degree, zero fill of df included (as push's row)
-        dd = ct.byref(descs[deg])
-        vjp_ms[deg] = _time_ms_median(
-            torch, lambda: _lib.check(L.pp_pull_vjp(dd, P(dphi), P(df), P(phi), P(f), P(delta), 0, stream), "vjp"),
-            args.steps)
     timed_ops = ("pull", "vjp") if args.schedule == "vjp" else ("pull", "push", "backward")
     per = {deg: {"pull": [], "push": [], "backward": []} for deg in DEGS}
     for deg, m in ev.rows:
