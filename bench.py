@@ -890,19 +890,19 @@ def main():
     ev = Ev()
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
-    # The GPU's clocks fall while it idles and take some 30 ms of load to come back: the host
-    # work before the timed steps (the collector's pass above all) is done first, and the GPU
-    # then stays busy -- the VJP timings, the serial steps of the per-op breakdown, the
-    # warm-up -- up to the timed steps (20 steps timed right after a collector pass and 5
-    # warm-up steps ran from 1.9 down to 1.52 ms a step, 1.66 ms on average).
-    gc.collect()
-    gc.disable()  # no collector pause from here to the end of the timed launches
     vjp_ms = {}
     for deg in DEGS:  # the VJP per degree, zero fill of df included (as push's row)
         dd = ct.byref(descs[deg])
         vjp_ms[deg] = _time_ms_median(
             torch, lambda: _lib.check(L.pp_pull_vjp(dd, P(dphi), P(df), P(phi), P(f), P(delta), 0, stream), "vjp"),
             args.steps)
+    # The GPU's clocks fall while it idles and take some 30 ms of load to come back: the host
+    # work before the timed steps (the collector's pass above all) is done first, and the GPU
+    # then stays busy -- the serial steps of the per-op breakdown, the warm-up -- up to the
+    # timed steps (20 steps timed right after a collector pass and 5 warm-up steps ran from
+    # 1.9 down to 1.52 ms a step, 1.66 ms on average).
+    gc.collect()
+    gc.disable()  # no collector pause from here to the end of the timed launches
     if args.schedule != "serial":  # per-op times come from separate serial steps
         _progress("serial steps for the per-op breakdown")
         for _ in range(args.steps):
