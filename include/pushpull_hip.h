@@ -242,10 +242,12 @@ int pp_relax(const pp_sparse_kernel *kernel, float *v, const float *g, const flo
 /* ---- batches of uneven volumes --------------------------------------------
  * No counterpart in the reference, whose wrappers loop over same-size (channel, batch)
  * slices with one native call each (src/pushpull_cpu.jl:42-51, ext/pushpull_gpu.jl:37-48).
- * A plan describes many volumes of different sizes, channel counts and degrees; running it
- * is one persistent launch over all the tiles of the volumes that share a degree (one per
- * degree present; plus one zeroing launch for the accumulating ops), on kernels only, so
- * it can be captured into a HIP graph. */
+ * A plan describes many volumes of different sizes, channel counts, degrees (a triple per
+ * volume) and boundary codes (a triple per volume or per channel); running it is one
+ * persistent launch over all the tiles of the volumes that share a degree group (their
+ * largest degree, and whether their degrees differ between axes; plus one zeroing launch
+ * for the accumulating ops), on kernels only, so it can be captured into a HIP graph. */
+#define PP_BATCH_CHAN_BND 10 /* channels of a volume that may carry boundary codes of their own */
 enum {
   PP_OP_PULL = 0,          /* out = pull(f, phi)                                   */
   PP_OP_PULL_GRAD = 1,     /* out = pull_grad(f, phi)                              */
@@ -268,7 +270,13 @@ typedef struct pp_volume {
   int32_t nchan;
   int32_t deg[3];    /* degree 0..4 per axis (X,Y,Z), as pp_desc */
   int32_t ext;
-  int32_t bnd[3];    /* shared by the volume's channels */
+  int32_t bnd[3];    /* shared by the volume's channels (nbnd 0 or 1) */
+  /* nbnd = nchan (<= PP_BATCH_CHAN_BND): chan_bnd[3c + axis] is channel c's code instead
+   * (src/pushpull_cpu.jl:43 takes a triple per channel). Along an axis the channels' codes
+   * must be equal or all in {1, 2} (reflect, antisymmetric: the same fold, a sign apart);
+   * other mixes take single calls. */
+  int32_t nbnd;
+  int32_t chan_bnd[3 * PP_BATCH_CHAN_BND];
 } pp_volume;
 
 typedef struct pp_batch_plan pp_batch_plan;

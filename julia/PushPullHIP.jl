@@ -32,6 +32,7 @@ export Settings, DArray, todevice, tohost, free!, id,
 
 const lib = Ref{Ptr{Cvoid}}(C_NULL)
 const PP_MAX_CHAN_BND = 64
+const PP_BATCH_CHAN_BND = 10
 
 function __init__()
     lib[] = Libdl.dlopen(get(ENV, "PUSHPULL_HIP_LIB", "libpushpull_hip.so"))
@@ -354,19 +355,29 @@ struct Volume
     deg::NTuple{3,Int32}
     ext::Int32
     bnd::NTuple{3,Int32}
+    nbnd::Int32
+    chan_bnd::NTuple{3 * PP_BATCH_CHAN_BND,Int32}
 end
 
 """
     Volume(f, ϕ; deg, bnd=(1,1,1), ext=1, aux, out, out2, loss, d₀)
 
-One volume of a batch (`deg` one degree or a degree per axis); the DArrays must outlive
-every run of the plan.
+One volume of a batch (`deg` one degree or a degree per axis; `bnd` one triple, or a vector
+of triples, one per channel); the DArrays must outlive every run of the plan.
 """
 function Volume(f::DArray, ϕ::DArray; deg::Union{Integer,NTuple{3,<:Integer}}, bnd=(1, 1, 1), ext::Integer=1,
                 aux=nothing, out::DArray, out2=nothing, loss::Ptr{Float64}=Ptr{Float64}(C_NULL), d₀=spatial(f))
     p(a) = a === nothing ? Ptr{Float32}(C_NULL) : a.ptr
+    C = size(f, 4)
+    per = bnd isa AbstractVector
+    per && @assert length(bnd) == C <= PP_BATCH_CHAN_BND
+    codes = zeros(Int32, 3 * PP_BATCH_CHAN_BND)
+    per && for c in 1:C, q in 1:3
+        codes[3 * (c - 1) + q] = bnd[c][q]
+    end
     Volume(f.ptr, ϕ.ptr, p(aux), out.ptr, p(out2), loss, Int64.(Tuple(d₀)), Int64.(spatial(ϕ)),
-           Int32(size(f, 4)), Int32.(deg isa Integer ? (deg, deg, deg) : deg), Int32(ext), Int32.(Tuple(bnd)))
+           Int32(C), Int32.(deg isa Integer ? (deg, deg, deg) : deg), Int32(ext),
+           Int32.(Tuple(per ? bnd[1] : bnd)), Int32(per ? C : 0), Tuple(codes))
 end
 
 mutable struct BatchPlan

@@ -209,9 +209,10 @@ bool sign_split(const pp_desc* d, int base[3], unsigned& mask) {
   return true;
 }
 
-// Launch `op` over all channels: one launch when the boundary codes are shared (or, for
-// the tiled pull and push, differ only in sign), else one per channel (pointers offset by
-// the channel strides; channel-reducing outputs accumulate after the first launch).
+// Launch `op` over all channels: one launch when the boundary codes are shared (or differ
+// only in sign, on the tiled kernels of an isotropic degree 1-4), else one per channel
+// (pointers offset by the channel strides; channel-reducing outputs accumulate after the
+// first launch).
 int run(pp::Op op, const pp_desc* d, const Geo& g, pp::KArgs a, hipStream_t s) {
   const bool reduce =
       (op == pp::OP_PULL_BWD || op == pp::OP_PULLG_BWD || op == pp::OP_OBJGRAD || op == pp::OP_PULL_VJP);
@@ -226,18 +227,22 @@ int run(pp::Op op, const pp_desc* d, const Geo& g, pp::KArgs a, hipStream_t s) {
   int base[3];
   unsigned mask;
   const bool iso_tiled = a.dpx == a.dpy && a.dpx == a.dpz && a.dpx >= 2 && a.dpx <= 5;
-  if ((op == pp::OP_PULL || op == pp::OP_PUSH) && iso_tiled && g_tiling.load(std::memory_order_relaxed) != 0 &&
-      sign_split(d, base, mask)) {
-    a.bx = base[0];
-    a.by = base[1];
-    a.bz = base[2];
-    a.csign_on = 1;
-    a.csign = mask;
-    a.nchan = d->nchan;
-    // the per-sample pull (trilinear: faster there) and the tiled kernels take the signs
-    const bool per_sample = op == pp::OP_PULL && a.dpx == 2;
-    hipError_t e = launch_variant(op, a, d->nbatch, s, per_sample ? 0 : 1 + g_shape.load(std::memory_order_relaxed));
-    return e == hipSuccess ? ok() : hip_fail(e, "kernel launch");
+  if (iso_tiled && g_tiling.load(std::memory_order_relaxed) != 0 && sign_split(d, base, mask)) {
+    pp::KArgs b = a;
+    b.bx = base[0];
+    b.by = base[1];
+    b.bz = base[2];
+    b.csign_on = 1;
+    b.csign = mask;
+    b.nchan = d->nchan;
+    // the per-sample pull (trilinear: faster there) and the tiled kernels take the signs; an
+    // op the tiled kernels decline runs once per channel below
+    hipError_t e = op == pp::OP_PULL && a.dpx == 2
+                       ? pp::launch_op(op, b, d->nbatch, s, kBlock)
+                       : pp::launch_tiled(op, b, d->nbatch, s,
+                                          pp::TileCfg{g_shape.load(std::memory_order_relaxed),
+                                                      g_cap.load(std::memory_order_relaxed)});
+    if (e != hipErrorNotSupported) return e == hipSuccess ? ok() : hip_fail(e, "kernel launch");
   }
   for (int c = 0; c < d->nchan; ++c) {
     pp::KArgs b = a;
@@ -821,6 +826,47 @@ int deg_group(const pp_volume& v) {
 }
 constexpr int kDegGroups = 10;
 
+// Volume i's boundary codes into a: one triple for every channel (nbnd 0 or 1), or a triple
+// per channel (nbnd = nchan) whose codes along each axis are equal or all reflect (1) /
+// antisymmetric (2): those fold to the same indices and differ in sign only, so all
+// channels fold with one code per axis and each takes its own signs (KArgs::csign).
+int batch_bnd(const pp_volume& v, int i, pp::KArgs& a) {
+  const int nb = v.nbnd;
+  if (nb < 0 || (nb > 1 && nb != v.nchan))
+    return fail(PP_EINVAL, "volume %d: nbnd must be 0, 1 or nchan (%d), got %d", i, v.nchan, nb);
+  if (nb > PP_BATCH_CHAN_BND)
+    return fail(PP_EINVAL, "volume %d: at most %d channels take boundary codes of their own, got %d", i,
+                PP_BATCH_CHAN_BND, nb);
+  const int nc = nb > 1 ? nb : 1;
+  const int32_t* codes = nb > 1 ? v.chan_bnd : v.bnd;
+  int base[3];
+  unsigned mask = 0u;
+  bool split = false;
+  for (int q = 0; q < 3; ++q) {
+    bool any2 = false, same = true, refl = true;
+    for (int c = 0; c < nc; ++c) {
+      const int code = codes[3 * c + q];
+      if (code < 0 || code > 2) return fail(PP_EINVAL, "volume %d: boundary code %d", i, code);
+      any2 |= code == 2;
+      same &= code == codes[q];
+      refl &= code == 1 || code == 2;
+      if (code == 2) mask |= 1u << (3 * c + q);
+    }
+    if (!same && !refl)
+      return fail(PP_EINVAL,
+                  "volume %d: the channels' boundary codes along axis %d differ other than between 1 and 2 "
+                  "(run such a volume with single calls)", i, q);
+    split |= !same;
+    base[q] = any2 ? 2 : codes[q];
+  }
+  a.bx = base[0];
+  a.by = base[1];
+  a.bz = base[2];
+  a.csign_on = split ? 1 : 0;
+  a.csign = split ? mask : 0u;
+  return PP_OK;
+}
+
 // Validate a batch and lay it out: hv[0..nvol) receives the volumes' kernel arguments
 // ordered by degree group (stable: input order within a group), groups[0..*ngroups) one
 // launch each (at most nine groups). Writes only into the caller's arrays (no allocation).
@@ -845,10 +891,11 @@ int build_batch(int op, const pp_volume* vols, int nvol, pp::BatchVol* hv, pp_ba
         return fail(PP_EINVAL, "volume %d: grid size out of range on axis %d", i, q);
       n0 *= v.d0[q];
       n1 *= v.d1[q];
-      if (v.bnd[q] < 0 || v.bnd[q] > 2) return fail(PP_EINVAL, "volume %d: boundary code %d", i, v.bnd[q]);
     }
     if (n0 > kMaxVox || n1 > kMaxVox) return fail(PP_ERANGE, "volume %d holds more than 2^31-1 voxels", i);
     if (v.nchan < 1) return fail(PP_EINVAL, "volume %d: nchan must be >= 1", i);
+    pp::KArgs codes{};
+    if (int r = batch_bnd(v, i, codes)) return r;
     if (!v.f || !v.phi || !v.out) return fail(PP_EINVAL, "volume %d: null array", i);
     if ((op == PP_OP_PULL_BACKWARD || op == PP_OP_OBJECTIVE || op == PP_OP_PULL_VJP) && !v.aux)
       return fail(PP_EINVAL, "volume %d: null aux", i);
@@ -875,9 +922,11 @@ int build_batch(int op, const pp_volume* vols, int nvol, pp::BatchVol* hv, pp_ba
     a.dpx = v.deg[0] + 1;
     a.dpy = v.deg[1] + 1;
     a.dpz = v.deg[2] + 1;
-    a.bx = v.bnd[0];
-    a.by = v.bnd[1];
-    a.bz = v.bnd[2];
+    a.bx = codes.bx;
+    a.by = codes.by;
+    a.bz = codes.bz;
+    a.csign_on = codes.csign_on;
+    a.csign = codes.csign;
     a.ext = v.ext != 0;
     a.lx = (float)v.d0[0] - 0.99f;
     a.ly = (float)v.d0[1] - 0.99f;

@@ -64,7 +64,7 @@ struct KArgs {
   float lx, ly, lz;          // d0 - 0.99f per axis (VOXOK upper bounds)
   int affine;                // 1: sample coordinates come from aff, not from phi
   float aff[12];             // 0-based 3x4 affine, column-major (artifacts/C/pushpull.c:185-201)
-  // Per-channel boundary signs (tiled pull and push): all channels fold with bx/by/bz;
+  // Per-channel boundary signs (the tiled kernels): all channels fold with bx/by/bz;
   // channel c takes the antisymmetric sign along axis a only where bit 3c+a of csign is
   // set (codes 1 and 2 fold to the same index and differ only in sign).
   int csign_on;

@@ -1065,14 +1065,17 @@ __device__ __forceinline__ void lds_hess(const float* box, const LSampleH<DP>& L
   }
 }
 
-// Gradient gather of one sample outside the tile's box, through global memory; for the
-// objective (KIND 2) it also scatters the residual, for the VJP (3) the upstream gradient.
-// Returns the weight w of the pullback (the upstream gradient, or the residual). Rarely
-// taken, so it gathers lean: it must not set the register budget of the whole kernel.
+// Gradient gather of one sample of channel c outside the tile's box, through global
+// memory; for the objective (KIND 2) it also scatters the residual, for the VJP (3) the
+// upstream gradient. Returns the weight w of the pullback (the upstream gradient, or the
+// residual). Rarely taken, so it gathers lean: it must not set the register budget of the
+// whole kernel.
 template <int DP, int KIND, bool AN>
-__device__ __forceinline__ float outlier_grad(const KArgs& a, const float* __restrict__ src, float* __restrict__ dst2,
-                                           float x, float y, float z, float aux, float& gx, float& gy, float& gz) {
-  return with_axes<DP, kW | kDW, AN>(a, x, y, z, a.bx, a.by, a.bz, [&](const auto& ax, const auto& ay, const auto& az) {
+__device__ __forceinline__ float outlier_grad(const KArgs& a, int c, const float* __restrict__ src,
+                                              float* __restrict__ dst2, float x, float y, float z, float aux, float& gx,
+                                              float& gy, float& gz) {
+  return with_axes<DP, kW | kDW, AN>(a, x, y, z, chan_code(a, c, 0, a.bx), chan_code(a, c, 1, a.by),
+                                     chan_code(a, c, 2, a.bz), [&](const auto& ax, const auto& ay, const auto& az) {
     constexpr int D = PP_AXIS_DP(ax);
     float val = 0.f;
     if (KIND == 2) sample_val_grad<D, true>(src, ax, ay, az, val, gx, gy, gz);
@@ -1194,8 +1197,8 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, in
           lsample_h<DP, AN>(a, x, y, z, S, P, Q, L);
           lds_hess<DP>(box, L, P, Q, h);
         } else {
-          with_axes<DP, kW | kDW | kHW, AN>(a, x, y, z, a.bx, a.by, a.bz,
-                                        [&](const auto& ax, const auto& ay, const auto& az) {
+          with_axes<DP, kW | kDW | kHW, AN>(a, x, y, z, chan_code(a, c, 0, a.bx), chan_code(a, c, 1, a.by),
+                                            chan_code(a, c, 2, a.bz), [&](const auto& ax, const auto& ay, const auto& az) {
                                           sample_hess<PP_AXIS_DP(ax)>(src, ax, ay, az, h);
                                           return 0;
                                         });
@@ -1223,7 +1226,8 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, in
           float val;
           lds_grad<DP, false>(box, L, P, Q, val, gx, gy, gz);
         } else {
-          with_axes<DP, kW | kDW, AN>(a, x, y, z, a.bx, a.by, a.bz, [&](const auto& ax, const auto& ay, const auto& az) {
+          with_axes<DP, kW | kDW, AN>(a, x, y, z, chan_code(a, c, 0, a.bx), chan_code(a, c, 1, a.by),
+                                      chan_code(a, c, 2, a.bz), [&](const auto& ax, const auto& ay, const auto& az) {
             sample_grad<PP_AXIS_DP(ax), true>(src, ax, ay, az, gx, gy, gz);
             return 0;
           });
@@ -1243,7 +1247,7 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, in
           rfin |= 1u << q;
         }
       } else {
-        w = outlier_grad<DP, KIND, AN>(a, src, a.out2 + bb * a.out2_bs + c * a.out2_cs, x, y, z, aux, gx, gy, gz);
+        w = outlier_grad<DP, KIND, AN>(a, c, src, a.out2 + bb * a.out2_bs + c * a.out2_cs, x, y, z, aux, gx, gy, gz);
       }
       if (KIND == 2 && isfinite(w)) lsum += 0.5f * w * w;
       gsum[0][qi] += w * gx;

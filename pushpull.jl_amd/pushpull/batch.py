@@ -12,8 +12,10 @@ Per volume (C-order shapes, as everywhere in this package):
        (C, 3, Z1, Y1, X1) for push_grad
   phi  (3, Z1, Y1, X1), 1-based
   aux  upstream gradient (pull_backward) or target (objective), (C, Z1, Y1, X1)
-  deg  degree 0..4, one for every axis or a triple (X, Y, Z) as Settings takes; bnd a
-       boundary triple shared by the volume's channels; ext
+  deg  degree 0..4, one for every axis or a triple (X, Y, Z) as Settings takes
+  bnd  a boundary triple for all channels, or a list of C triples, one per channel (at
+       most 10; along an axis the channels' codes equal or all in {1, 2})
+  ext
 """
 from __future__ import annotations
 
@@ -24,6 +26,7 @@ import torch
 from . import _lib
 
 OPS = {"pull": 0, "push": 3, "push_grad": 4, "pull_backward": 5, "objective": 7, "pull_vjp": 8}
+BATCH_CHAN_BND = 10  # PP_BATCH_CHAN_BND: channels of a volume with boundary codes of their own
 
 
 class PPVolume(ct.Structure):
@@ -32,6 +35,7 @@ class PPVolume(ct.Structure):
         ("out2", ct.c_void_p), ("loss", ct.c_void_p),
         ("d0", ct.c_int64 * 3), ("d1", ct.c_int64 * 3),
         ("nchan", ct.c_int32), ("deg", ct.c_int32 * 3), ("ext", ct.c_int32), ("bnd", ct.c_int32 * 3),
+        ("nbnd", ct.c_int32), ("chan_bnd", ct.c_int32 * (3 * BATCH_CHAN_BND)),
     ]
 
 
@@ -51,6 +55,22 @@ def _sigs():
         fn.argtypes = a
         fn.restype = ct.c_int
     return L
+
+
+def _boundaries(b, C, where):
+    """`bnd` of a volume as a list of triples: one for all channels, or one per channel."""
+    try:
+        nested = len(b) > 0 and hasattr(b[0], "__len__")
+        codes = [tuple(int(x) for x in t) for t in b] if nested else [tuple(int(x) for x in b)]
+    except TypeError:
+        codes = [()]
+    if any(len(t) != 3 or any(x not in (0, 1, 2) for x in t) for t in codes):
+        raise ValueError(f"{where}: `bnd` must be three codes in 0..2, or one such triple per channel, got {b}")
+    if len(codes) > 1 and len(codes) != C:
+        raise ValueError(f"{where}: `bnd` holds {len(codes)} triples for {C} channels")
+    if len(codes) > BATCH_CHAN_BND:
+        raise ValueError(f"{where}: at most {BATCH_CHAN_BND} channels take boundary codes of their own")
+    return codes
 
 
 def _p(t):
@@ -91,9 +111,6 @@ def _volume_records(op: str, volumes: list[dict]):
         deg = tuple(int(x) for x in deg) if isinstance(deg, (tuple, list)) else (int(deg),) * 3
         if len(deg) != 3 or any(not 0 <= x <= 4 for x in deg):
             raise ValueError(f"{where}: `deg` must be a degree or three degrees in 0..4, got {v['deg']}")
-        bnd = tuple(int(b) for b in v.get("bnd", (1, 1, 1)))
-        if len(bnd) != 3 or any(b not in (0, 1, 2) for b in bnd):
-            raise ValueError(f"{where}: `bnd` must be three codes in 0..2, got {bnd}")
         if op in ("push", "push_grad"):
             if "dim" not in v:
                 raise ValueError(f"{where}: push needs `dim`, the (Z, Y, X) output grid")
@@ -110,6 +127,7 @@ def _volume_records(op: str, volumes: list[dict]):
                 raise ValueError(f"{where}: `f` must be (C, Z0, Y0, X0), got {tuple(f.shape)}")
             d0 = tuple(f.shape[1:])
         C = int(f.shape[0])
+        bnd = _boundaries(v.get("bnd", (1, 1, 1)), C, where)
         if op in ("pull_backward", "objective", "pull_vjp"):
             if aux is None:
                 raise ValueError(f"{where}: `aux` is required for {op}")
@@ -156,8 +174,13 @@ def _volume_records(op: str, volumes: list[dict]):
         for q in range(3):
             s.d0[q] = int(d0[2 - q])
             s.d1[q] = int(d1[2 - q])
-            s.bnd[q] = bnd[q]
+            s.bnd[q] = bnd[0][q]
             s.deg[q] = deg[q]
+        s.nbnd = len(bnd) if len(bnd) > 1 else 0
+        if len(bnd) > 1:
+            for c, t in enumerate(bnd):
+                for q in range(3):
+                    s.chan_bnd[3 * c + q] = t[q]
         s.nchan, s.ext = C, int(v.get("ext", 1))
     return recs, arr
 

@@ -277,3 +277,74 @@ def test_batch_aniso_degrees_match_per_volume_calls(op):
             _check(H(o["out2"]), b, f"aniso {op} replay {rep} df vol {i}")
     graph.close()
     plan.close()
+
+
+CHAN_BND = [[(2, 1, 1), (1, 2, 1), (1, 1, 2)], [(1, 2, 0), (2, 1, 0)], [(2, 2, 1), (2, 2, 1), (1, 1, 1), (2, 1, 2)]]
+CHAN_DEGS = [1, 3, (1, 3, 0), 2, 0, (3, 1, 2)]
+
+
+def _chan_bnd_volumes(seed):
+    """Volumes whose channels take boundary triples of their own (the sliding boundaries of a
+    displacement field among them), a few samples far outside the field of view."""
+    rng = np.random.default_rng(seed)
+    vols = []
+    for i, deg in enumerate(CHAN_DEGS):
+        bnd = CHAN_BND[i % len(CHAN_BND)]
+        C = len(bnd)
+        d = tuple(int(v) for v in rng.integers(16, 34, size=3))
+        f = np.stack([smooth_field(rng, d, 1.5, 1.0)[0] for _ in range(C)]).astype(np.float32)
+        phi = identity(d) + smooth_field(rng, d, 3.0, 4.0)
+        flat = phi.reshape(3, -1)
+        idx = rng.choice(flat.shape[1], size=flat.shape[1] // 25, replace=False)
+        flat[1, idx] = rng.uniform(-15, d[1] + 15, size=idx.size).astype(np.float32)
+        aux = rng.standard_normal((C,) + d).astype(np.float32)
+        vols.append(dict(f=f, phi=phi, aux=aux, deg=deg, bnd=bnd, ext=int(i % 2 == 0), d=d, C=C))
+    return vols
+
+
+def test_batch_channel_boundaries_match_oracle():
+    """Per-channel boundary triples in one plan (one fold per axis, per-channel signs):
+    pull, the pullback, push and push_grad against the FP64 oracle."""
+    vols = _chan_bnd_volumes(11)
+    outs = ppb.BatchPlan("pull", [_dev(v) for v in vols]).run()
+    outs_b = ppb.BatchPlan("pull_backward", [_dev(v) for v in vols]).run()
+    rng = np.random.default_rng(12)
+    dev, dev3 = [], []
+    for v in vols:
+        v["dim"] = tuple(int(x) for x in np.array(v["d"]) + rng.integers(-3, 4, size=3))
+        v["g3"] = rng.standard_normal((v["C"], 3) + v["d"]).astype(np.float32)
+        common = dict(phi=T(v["phi"]), deg=v["deg"], bnd=v["bnd"], ext=v["ext"], dim=v["dim"])
+        dev.append(dict(f=T(v["aux"]), **common))
+        dev3.append(dict(f=T(v["g3"]), **common))
+    outs_p = ppb.BatchPlan("push", dev).run()
+    outs_g = ppb.BatchPlan("push_grad", dev3).run()
+    torch.cuda.synchronize()
+    for i, v in enumerate(vols):
+        s = S(v["deg"], v["bnd"], v["ext"])
+        f, phi, aux = v["f"][None], v["phi"][None], v["aux"][None]
+        _check(H(outs[i]["out"]), oracle.pull(f, phi, s, prec="f64")[0], f"chan-bnd pull vol {i}")
+        _check(H(outs_b[i]["out"]), oracle.backward_phi(f, phi, aux, s)[0], f"chan-bnd pullback vol {i}")
+        _check(H(outs_p[i]["out"]), oracle.push(aux, phi, v["dim"], s, prec="f64")[0], f"chan-bnd push vol {i}")
+        ref = oracle.push_grad(v["g3"][None], phi, v["dim"], s, prec="f64")[0]
+        _check(H(outs_g[i]["out"]), ref, f"chan-bnd push_grad vol {i}")
+
+
+@pytest.mark.parametrize("op", ["objective", "pull_vjp"])
+def test_batch_channel_boundaries_match_per_volume_calls(op):
+    """The objective and the pull VJP over volumes with per-channel boundary triples against
+    per-volume calls (which launch once per channel), eagerly and through
+    pp_batch_run_static."""
+    vols = _chan_bnd_volumes(13)
+    outs = ppb.BatchPlan(op, [_dev(v) for v in vols]).run()
+    static = ppb.run_static(op, [_dev(v) for v in vols])
+    torch.cuda.synchronize()
+    for i, (v, o, st) in enumerate(zip(vols, outs, static)):
+        s = pp.Settings(v["deg"], v["bnd"], v["ext"])
+        if op == "objective":
+            loss, dphi, df = pp.objective_grad(T(v["f"][None]), T(v["phi"][None]), T(v["aux"][None]), s)
+            assert abs(float(o["loss"]) - float(loss)) <= TOL * abs(float(loss)), i
+        else:
+            df, dphi = pp.pull_vjp(T(v["f"][None]), T(v["phi"][None]), T(v["aux"][None]), s)
+        for got, what in ((o, "plan"), (st, "static")):
+            _check(H(got["out"]), H(dphi[0]), f"chan-bnd {op} {what} dphi vol {i}")
+            _check(H(got["out2"]), H(df[0]), f"chan-bnd {op} {what} df vol {i}")

@@ -94,6 +94,10 @@ def test_volume_layout_matches_header(tmp_path):
     want = [(getattr(PPVolume, n).offset, getattr(PPVolume, n).size) for n in fields] + [(ct.sizeof(PPVolume), 0)]
     assert got == want
     assert getattr(PPVolume, "deg").size == 12
+    src.write_text('#include "pushpull_hip.h"\n#include <stdio.h>\nint main(void){printf("%d\\n", PP_BATCH_CHAN_BND); return 0;}\n')
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    from pushpull.batch import BATCH_CHAN_BND
+    assert int(subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout) == BATCH_CHAN_BND
 
 
 def _desc(**kw):
@@ -216,6 +220,32 @@ def test_batch_plan_validation_without_gpu():
     assert L.pp_batch_plan_create(9, ct.cast(v, ct.c_void_p), 1, ct.byref(h)) == -1
 
 
+@pytest.mark.parametrize("nbnd,codes,msg", [
+    (3, [(1, 1, 1)] * 3, "nbnd must be 0, 1 or nchan (2), got 3"),
+    (2, [(1, 1, 1), (1, 3, 1)], "boundary code 3"),
+    (2, [(0, 1, 1), (1, 1, 1)], "along axis 0 differ other than between 1 and 2"),
+    (2, [(1, 1, 2), (1, 0, 2)], "along axis 1 differ other than between 1 and 2"),
+])
+def test_batch_channel_boundaries_validated_without_gpu(nbnd, codes, msg):
+    """A volume's per-channel boundary triples: their count, codes, and the rule that the
+    channels' codes along an axis agree or differ only between reflect and antisymmetric
+    (checked before any device call)."""
+    from pushpull import batch as ppb
+
+    L = ppb._sigs()
+    h = ct.c_void_p()
+    v = (ppb.PPVolume * 1)()
+    v[0].nchan, v[0].nbnd = 2, nbnd
+    for q in range(3):
+        v[0].d0[q] = v[0].d1[q] = 4
+        v[0].deg[q] = 1
+        for c, t in enumerate(codes):
+            v[0].chan_bnd[3 * c + q] = t[q]
+    v[0].f = v[0].phi = v[0].out = 16
+    assert L.pp_batch_plan_create(0, ct.cast(v, ct.c_void_p), 1, ct.byref(h)) == -1
+    assert msg in _lib.last_error(), _lib.last_error()
+
+
 def _cpu_vol(op, C=2, d1=(5, 6, 7), d0=(4, 5, 6), **over):
     import torch
 
@@ -236,6 +266,7 @@ def _cpu_vol(op, C=2, d1=(5, 6, 7), d0=(4, 5, 6), **over):
     ("pull", dict(deg=(1, 2)), ValueError, "three degrees"),
     ("pull", dict(deg=(1, 5, 0)), ValueError, "three degrees"),
     ("pull", dict(bnd=(1, 3, 1)), ValueError, "bnd"),
+    ("pull", dict(bnd=[(1, 1, 1)] * 3), ValueError, "3 triples for 2 channels"),
     ("pull", dict(out=np.float32), TypeError, "torch tensor"),
     ("push", dict(dim=(4, 5)), ValueError, "dim"),
     ("push", dict(f="grid"), ValueError, "phi's grid"),

@@ -15,7 +15,7 @@ import re
 import pytest
 
 from pushpull import _lib
-from pushpull.batch import PPVolume
+from pushpull.batch import BATCH_CHAN_BND, PPVolume
 
 JL = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "julia", "PushPullHIP.jl")
 
@@ -87,8 +87,9 @@ def _offsets(layout):
 @pytest.mark.parametrize("jl_name,ctype", [("Desc", _lib.PPDesc), ("Volume", PPVolume)])
 def test_julia_structs_match_c_layout(jl_name, ctype):
     src = _source()
-    consts = {"PP_MAX_CHAN_BND": int(re.search(r"const PP_MAX_CHAN_BND = (\d+)", src).group(1))}
+    consts = {k: int(re.search(rf"const {k} = (\d+)", src).group(1)) for k in ("PP_MAX_CHAN_BND", "PP_BATCH_CHAN_BND")}
     assert consts["PP_MAX_CHAN_BND"] == _lib.PP_MAX_CHAN_BND
+    assert consts["PP_BATCH_CHAN_BND"] == BATCH_CHAN_BND
     offs, size = _offsets(_jl_field_layout(src, jl_name, consts))
     cfields = [(f[0], getattr(ctype, f[0]).offset, getattr(ctype, f[0]).size) for f in ctype._fields_]
     assert [o[0] for o in offs] == [c[0] for c in cfields], "field order differs from the C struct"
