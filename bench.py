@@ -149,12 +149,16 @@ def make_inputs(torch, dev, seed=1):
 
 
 def host_cpu():
-    """The host's processor as lscpu names it, and the threads this process may use."""
+    """The host's processor as lscpu names it, and the threads this process may use. (Under a
+    hypervisor the model name may be generic, e.g. "AMD EPYC"; family, model number and
+    stepping still identify the generation: family 26 is AMD's Zen 5.)"""
     info = {"os_cpu_count": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
         keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
-                "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus"}
+                "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus", "CPU family": "family",
+                "Model": "model_number", "Stepping": "stepping", "Hypervisor vendor": "hypervisor",
+                "L3 cache": "l3"}
         for line in out.splitlines():
             k, _, v = line.partition(":")
             if k.strip() in keys:
