@@ -320,3 +320,15 @@ def test_autograd_refuses_cpu_tensors_that_require_grad(fn):
             pp.push(f, phi, d, s)
         else:
             getattr(pp, fn)(f, phi, s)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU: --gpus above 1 exits at once with a message (no device
+    touched, so this runs without a GPU)."""
+    import json
+    import sys
+
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
