@@ -602,11 +602,12 @@ def cpu_regulariser_baseline(res, budget_s=8.0):
 def bench_matching(torch, pp, iters=50, alpha=0.02, seed=5, check=True):
     """8 subjects at 256^3, 4 channels, degree 3: `iters` iterations of objective and
     gradient (pull, residual, push, pullback) with phi <- phi - alpha * dL/dphi, all
-    subjects batched and each iteration replayed from a HIP graph. Host copies of subject
-    0's inputs and dL/dphi on a 256x256x8 slab at iterations 1 and `iters` are returned
-    under "_fp64_pending" for the CPU leg, where the FP64 oracle re-evaluates dL/dphi from
-    the GPU's own inputs (fp64_checks)."""
-    from pushpull import _lib
+    subjects batched and each iteration replayed from a HIP graph; the descent step is fused
+    into the gradient pass (each volume's phi_next is its phi), one launch per iteration
+    beside the zero fill of dL/df. Host copies of subject 0's inputs and dL/dphi on a
+    256x256x8 slab at iterations 1 and `iters` are returned under "_fp64_pending" for the
+    CPU leg, where the FP64 oracle re-evaluates dL/dphi from the GPU's own inputs
+    (fp64_checks)."""
     from pushpull import batch as ppb
 
     dev = torch.device("cuda:0")
@@ -615,15 +616,12 @@ def bench_matching(torch, pp, iters=50, alpha=0.02, seed=5, check=True):
     dphi = torch.empty_like(phi)
     df = torch.empty_like(f)
     loss = torch.zeros(B, dtype=torch.float64, device=dev)
-    vols = [dict(f=f[b], phi=phi[b], aux=g[b], deg=3, bnd=(1, 1, 1), ext=1, out=dphi[b], out2=df[b], loss=loss[b])
-            for b in range(B)]
+    vols = [dict(f=f[b], phi=phi[b], aux=g[b], deg=3, bnd=(1, 1, 1), ext=1, out=dphi[b], out2=df[b], loss=loss[b],
+                 phi_next=phi[b], step=-alpha) for b in range(B)]
     plan = ppb.BatchPlan("objective", vols)
-    L = _lib.lib()
 
     def iteration(s):
         plan.run(s)
-        _lib.check(L.pp_axpy(ct.c_void_p(phi.data_ptr()), ct.c_void_p(dphi.data_ptr()), -alpha, phi.numel(),
-                             ct.c_void_p(s.cuda_stream)), "axpy")
 
     graph = ppb.Graph(iteration)
     st = graph.stream

@@ -277,6 +277,13 @@ typedef struct pp_volume {
    * other mixes take single calls. */
   int32_t nbnd;
   int32_t chan_bnd[3 * PP_BATCH_CHAN_BND];
+  /* OBJECTIVE only, optional (no counterpart in the reference: a registration loop's descent
+   * step, phi <- phi - alpha dL/dphi, fused into the gradient pass): when set, phi_next
+   * receives fmaf(step, dphi, phi) voxel by voxel, rounded as pp_axpy(phi_next, dphi, step)
+   * would; it may be phi itself (each voxel's phi is read only by the tile computing its
+   * dphi). Must be NULL for the other ops. */
+  float *phi_next;
+  float step;
 } pp_volume;
 
 typedef struct pp_batch_plan pp_batch_plan;

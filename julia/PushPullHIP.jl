@@ -357,16 +357,21 @@ struct Volume
     bnd::NTuple{3,Int32}
     nbnd::Int32
     chan_bnd::NTuple{3 * PP_BATCH_CHAN_BND,Int32}
+    phi_next::Ptr{Float32}
+    step::Float32
 end
 
 """
     Volume(f, ϕ; deg, bnd=(1,1,1), ext=1, aux, out, out2, loss, d₀)
 
 One volume of a batch (`deg` one degree or a degree per axis; `bnd` one triple, or a vector
-of triples, one per channel); the DArrays must outlive every run of the plan.
+of triples, one per channel); the DArrays must outlive every run of the plan. For the
+objective, `ϕnext` (may be ϕ itself) receives ϕ + `step`·∂L/∂ϕ, a descent step fused into
+the gradient pass.
 """
 function Volume(f::DArray, ϕ::DArray; deg::Union{Integer,NTuple{3,<:Integer}}, bnd=(1, 1, 1), ext::Integer=1,
-                aux=nothing, out::DArray, out2=nothing, loss::Ptr{Float64}=Ptr{Float64}(C_NULL), d₀=spatial(f))
+                aux=nothing, out::DArray, out2=nothing, loss::Ptr{Float64}=Ptr{Float64}(C_NULL), d₀=spatial(f),
+                ϕnext=nothing, step::Real=0)
     p(a) = a === nothing ? Ptr{Float32}(C_NULL) : a.ptr
     C = size(f, 4)
     per = bnd isa AbstractVector
@@ -377,7 +382,7 @@ function Volume(f::DArray, ϕ::DArray; deg::Union{Integer,NTuple{3,<:Integer}}, 
     end
     Volume(f.ptr, ϕ.ptr, p(aux), out.ptr, p(out2), loss, Int64.(Tuple(d₀)), Int64.(spatial(ϕ)),
            Int32(C), Int32.(deg isa Integer ? (deg, deg, deg) : deg), Int32(ext),
-           Int32.(Tuple(per ? bnd[1] : bnd)), Int32(per ? C : 0), Tuple(codes))
+           Int32.(Tuple(per ? bnd[1] : bnd)), Int32(per ? C : 0), Tuple(codes), p(ϕnext), Float32(step))
 end
 
 mutable struct BatchPlan

@@ -901,8 +901,10 @@ int build_batch(int op, const pp_volume* vols, int nvol, pp::BatchVol* hv, pp_ba
       return fail(PP_EINVAL, "volume %d: null aux", i);
     if (op == PP_OP_OBJECTIVE && (!v.out2 || !v.loss)) return fail(PP_EINVAL, "volume %d: null out2/loss", i);
     if (op == PP_OP_PULL_VJP && !v.out2) return fail(PP_EINVAL, "volume %d: null out2", i);
+    if (v.phi_next && op != PP_OP_OBJECTIVE)
+      return fail(PP_EINVAL, "volume %d: phi_next (a fused descent step) is only for PP_OP_OBJECTIVE", i);
     for (const Arr& m : {Arr{v.f, "f"}, Arr{v.phi, "phi"}, Arr{v.aux, "aux"}, Arr{v.out, "out"}, Arr{v.out2, "out2"},
-                         Arr{v.loss, "loss"}}) {
+                         Arr{v.loss, "loss"}, Arr{v.phi_next, "phi_next"}}) {
       if (int r = check_mem({m})) {
         char why[sizeof t_err];
         std::snprintf(why, sizeof why, "%s", t_err);
@@ -950,6 +952,8 @@ int build_batch(int op, const pp_volume* vols, int nvol, pp::BatchVol* hv, pp_ba
         a.out2 = v.out2;
         a.out2_cs = n0;
         a.red = op == PP_OP_OBJECTIVE ? v.loss : nullptr;
+        a.phi_next = op == PP_OP_OBJECTIVE ? v.phi_next : nullptr;
+        a.step = v.step;
         bv.zero_n = n0 * v.nchan;
         break;
     }

@@ -313,7 +313,7 @@ __global__ void k_fill(float* p, long long n, float v) {
 __global__ void k_axpy(float* __restrict__ y, const float* __restrict__ x, float a, long long n) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long stride = (long long)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) y[i] += a * x[i];
+  for (; i < n; i += stride) y[i] = fmaf(a, x[i], y[i]);  // (the fused objective step rounds alike)
 }
 
 __global__ void k_fill_d(double* p, long long n, double v) {

@@ -82,6 +82,11 @@ struct KArgs {
   // axis (channel + chan0) (pushing the identity, src/multigrid.jl:270). With out2 set, the
   // tiled push also scatters ones into out2 as one more channel (index nchan): the count.
   int vmode;
+  // Objective (tiled): when set, phi_next[v] = fmaf(step, dphi[v], phi[v]) for every phi
+  // voxel v of the tile once its dphi is final (a descent step fused into the gradient
+  // pass; may be phi itself, since each voxel's phi is read only by the tile that owns it).
+  PP_GLOBAL float* phi_next;
+  float step;
 };
 
 // Boundary code of channel c along `axis` for a launch folding with `code`.

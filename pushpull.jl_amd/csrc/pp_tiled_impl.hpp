@@ -1306,12 +1306,29 @@ __device__ __forceinline__ void gather_tile(const KArgs& a, int tile, int bb, in
     }
   }
   if (RED) {
+    const bool step = KIND == 2 && a.phi_next;
 #pragma unroll
     for (int q = 0; q < VPT; ++q) {
       if (!((V.inside >> q) & 1u)) continue;
       const int v = V.v[q];
-      if ((V.valid >> q) & 1u) put3(a.out + bb * a.out_bs + v, n1, gsum[0][q], gsum[1][q], gsum[2][q], a.accum);
-      else put3(a.out + bb * a.out_bs + v, n1, nan, nan, nan, 0);
+      const bool ok = (V.valid >> q) & 1u;
+      float* o = a.out + bb * a.out_bs + v;
+      float g0 = ok ? gsum[0][q] : nan, g1 = ok ? gsum[1][q] : nan, g2 = ok ? gsum[2][q] : nan;
+      if (ok && a.accum) {
+        g0 += o[0];
+        g1 += o[n1];
+        g2 += o[2 * n1];
+      }
+      o[0] = g0;
+      o[n1] = g1;
+      o[2 * n1] = g2;
+      if (step) {  // phi + step * dL/dphi, as pp_axpy rounds it
+        float* pn = a.phi_next + bb * a.phi_bs + v;
+        const float p0 = phi[v], p1 = phi[v + n1], p2 = phi[v + 2 * n1];
+        pn[0] = fmaf(a.step, g0, p0);
+        pn[n1] = fmaf(a.step, g1, p1);
+        pn[2 * n1] = fmaf(a.step, g2, p2);
+      }
     }
   }
   if (KIND == 2) {

@@ -15,7 +15,14 @@ Per volume (C-order shapes, as everywhere in this package):
   deg  degree 0..4, one for every axis or a triple (X, Y, Z) as Settings takes
   bnd  a boundary triple for all channels, or a list of C triples, one per channel (at
        most 10; along an axis the channels' codes equal or all in {1, 2})
-  ext
+  ext  extrapolation flag (default 1)
+  dim  push, push_grad: the (Z, Y, X) output grid
+  out, out2, loss
+       outputs (allocated when absent): out = pull / push / dphi; out2 = dL/df (objective)
+       or push(aux) (pull_vjp); loss = the objective's 0.5 sum r^2 (float64)
+  phi_next, step
+       objective only: phi_next (phi's shape; may be phi itself) receives
+       phi + step * dL/dphi, a descent step fused into the gradient pass
 """
 from __future__ import annotations
 
@@ -36,6 +43,7 @@ class PPVolume(ct.Structure):
         ("d0", ct.c_int64 * 3), ("d1", ct.c_int64 * 3),
         ("nchan", ct.c_int32), ("deg", ct.c_int32 * 3), ("ext", ct.c_int32), ("bnd", ct.c_int32 * 3),
         ("nbnd", ct.c_int32), ("chan_bnd", ct.c_int32 * (3 * BATCH_CHAN_BND)),
+        ("phi_next", ct.c_void_p), ("step", ct.c_float),
     ]
 
 
@@ -101,6 +109,7 @@ def _volume_records(op: str, volumes: list[dict]):
     for i, v in enumerate(volumes):
         f, phi, aux = _shape3(v, "f"), _shape3(v, "phi"), _shape3(v, "aux")
         out, out2, loss = _shape3(v, "out"), _shape3(v, "out2"), _shape3(v, "loss")
+        phi_next = _shape3(v, "phi_next")
         where = f"volume {i}"
         if f is None or phi is None:
             raise ValueError(f"{where}: `f` and `phi` are required")
@@ -140,19 +149,25 @@ def _volume_records(op: str, volumes: list[dict]):
             raise ValueError(f"{where}: `out2` must be {(C,) + d0}, got {tuple(out2.shape)}")
         if op == "objective" and loss is not None and tuple(loss.shape) not in ((), (1,)):
             raise ValueError(f"{where}: `loss` must hold one value, got shape {tuple(loss.shape)}")
+        if phi_next is not None:
+            if op != "objective":
+                raise ValueError(f"{where}: `phi_next` (a fused descent step) is only for the objective")
+            if tuple(phi_next.shape) != tuple(phi.shape):
+                raise ValueError(f"{where}: `phi_next` must be {tuple(phi.shape)}, got {tuple(phi_next.shape)}")
         # dtypes
         for key, t, dt in (("f", f, torch.float32), ("phi", phi, torch.float32), ("aux", aux, torch.float32),
                            ("out", out, torch.float32), ("out2", out2, torch.float32),
-                           ("loss", loss, torch.float64)):
+                           ("loss", loss, torch.float64), ("phi_next", phi_next, torch.float32)):
             if t is not None and t.dtype != dt:
                 raise TypeError(f"{where}: `{key}` must be {dt}, got {t.dtype}")
         # devices and layout
         if not f.is_cuda:
             raise TypeError(f"{where}: batched volumes are device tensors")
-        for key, t in (("phi", phi), ("aux", aux), ("out", out), ("out2", out2), ("loss", loss)):
+        for key, t in (("phi", phi), ("aux", aux), ("out", out), ("out2", out2), ("loss", loss),
+                       ("phi_next", phi_next)):
             if t is not None and t.device != f.device:
                 raise ValueError(f"{where}: `{key}` is on {t.device}, `f` on {f.device}")
-        for key, t in (("out", out), ("out2", out2), ("loss", loss)):
+        for key, t in (("out", out), ("out2", out2), ("loss", loss), ("phi_next", phi_next)):
             if t is not None and not t.is_contiguous():
                 raise ValueError(f"{where}: `{key}` must be contiguous (it is written in place)")
         f, phi = f.contiguous(), phi.contiguous()
@@ -167,10 +182,11 @@ def _volume_records(op: str, volumes: list[dict]):
             out2 = None
         if op != "objective":
             loss = None
-        rec = dict(f=f, phi=phi, aux=aux, out=out, out2=out2, loss=loss)
+        rec = dict(f=f, phi=phi, aux=aux, out=out, out2=out2, loss=loss, phi_next=phi_next)
         recs.append(rec)
         s = arr[i]
         s.f, s.phi, s.aux, s.out, s.out2, s.loss = (_p(rec[k]) for k in ("f", "phi", "aux", "out", "out2", "loss"))
+        s.phi_next, s.step = _p(phi_next), float(v.get("step", 0.0))
         for q in range(3):
             s.d0[q] = int(d0[2 - q])
             s.d1[q] = int(d1[2 - q])

@@ -348,3 +348,64 @@ def test_batch_channel_boundaries_match_per_volume_calls(op):
         for got, what in ((o, "plan"), (st, "static")):
             _check(H(got["out"]), H(dphi[0]), f"chan-bnd {op} {what} dphi vol {i}")
             _check(H(got["out2"]), H(df[0]), f"chan-bnd {op} {what} df vol {i}")
+
+
+@pytest.mark.parametrize("vol_set", ["isotropic", "per-axis"])
+def test_batch_objective_fused_step_equals_axpy(vol_set):
+    """The objective's fused descent step (phi_next = phi + step * dL/dphi) against the
+    objective followed by pp_axpy, bit for bit: into a separate buffer, in place, and over
+    three graph-replayed iterations of a matching loop updating phi in place."""
+    import ctypes as ct
+
+    from pushpull import _lib
+
+    vols = _volumes(21) if vol_set == "isotropic" else _aniso_volumes(22)
+    step = -0.03
+    L = _lib.lib()
+
+    def same(a, b):  # identical values, NaN where the other has NaN (invalid samples, ext=0)
+        na, nb = torch.isnan(a), torch.isnan(b)
+        return torch.equal(na, nb) and torch.equal(a[~na], b[~nb])
+
+    def axpy(y, x, a):
+        _lib.check(L.pp_axpy(ct.c_void_p(y.data_ptr()), ct.c_void_p(x.data_ptr()), a, y.numel(),
+                             ct.c_void_p(torch.cuda.current_stream().cuda_stream)), "axpy")
+
+    # reference: the objective, then phi + step * dphi through pp_axpy
+    ref_vols = [_dev(v) for v in vols]
+    ref_outs = ppb.BatchPlan("objective", ref_vols).run()
+    expect = [d["phi"].clone() for d in ref_vols]
+    for e, o in zip(expect, ref_outs):
+        axpy(e, o["out"], step)
+    # into a separate buffer
+    dev = [_dev(v) for v in vols]
+    nxt = [torch.full_like(d["phi"], 7.0) for d in dev]
+    for d, n in zip(dev, nxt):
+        d.update(phi_next=n, step=step)
+    outs = ppb.BatchPlan("objective", dev).run()
+    torch.cuda.synchronize()
+    for i, (n, e, o, r) in enumerate(zip(nxt, expect, outs, ref_outs)):
+        assert same(o["out"], r["out"]), i
+        assert same(n, e), f"vol {i}: fused step differs from pp_axpy"
+    # in place, replayed: three iterations against objective + axpy iterations
+    dev = [_dev(v) for v in vols]
+    for d in dev:
+        d.update(phi_next=d["phi"], step=step)
+    plan = ppb.BatchPlan("objective", dev)
+    ref_plan = ppb.BatchPlan("objective", ref_vols)
+    ref_phi = [d["phi"] for d in ref_vols]
+    for d, r in zip(ref_vols, dev):  # both start from the same deformation
+        d["phi"].copy_(r["phi"])
+    graph = ppb.Graph(lambda s: plan.run(s))
+    for it in range(3):
+        graph.launch()
+        graph.stream.synchronize()
+        r_outs = ref_plan.run()
+        for p, o in zip(ref_phi, r_outs):
+            axpy(p, o["out"], step)
+        torch.cuda.synchronize()
+        for i, (d, p) in enumerate(zip(dev, ref_phi)):
+            assert same(d["phi"], p), f"iteration {it} vol {i}: in-place step differs"
+    graph.close()
+    plan.close()
+    ref_plan.close()

@@ -218,6 +218,10 @@ def test_batch_plan_validation_without_gpu():
     assert L.pp_batch_plan_create(0, ct.cast(v, ct.c_void_p), 1, ct.byref(h)) == -1
     assert "degree 7 outside 0..4 on axis 1" in _lib.last_error()
     assert L.pp_batch_plan_create(9, ct.cast(v, ct.c_void_p), 1, ct.byref(h)) == -1
+    v[0].deg[1] = 1
+    v[0].phi_next = 16  # a fused descent step on an op other than the objective
+    assert L.pp_batch_plan_create(0, ct.cast(v, ct.c_void_p), 1, ct.byref(h)) == -1
+    assert "phi_next (a fused descent step) is only for PP_OP_OBJECTIVE" in _lib.last_error()
 
 
 @pytest.mark.parametrize("nbnd,codes,msg", [
@@ -267,6 +271,8 @@ def _cpu_vol(op, C=2, d1=(5, 6, 7), d0=(4, 5, 6), **over):
     ("pull", dict(deg=(1, 5, 0)), ValueError, "three degrees"),
     ("pull", dict(bnd=(1, 3, 1)), ValueError, "bnd"),
     ("pull", dict(bnd=[(1, 1, 1)] * 3), ValueError, "3 triples for 2 channels"),
+    ("pull", dict(phi_next="phi"), ValueError, "only for the objective"),
+    ("objective", dict(phi_next="small"), ValueError, "`phi_next` must be"),
     ("pull", dict(out=np.float32), TypeError, "torch tensor"),
     ("push", dict(dim=(4, 5)), ValueError, "dim"),
     ("push", dict(f="grid"), ValueError, "phi's grid"),
@@ -294,6 +300,8 @@ def test_batch_volume_validation(op, over, exc, msg):
         "shape": lambda k: torch.zeros(2, dtype=torch.float64),
         "f32": lambda k: torch.zeros((), dtype=torch.float32),
         "f64": lambda k: torch.zeros((C,) + d0, dtype=torch.float64),
+        "phi": lambda k: torch.zeros((3,) + d1),
+        "small": lambda k: torch.zeros((3,) + d0),
     }
     over = {k: (subs[v](k) if isinstance(v, str) and v in subs else v) for k, v in over.items()}
     if "out" in over and over["out"] is np.float32:
