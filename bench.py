@@ -263,6 +263,19 @@ def _time_ms(torch, fn, reps, stream):
     return e0.elapsed_time(e1) / reps
 
 
+def _warm(torch, fn, target_ms=100.0):
+    """Run fn back to back for about target_ms of GPU time before it is timed: an idle GPU's
+    clocks need some 30 ms of load to come back, and a short timing right after host-side
+    work (input generation, a collector pass) would run partly at the lower clocks."""
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    fn()
+    e1.record()
+    e1.synchronize()
+    for _ in range(min(2000, int(target_ms / max(e0.elapsed_time(e1), 0.01)) + 1)):
+        fn()
+
+
 def _time_ms_median(torch, fn, reps):
     """Median ms of fn() over reps calls, one pair of HIP events around each call on the
     current stream: a host stall between launches (which shows as idle GPU time inside
@@ -429,7 +442,7 @@ def bench_exp(torch, pp, reps=5, seed=6, nsq=8):
     def run():
         pp._lib.check(L.pp_exp(P(out), P(v), P(work), dims, 1, nsq, st), "pp_exp")
 
-    run()
+    _warm(torch, run)
     ms = _time_ms(torch, run, reps, None)
     n = N ** 3
     per = ms / nsq
@@ -474,6 +487,7 @@ def bench_invert_def(torch, pp, reps=5, seed=7):
 
     fused()
     separate()
+    _warm(torch, fused)
     ms_f = _time_ms(torch, fused, reps, None)
     ms_s = _time_ms(torch, separate, reps, None)
     n = N ** 3
@@ -515,7 +529,7 @@ def bench_affine(torch, pp, reps=5, seed=9):
         def push():
             pp._lib.check(L.pp_affine_push(ct.byref(dsc), P(f0), P(f1), ha, 1, st), "pp_affine_push")
 
-        pull()
+        _warm(torch, pull, 50.0)
         push()
         ms_pull = _time_ms(torch, pull, reps, None)
         ms_push = _time_ms(torch, push, reps, None)

@@ -329,9 +329,9 @@ int pp_device_sync(void);
 /* ---- kernel selection (no counterpart in the reference) -------------------- */
 enum {
   PP_OPT_TILING = 1,     /* 1 (default): automatic -- the autotuner's choice where it has one,
-                            else one work-item per sample for the trilinear gathers (pull,
-                            pull_grad, pull_hess and their pullbacks) and LDS-tiled kernels for
-                            the rest; 2: LDS-tiled kernels wherever they exist (pull, pull_grad,
+                            else LDS-tiled kernels (one work-item per sample for degree 0 and
+                            for trilinear gathers over a one-row grid, as Tier A passes);
+                            2: LDS-tiled kernels wherever they exist (pull, pull_grad,
                             pull_hess, push, push_grad, the pullbacks, the pull VJP, the fused
                             objective; degrees 1-4, a degree per axis running on the 16x8x8
                             kernels of the largest); 0: one work-item per sample */

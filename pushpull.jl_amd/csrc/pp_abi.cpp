@@ -124,11 +124,15 @@ hipError_t launch(pp::Op op, const pp::KArgs& a, int nbatch, hipStream_t s) {
     auto it = g_tune.find({(int)op, dp_key(a.dpx, a.dpy, a.dpz), size_class(a.n1)});
     if (it != g_tune.end()) return launch_variant(op, a, nbatch, s, it->second.variant, it->second.cap);
   }
-  // Built-in choice: trilinear gathers (8 taps, served by L1/L2) run per sample, which
-  // beats staging a box at that degree on gfx950; everything else runs tiled.
+  // Built-in choice: the tiled kernels (at 256^3 the trilinear gathers run 10-25% faster
+  // tiled than per sample, three channels with sliding boundaries twice as fast); a
+  // trilinear gather over a flat list of samples (a one-row grid, as Tier A passes: a tile
+  // would hold a single row of 16 samples) runs per sample.
   const bool trilinear = a.dpx == 2 && a.dpy == 2 && a.dpz == 2;
-  if (trilinear && (op == pp::OP_PULL || op == pp::OP_PULLG || op == pp::OP_PULLH || op == pp::OP_PULL_BWD ||
-                    op == pp::OP_PULLG_BWD))
+  const bool flat = a.d1y == 1 && a.d1z == 1;
+  if (trilinear && flat &&
+      (op == pp::OP_PULL || op == pp::OP_PULLG || op == pp::OP_PULLH || op == pp::OP_PULL_BWD ||
+       op == pp::OP_PULLG_BWD))
     return launch_variant(op, a, nbatch, s, 0);
   return launch_variant(op, a, nbatch, s, tiled);
 }
@@ -227,7 +231,10 @@ int run(pp::Op op, const pp_desc* d, const Geo& g, pp::KArgs a, hipStream_t s) {
   int base[3];
   unsigned mask;
   const bool iso_tiled = a.dpx == a.dpy && a.dpx == a.dpz && a.dpx >= 2 && a.dpx <= 5;
-  if (iso_tiled && g_tiling.load(std::memory_order_relaxed) != 0 && sign_split(d, base, mask)) {
+  const int tiling = g_tiling.load(std::memory_order_relaxed);
+  // per-sample kernels where they are forced: only the trilinear pull has a signed one
+  const bool signed_ps = tiling == 0 && op == pp::OP_PULL && a.dpx == 2 && iso_tiled;
+  if (((iso_tiled && tiling != 0) || signed_ps) && sign_split(d, base, mask)) {
     pp::KArgs b = a;
     b.bx = base[0];
     b.by = base[1];
@@ -235,9 +242,9 @@ int run(pp::Op op, const pp_desc* d, const Geo& g, pp::KArgs a, hipStream_t s) {
     b.csign_on = 1;
     b.csign = mask;
     b.nchan = d->nchan;
-    // the per-sample pull (trilinear: faster there) and the tiled kernels take the signs; an
-    // op the tiled kernels decline runs once per channel below
-    hipError_t e = op == pp::OP_PULL && a.dpx == 2
+    // the tiled kernels and the per-sample trilinear pull take the signs; an op the tiled
+    // kernels decline runs once per channel below
+    hipError_t e = signed_ps
                        ? pp::launch_op(op, b, d->nbatch, s, kBlock)
                        : pp::launch_tiled(op, b, d->nbatch, s,
                                           pp::TileCfg{g_shape.load(std::memory_order_relaxed),

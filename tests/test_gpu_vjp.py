@@ -109,12 +109,13 @@ def test_autograd_pull_uses_vjp_and_matches_oracle():
     check_close(H(tp.grad), oracle.backward_phi(f, phi, delta, so), "autograd dphi")
 
 
-@pytest.mark.parametrize("tiling", [1, 2])
+@pytest.mark.parametrize("tiling", [0, 1, 2])
 @pytest.mark.parametrize("deg", [1, 2])
 def test_pull_channels_differing_only_in_sign(tiling, deg):
     """Channels whose boundary codes differ only between reflect (1) and antisymmetric (2)
-    run as one launch with per-channel signs: the per-sample signed pull (trilinear, automatic
-    choice) and the tiled kernels, against the FP64 oracle, with out-of-FOV samples folding."""
+    run as one launch with per-channel signs: the per-sample signed pull (trilinear, where
+    per-sample kernels are forced) and the tiled kernels (the automatic choice), against the
+    FP64 oracle, with out-of-FOV samples folding."""
     old = pp._lib.get_option(pp._lib.PP_OPT_TILING)
     pp._lib.set_option(pp._lib.PP_OPT_TILING, tiling)
     try:
