@@ -163,6 +163,15 @@ int pp_pull_grad_backward(const pp_desc *d, float *dphi, const float *phi, const
  *   0.5 * sum of r^2 over finite r (zeroed first unless accumulate). */
 int pp_objective_grad(const pp_desc *d, float *dphi, float *df0, double *loss, const float *phi,
                       const float *f0, const float *g, int accumulate, void *stream);
+/* pp_objective_grad followed by the descent step phi_next = fmaf(step, dphi, phi), voxel by
+ * voxel (no counterpart in the reference: a registration loop's update fused into the
+ * gradient pass). phi_next may be phi itself (the update in place); it must not overlap dphi.
+ * Rounded exactly as pp_axpy(phi_next, dphi, step) after a copy of phi would round it. One
+ * launch where the channels share their boundary codes (or differ only between reflect and
+ * antisymmetric, at one degree) and the tiled kernels run; else the objective, then the step. */
+int pp_objective_grad_step(const pp_desc *d, float *dphi, float *df0, double *loss, const float *phi,
+                           const float *f0, const float *g, float *phi_next, float step, int accumulate,
+                           void *stream);
 
 /* Both cotangents of pull in one pass (the rrule of pull, src/pushpull_pb.jl:11-17, whose
  * two thunks call push and pull_grad separately; the reference has no fused entry point):

@@ -602,6 +602,68 @@ int pp_objective_grad(const pp_desc* d, float* dphi, float* df0, double* loss, c
   return run(pp::OP_OBJGRAD, d, g, a, S(s));
 }
 
+int pp_objective_grad_step(const pp_desc* d, float* dphi, float* df0, double* loss, const float* phi,
+                           const float* f0, const float* gt, float* phi_next, float step, int accumulate, void* s) {
+  Geo g;
+  if (int r = check_desc(d, g)) return r;
+  if (!phi_next) return fail(PP_EINVAL, "null phi_next");
+  if (int r = check_mem({{phi_next, "phi_next"}})) return r;
+  const long long n = 3 * g.n1 * d->nbatch;
+  auto ovl = [n](const float* p, const float* q) { return q && p < q + n && q < p + n; };
+  if (phi_next != phi && ovl(phi_next, phi)) return fail(PP_EINVAL, "phi_next overlaps phi without being it");
+  if (ovl(phi_next, dphi)) return fail(PP_EINVAL, "phi_next must not overlap dphi");
+  // One tiled launch covering every channel takes the step once a tile's dphi is final:
+  // shared boundary codes, or codes that differ only in sign.
+  int base[3];
+  unsigned mask = 0u;
+  const bool uni_b = uniform_bnd(d);
+  const bool tiled_ok = g_tiling.load(std::memory_order_relaxed) != 0 &&
+                        (uni_b || (d->deg[0] == d->deg[1] && d->deg[0] == d->deg[2] && sign_split(d, base, mask)));
+  if (tiled_ok) {
+    if (!dphi || !df0 || !loss || !phi || !f0 || !gt) return fail(PP_EINVAL, "null array");
+    if (int r = check_mem({{dphi, "dphi"}, {df0, "df0"}, {loss, "loss"}, {phi, "phi"}, {f0, "f0"}, {gt, "g"}}))
+      return r;
+    pp::KArgs a;
+    base_args(d, a, g);
+    a.f = f0;
+    a.f_cs = g.n0;
+    a.f_bs = g.n0 * d->nchan;
+    a.phi = phi;
+    a.aux = gt;
+    a.aux_cs = g.n1;
+    a.aux_bs = g.n1 * d->nchan;
+    a.out = dphi;
+    a.out_bs = 3 * g.n1;
+    a.out2 = df0;
+    a.out2_cs = g.n0;
+    a.out2_bs = g.n0 * d->nchan;
+    a.red = loss;
+    a.nchan = d->nchan;
+    a.bx = uni_b ? d->bnd[0] : base[0];
+    a.by = uni_b ? d->bnd[1] : base[1];
+    a.bz = uni_b ? d->bnd[2] : base[2];
+    a.csign_on = uni_b ? 0 : 1;
+    a.csign = uni_b ? 0u : mask;
+    a.phi_next = phi_next;
+    a.step = step;
+    hipError_t e = hipSuccess;
+    if (!accumulate) {
+      e = pp::launch_fill(df0, g.n0 * d->nchan * d->nbatch, 0.0f, S(s));
+      if (e == hipSuccess) e = pp::launch_fill_d(loss, 1, 0.0, S(s));
+    }
+    if (e == hipSuccess)
+      e = pp::launch_tiled(pp::OP_OBJGRAD, a, d->nbatch, S(s),
+                           pp::TileCfg{g_shape.load(std::memory_order_relaxed), g_cap.load(std::memory_order_relaxed)});
+    if (e != hipErrorNotSupported) return e == hipSuccess ? ok() : hip_fail(e, "kernel launch");
+    // (degree 0 on every axis: no tiled kernel; the zero fills above are repeated below)
+  }
+  if (int r = pp_objective_grad(d, dphi, df0, loss, phi, f0, gt, accumulate, s)) return r;
+  hipError_t e = hipSuccess;
+  if (phi_next != phi) e = hipMemcpyAsync(phi_next, phi, sizeof(float) * n, hipMemcpyDeviceToDevice, S(s));
+  if (e == hipSuccess) e = pp::launch_axpy(phi_next, dphi, step, n, S(s));
+  return e == hipSuccess ? ok() : hip_fail(e, "descent step");
+}
+
 int pp_pull_vjp(const pp_desc* d, float* dphi, float* df0, const float* phi, const float* f0, const float* delta,
                 int accumulate, void* s) {
   Geo g;

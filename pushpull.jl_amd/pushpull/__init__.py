@@ -8,6 +8,7 @@ there is no CPU fallback.
 from .settings import Settings, id  # noqa: A001
 from .autograd import pull, pull_grad, push
 from .ops import push_grad, pull_hess, pull_backward, pull_grad_backward, pull_vjp, objective_grad
+from .ops import objective_grad_step
 from .affine import affine_pull, affine_push, adjust_affine
 from .compositions import compose, compose_, Exp
 from .multigrid import push_count, affine_push_count, invert_def_rhs, restrict, prolong
@@ -16,7 +17,7 @@ from . import _lib
 
 __all__ = [
     "Settings", "id", "pull", "push", "pull_grad", "push_grad", "pull_hess",
-    "pull_backward", "pull_grad_backward", "pull_vjp", "objective_grad", "affine_pull", "affine_push", "adjust_affine",
+    "pull_backward", "pull_grad_backward", "pull_vjp", "objective_grad", "objective_grad_step", "affine_pull", "affine_push", "adjust_affine",
     "compose", "compose_", "Exp", "push_count", "affine_push_count", "invert_def_rhs", "restrict", "prolong",
     "registration_operator", "reduce2fit", "sparsify", "vel2mom", "vel2mom_", "relax_", "SparseKernel",
 ]

@@ -59,6 +59,7 @@ _SIGS = {
     "pp_pull_backward": ([_P, _P, _P, _P, _P, _P], ct.c_int),
     "pp_pull_grad_backward": ([_P, _P, _P, _P, _P, _P], ct.c_int),
     "pp_objective_grad": ([_P, _P, _P, _P, _P, _P, _P, ct.c_int, _P], ct.c_int),
+    "pp_objective_grad_step": ([_P, _P, _P, _P, _P, _P, _P, _P, ct.c_float, ct.c_int, _P], ct.c_int),
     "pp_pull_vjp": ([_P, _P, _P, _P, _P, _P, ct.c_int, _P], ct.c_int),
     "pp_push_count": ([_P, _P, _P, _P, _P, ct.c_int, _P], ct.c_int),
     "pp_affine_push_count": ([_P, _P, _P, _P, _P, ct.c_int, _P], ct.c_int),

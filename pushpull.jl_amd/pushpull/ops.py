@@ -309,3 +309,39 @@ def objective_grad(f0, phi, g, sett=None, df0=None, accumulate=False):
     _lib.check(_lib.lib().pp_objective_grad(ct.byref(d), _ptr(dphi), _ptr(df0), _ptr(loss), _ptr(phi), _ptr(f0),
                                             _ptr(g), int(accumulate), _stream(f0)), "pp_objective_grad")
     return loss, dphi, df0
+
+
+def objective_grad_step(f0, phi, g, step, sett=None, phi_next=None, df0=None, accumulate=False):
+    """objective_grad followed by the descent step phi_next = phi + step * dL/dphi in the
+    same pass where the kernels allow (pp_objective_grad_step); phi_next may be phi itself
+    (updated in place; it must then be a contiguous float32 device tensor). Returns (loss,
+    dphi, df0, phi_next)."""
+    sett = sett or Settings()
+    f0 = _f32(f0, "f₀")
+    phi_in = phi
+    phi = _f32(phi, "ϕ")
+    g = _f32(g, "g")
+    _req(_is_dev(f0) and _is_dev(phi) and _is_dev(g), "objective_grad_step takes device tensors")
+    _same_device(("f₀", f0), ("ϕ", phi), ("g", g))
+    lead, d0 = _dims5(f0.shape, 3, 5, "f₀")
+    nb, d1 = _phi_dims(phi)
+    _req(nb == (f0.shape[0] if f0.ndim == 5 else 1), "`ϕ` & `f₀` must have the same batch size")
+    nc = f0.shape[-4] if f0.ndim >= 4 else 1
+    _req(tuple(g.shape) == lead + d1, "g must have the shape of pull(f0, phi)")
+    if phi_next is None:
+        phi_next = torch.empty_like(phi)
+    elif phi_next is phi_in:
+        _req(phi_in is phi, "an in-place step needs phi as a contiguous float32 device tensor")
+    else:
+        _out_arg(phi_next, phi.shape, phi, "phi_next")
+    dphi = torch.empty(tuple(phi.shape), dtype=torch.float32, device=f0.device)
+    if df0 is None:
+        df0 = torch.empty(tuple(f0.shape), dtype=torch.float32, device=f0.device)
+    else:
+        _out_arg(df0, f0.shape, f0, "df0")
+    loss = torch.empty((), dtype=torch.float64, device=f0.device)
+    d = _desc(sett, d0, d1, nc, nb)
+    _lib.check(_lib.lib().pp_objective_grad_step(ct.byref(d), _ptr(dphi), _ptr(df0), _ptr(loss), _ptr(phi),
+                                                 _ptr(f0), _ptr(g), _ptr(phi_next), float(step), int(accumulate),
+                                                 _stream(f0)), "pp_objective_grad_step")
+    return loss, dphi, df0, phi_next

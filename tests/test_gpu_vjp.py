@@ -172,3 +172,55 @@ def test_every_op_with_channels_differing_only_in_sign(deg):
     assert abs(float(loss) - 0.5 * np.sum(r[np.isfinite(r)] ** 2)) <= 1e-5 * 0.5 * np.sum(r[np.isfinite(r)] ** 2)
     for k, v in e.items():
         report(f"{tag} {k} vs f64", v)
+
+
+@pytest.mark.parametrize("case", ["deg1", "deg3", "sliding", "per-axis", "deg0", "mixed-codes", "per-sample",
+                                  "accumulate"])
+def test_objective_grad_step_equals_objective_then_axpy(case):
+    """pp_objective_grad_step against pp_objective_grad followed by pp_axpy, bit for bit (NaN
+    where dL/dphi is NaN): the fused tiled launch (shared codes, codes differing in sign only,
+    a degree per axis) and the fallback (degree 0, codes that differ otherwise, per-sample
+    kernels forced), into a separate buffer and in place."""
+    import ctypes as ct
+
+    from pushpull import _lib
+
+    rng = np.random.default_rng(700 + len(case))
+    d = (22, 26, 30)
+    f, phi, g = _case(rng, d, 3, 5.0, out_frac=0.03)
+    deg, bnd, ext = {"deg1": (1, 1, 1), "deg3": (3, 1, 0), "sliding": (1, [(2, 1, 1), (1, 2, 1), (1, 1, 2)], 1),
+                     "per-axis": ((1, 3, 0), 1, 1), "deg0": (0, 1, 1), "mixed-codes": (2, [(1, 1, 1), (0, 2, 1), (1, 1, 1)], 1),
+                     "per-sample": (3, 1, 1), "accumulate": (2, 1, 1)}[case]
+    s = pp.Settings(deg, bnd, ext)
+    step = -0.05
+    acc = case == "accumulate"
+    L = _lib.lib()
+    old = pp._lib.get_option(pp._lib.PP_OPT_TILING)
+    if case == "per-sample":
+        pp._lib.set_option(pp._lib.PP_OPT_TILING, 0)
+    try:
+        base = torch.full(f.shape, 0.25, device=DEV)
+        loss_r, dphi_r, df_r = pp.objective_grad(T(f), T(phi), T(g), s, df0=base.clone() if acc else None,
+                                                 accumulate=acc)
+        ref = T(phi)
+        _lib.check(L.pp_axpy(ct.c_void_p(ref.data_ptr()), ct.c_void_p(dphi_r.data_ptr()), step, ref.numel(),
+                             ct.c_void_p(torch.cuda.current_stream().cuda_stream)), "axpy")
+        ph = T(phi)
+        loss, dphi, df, nxt = pp.objective_grad_step(T(f), ph, T(g), step, s, df0=base.clone() if acc else None,
+                                                      accumulate=acc)
+        ph2 = T(phi)
+        _, _, _, same_buf = pp.objective_grad_step(T(f), ph2, T(g), step, s, phi_next=ph2)
+        torch.cuda.synchronize()
+    finally:
+        pp._lib.set_option(pp._lib.PP_OPT_TILING, old)
+
+    def same(a, b):
+        na, nb = torch.isnan(a), torch.isnan(b)
+        return torch.equal(na, nb) and torch.equal(a[~na], b[~nb])
+
+    assert same(dphi, dphi_r), case
+    assert same(nxt, ref), f"{case}: step into a separate buffer differs from pp_axpy"
+    assert same_buf is ph2 and same(ph2, ref), f"{case}: step in place differs from pp_axpy"
+    check_close(H(df), H(df_r), f"{case} df", scale=1.0)
+    assert abs(float(loss) - float(loss_r)) <= 1e-6 * abs(float(loss_r)), case
+    assert torch.equal(ph, T(phi)), f"{case}: phi changed although phi_next was a separate buffer"
