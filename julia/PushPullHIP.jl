@@ -26,7 +26,7 @@ using Libdl
 
 export Settings, DArray, todevice, tohost, free!, id,
        pull, pull_grad, pull_hess, push, push_grad, affine_pull, affine_push, adjust_affine,
-       pull_backward, pull_grad_backward, pull_vjp, objective_grad,
+       pull_backward, pull_grad_backward, pull_vjp, objective_grad, objective_grad_step!,
        compose, compose!, Exp,
        Volume, BatchPlan, run!, run_static!, destroy!, Graph, launch, capture, drop_in!
 
@@ -308,6 +308,30 @@ function objective_grad(f₀::DArray, ϕ::DArray, g::DArray, sett=Settings(); st
                 (Ref{Desc}, Ptr{Float32}, Ptr{Float32}, Ptr{Float64}, Ptr{Float32}, Ptr{Float32}, Ptr{Float32}, Cint,
                  Ptr{Cvoid}),
                 d, dϕ.ptr, df₀.ptr, Ptr{Float64}(loss.ptr), ϕ.ptr, f₀.ptr, g.ptr, 0, stream))
+    L = Ref{Float64}(0.0)
+    check(ccall(sym(:pp_memcpy_d2h), Cint, (Ref{Float64}, Ptr{Cvoid}, Csize_t, Ptr{Cvoid}), L, loss.ptr, 8, stream))
+    free!(loss)
+    return L[], dϕ, df₀
+end
+
+"""
+    objective_grad_step!(f₀, ϕ, g, step, sett; ϕnext=ϕ)
+
+`objective_grad` with the descent step ϕnext = ϕ + step·∂L/∂ϕ fused into the gradient pass
+(in place by default). Returns (L, ∂L/∂ϕ, ∂L/∂f₀).
+"""
+function objective_grad_step!(f₀::DArray, ϕ::DArray, g::DArray, step::Real, sett=Settings();
+                              ϕnext::DArray=ϕ, stream::Ptr{Cvoid}=C_NULL)
+    check_pull_args(f₀, ϕ, sett)
+    @assert size(g) == (spatial(ϕ)..., size(f₀)[4:end]...) "g must have the shape of pull(f₀, ϕ)"
+    @assert size(ϕnext) == size(ϕ) "`ϕnext` must have the shape of ϕ"
+    df₀, dϕ = DArray(size(f₀)), DArray(size(ϕ))
+    loss = DArray((2,))  # one Float64
+    d = Ref(Desc(sett, spatial(f₀), spatial(ϕ), nchan(f₀), nbatch(f₀)))
+    check(ccall(sym(:pp_objective_grad_step), Cint,
+                (Ref{Desc}, Ptr{Float32}, Ptr{Float32}, Ptr{Float64}, Ptr{Float32}, Ptr{Float32}, Ptr{Float32},
+                 Ptr{Float32}, Cfloat, Cint, Ptr{Cvoid}),
+                d, dϕ.ptr, df₀.ptr, Ptr{Float64}(loss.ptr), ϕ.ptr, f₀.ptr, g.ptr, ϕnext.ptr, Float32(step), 0, stream))
     L = Ref{Float64}(0.0)
     check(ccall(sym(:pp_memcpy_d2h), Cint, (Ref{Float64}, Ptr{Cvoid}, Csize_t, Ptr{Cvoid}), L, loss.ptr, 8, stream))
     free!(loss)
